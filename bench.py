@@ -1,0 +1,189 @@
+"""FoilCFD MI355X benchmark: GLUPS and HBM roofline fraction of the fused
+D3Q19 TRT + Smagorinsky stream-collide on the 23.6 M-cell NACA 2412 wing
+(512 x 192 x 240, chord 256, AoA 8 deg), FP16 shifted storage vs FP32.
+
+One "step" is one lattice time step of the whole grid (the reference's
+LBMSolver::stepN unit). Steps run in calls of 100 (one macro store and one
+momentum-exchange sample per call, like the reference's selftest frame).
+
+Prints ONE JSON line. value = FP16-storage GLUPS (whole job, one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"f16": 76.0, "bf16": 76.0, "f32": 152.0, "f64": 304.0}
+METRIC = "GLUPS & % HBM peak, 23.6M-cell wing, FP16-store vs FP32 at ≤1% Cl/Cd err"
+REFERENCE_PUBLISHED_GLUPS_FP32 = 8.151  # RTX 5090, 23.6M cells, FP32 (notes/PERF.md:10-20)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_storage(case, storage, steps, warmup, per_call, kernel_steps):
+    from foil_cfd_amd import cases
+    s = cases.make_solver(case, storage)
+    n = case.dims[0] * case.dims[1] * case.dims[2]
+    # warmup (not timed)
+    done = 0
+    while done < warmup:
+        k = min(per_call, warmup - done)
+        s.step_n(k)
+        done += k
+    s.synchronize()
+    # timed region: synchronize on both sides, wall clock
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(per_call, steps - done)
+        s.step_n(k)
+        done += k
+    s.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    # per-launch HIP-event timing of the stream-collide kernel on the solver's stream
+    s.set_kernel_timing(True)
+    done = 0
+    while done < kernel_steps:
+        k = min(per_call, kernel_steps - done)
+        s.step_n(k)
+        done += k
+    launches, kms = s.kernel_timing()
+    s.set_kernel_timing(False)
+    nan = s.nan_detected()
+    fr = s.forces()
+    s.close()
+    avg_kernel_ms = kms / max(launches, 1)
+    res = {
+        "glups": n * steps / wall / 1e9,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_ms": avg_kernel_ms,
+        "kernel_launches": launches,
+        "kernel_gbs": BYTES_PER_CELL[storage] * n / (avg_kernel_ms * 1e-3) / 1e9,
+        "nan": bool(nan),
+        "cl_ema": fr.cl,
+        "cd_ema": fr.cd,
+    }
+    log(f"[bench] {storage}: {res['glups']:.2f} GLUPS wall, kernel {avg_kernel_ms:.4f} ms "
+        f"({res['kernel_gbs']:.0f} GB/s algorithmic)")
+    return res
+
+
+def cpu_baseline(case, seconds=12.0):
+    """The oracle's FP32 stream-collide (reference semantics, pull ping-pong)
+    on the host cores, on the same grid, for a bounded number of steps."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    nx, ny, nz = case.dims
+    n = nx * ny * nz
+    threads = orc.threads()
+    f = orc.init_equilibrium(case.flags, case.dims, 0.0)
+    sc = case.scaling
+    # one probe step to size the sample
+    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, nx, 64)
+    t0 = time.perf_counter()
+    orc.run(f, case.flags, case.dims, taus[:1], uins[:1], cs=case.cs, macro=False)
+    probe = time.perf_counter() - t0
+    steps = int(max(1, min(60, seconds / max(probe, 1e-3))))
+    t0 = time.perf_counter()
+    orc.run(f, case.flags, case.dims, taus[1:1 + steps], uins[1:1 + steps], cs=case.cs,
+            macro=False)
+    dt = time.perf_counter() - t0
+    del f
+    glups = n * steps / dt / 1e9
+    log(f"[bench] cpu oracle: {steps} steps in {dt:.1f} s on {threads} threads -> {glups:.4f} GLUPS")
+    return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 pull/ping-pong stream-collide, 512x192x240 wing, {steps} steps "
+                      f"({dt:.1f} s wall)", "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--per-call", type=int, default=100)
+    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--aoa", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storages", default="f16,f32")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (the solver core is single-GPU; "
+              "--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import foil_cfd_amd as fc
+    from foil_cfd_amd import cases
+    fc.load()
+    t_setup = time.perf_counter()
+    case = cases.config3_wing(args.aoa)
+    log(f"[bench] case {case.name}: flags built in {time.perf_counter() - t_setup:.1f} s")
+
+    storages = [s.strip() for s in args.storages.split(",") if s.strip()]
+    results = {}
+    for st in storages:
+        results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call,
+                                  args.kernel_steps)
+    head = "f16" if "f16" in results else storages[0]
+    r = results[head]
+    cpu = None if args.no_cpu else cpu_baseline(case)
+
+    ncells = case.dims[0] * case.dims[1] * case.dims[2]
+    out = {
+        "metric": METRIC,
+        "value": round(r["glups"], 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_{head}store",
+            "cells": ncells,
+            "storage": {"f16": "fp16_shifted", "bf16": "bf16_shifted", "f32": "fp32",
+                        "f64": "fp64"}[head],
+            "collision": "TRT(3/16) + Smagorinsky(0.1733)",
+            "streaming": "in-place AA",
+            "steps_per_call": args.per_call,
+            "not_included": "refinement patch, wall model, particles (coarse grid only)",
+            "parallelism": "single_gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(r["kernel_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r["kernel_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "stepKernel (fused stream-collide)",
+            "kernel_ms": round(r["kernel_ms"], 5),
+            "algorithmic_bytes_per_cell": BYTES_PER_CELL[head],
+        },
+        "cpu_baseline": cpu,
+        "per_storage": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv)
+                            for kk, vv in v.items()} for k, v in results.items()},
+        "reference_published_glups_fp32": REFERENCE_PUBLISHED_GLUPS_FP32,
+    }
+    if "f16" in results and "f32" in results:
+        out["fp16_vs_fp32_speedup"] = round(results["f16"]["glups"] / results["f32"]["glups"], 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+"""ctypes binding of the FoilCFD C ABI (include/foilcfd.h).
+
+The library is the in-tree ``foil_cfd_amd/libfoilcfd.so`` built for gfx950
+(``python -c "import __graft_entry__ as g; g.build()"``). There is no CPU
+fallback: loading fails loudly when the shared library is absent.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfoilcfd.so")
+
+FC_OK = 0
+FC_ERR_INVALID = 1
+FC_ERR_ALLOC = 2
+FC_ERR_DEVICE = 3
+FC_ERR_STATE = 4
+
+FC_STORAGE_F32 = 0
+FC_STORAGE_F16 = 1
+FC_STORAGE_BF16 = 2
+FC_STORAGE_F64 = 3
+
+STORAGE_NAMES = {"f32": FC_STORAGE_F32, "f16": FC_STORAGE_F16, "bf16": FC_STORAGE_BF16,
+                 "f64": FC_STORAGE_F64}
+STORAGE_BYTES = {FC_STORAGE_F32: 4, FC_STORAGE_F16: 2, FC_STORAGE_BF16: 2, FC_STORAGE_F64: 8}
+
+
+class FcDims(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+
+class FcScaling(C.Structure):
+    _fields_ = [("chord_m", C.c_float), ("airspeed_ms", C.c_float), ("nu_m2s", C.c_float),
+                ("chord_cells", C.c_int), ("u_lat", C.c_float), ("dx", C.c_float),
+                ("dt", C.c_float), ("nu_lat_target", C.c_float), ("nu_lat", C.c_float),
+                ("tau", C.c_float), ("tau_clamped", C.c_int), ("re_target", C.c_float),
+                ("re_effective", C.c_float)]
+
+
+class FcLayout(C.Structure):
+    _fields_ = [("dims", FcDims), ("chord_cells", C.c_int), ("anchor_x_frac", C.c_float),
+                ("anchor_y_frac", C.c_float), ("anchor_x_cells", C.c_float),
+                ("anchor_y_cells", C.c_float), ("anchor_absolute", C.c_int)]
+
+
+class FcPatchBox(C.Structure):
+    _fields_ = [("x0", C.c_int), ("y0", C.c_int), ("x1", C.c_int), ("y1", C.c_int)]
+
+
+class FcForceReadout(C.Structure):
+    _fields_ = [("cl", C.c_float), ("cd", C.c_float), ("lift_to_drag", C.c_float),
+                ("valid", C.c_int), ("flow_throughs", C.c_float),
+                ("cl_avg", C.c_float), ("cl_min", C.c_float), ("cl_max", C.c_float),
+                ("cl_median", C.c_float), ("cd_avg", C.c_float), ("cd_min", C.c_float),
+                ("cd_max", C.c_float), ("cd_median", C.c_float), ("ld_avg", C.c_float),
+                ("ld_min", C.c_float), ("ld_max", C.c_float), ("ld_median", C.c_float)]
+
+
+class FcPerfStats(C.Structure):
+    _fields_ = [("last_steps_per_frame", C.c_int), ("last_step_ms", C.c_double),
+                ("mlups", C.c_double), ("population_bytes_per_step", C.c_double)]
+
+
+class FcInitOptions(C.Structure):
+    _fields_ = [("storage", C.c_int), ("periodic_x", C.c_int), ("periodic_y", C.c_int)]
+
+
+_vp = C.c_void_p
+_u8p = C.POINTER(C.c_uint8)
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_llp = C.POINTER(C.c_longlong)
+
+# name -> (restype, argtypes); every symbol include/foilcfd.h declares.
+SIGNATURES = {
+    "fc_version": (C.c_char_p, []),
+    "fc_device_count": (C.c_int, []),
+    "fc_compute_scaling": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, C.c_float,
+                                     C.POINTER(FcScaling)]),
+    "fc_max_effective_re": (C.c_float, [C.c_int, C.c_float]),
+    "fc_refined_scaling": (C.c_int, [C.POINTER(FcScaling), C.c_int, C.POINTER(FcScaling)]),
+    "fc_fine_tau_for": (C.c_float, [C.c_float, C.c_int]),
+    "fc_ramped_tau": (C.c_float, [C.POINTER(FcScaling), C.c_longlong, C.c_int]),
+    "fc_ramped_u": (C.c_float, [C.POINTER(FcScaling), C.c_longlong, C.c_int]),
+    "fc_force_to_coefficient": (C.c_float, [C.POINTER(FcScaling), C.c_float, C.c_int]),
+    "fc_flow_through_steps": (C.c_float, [C.POINTER(FcScaling), C.c_int]),
+    "fc_generate_naca4": (C.c_int, [C.c_char_p, C.c_int, _fp, C.c_int, _ip, C.c_char_p, C.c_int]),
+    "fc_build_boundary_flags": (C.c_int, [FcDims, _u8p]),
+    "fc_voxelize_airfoil": (C.c_int, [_fp, C.c_int, C.c_float, C.POINTER(FcLayout), _u8p]),
+    "fc_close_trailing_edge_gaps": (C.c_int, [FcDims, _u8p]),
+    "fc_build_clean_foil_flags": (C.c_int, [_fp, C.c_int, C.c_float, C.POINTER(FcLayout), _u8p]),
+    "fc_make_fine_layout": (C.c_int, [C.POINTER(FcLayout), C.POINTER(FcPatchBox), C.c_int,
+                                      C.POINTER(FcLayout)]),
+    "fc_fine_dims": (C.c_int, [C.POINTER(FcPatchBox), FcDims, C.c_int, C.POINTER(FcDims)]),
+    "fc_stamp_interface_shell": (C.c_int, [FcDims, _u8p]),
+    "fc_build_fine_patch_flags": (C.c_int, [_fp, C.c_int, C.c_float, C.POINTER(FcLayout),
+                                            C.POINTER(FcPatchBox), C.c_int, _u8p]),
+    "fc_derive_patch_box": (C.c_int, [FcDims, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(FcPatchBox)]),
+    "fc_create": (_vp, []),
+    "fc_destroy": (None, [_vp]),
+    "fc_init": (C.c_int, [_vp, FcDims, C.POINTER(FcScaling), _u8p, C.POINTER(FcInitOptions),
+                          C.c_char_p, C.c_int]),
+    "fc_shutdown": (None, [_vp]),
+    "fc_reset": (C.c_int, [_vp]),
+    "fc_set_flags": (C.c_int, [_vp, _u8p]),
+    "fc_set_startup_ramp": (C.c_int, [_vp, C.c_int]),
+    "fc_set_force_ema_window": (C.c_int, [_vp, C.c_float]),
+    "fc_set_smagorinsky_cs": (C.c_int, [_vp, C.c_float]),
+    "fc_step": (C.c_int, [_vp, C.c_int]),
+    "fc_synchronize": (C.c_int, [_vp]),
+    "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),
+    "fc_force_raw": (C.c_int, [_vp, _dp, _llp]),
+    "fc_force_now": (C.c_int, [_vp, _dp]),
+    "fc_mass": (C.c_int, [_vp, _dp]),
+    "fc_read_macro": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fc_read_flags": (C.c_int, [_vp, _u8p]),
+    "fc_download_f": (C.c_int, [_vp, _vp, C.c_int]),
+    "fc_upload_f": (C.c_int, [_vp, _vp, C.c_int]),
+    "fc_notify_snapshot_restored": (C.c_int, [_vp, C.c_int]),
+    "fc_nan_detected": (C.c_int, [_vp]),
+    "fc_nan_kind": (C.c_int, [_vp]),
+    "fc_get_perf_stats": (C.c_int, [_vp, C.POINTER(FcPerfStats)]),
+    "fc_step_count": (C.c_longlong, [_vp]),
+    "fc_current_tau": (C.c_float, [_vp]),
+    "fc_flow_throughs_completed": (C.c_float, [_vp]),
+    "fc_get_dims": (C.c_int, [_vp, C.POINTER(FcDims)]),
+    "fc_get_scaling": (C.c_int, [_vp, C.POINTER(FcScaling)]),
+    "fc_storage": (C.c_int, [_vp]),
+    "fc_device_fields": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                   C.POINTER(_vp), C.POINTER(_vp)]),
+    "fc_device_bytes": (C.c_longlong, [_vp]),
+    "fc_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
+    "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+}
+
+_LIB = None
+
+
+def load(path: str | None = None) -> C.CDLL:
+    """Load libfoilcfd.so and bind every C-ABI symbol; raises if missing."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"FoilCFD HIP library not built: {p} is missing. Run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` first.")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(rc: int, what: str, msg: str = "") -> None:
+    if rc != FC_OK:
+        raise RuntimeError(f"{what} failed with status {rc}{': ' + msg if msg else ''}")

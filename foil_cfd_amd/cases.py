@@ -1,0 +1,97 @@
+"""Case builders for the FoilCFD workloads (flags, scaling, initial states).
+
+* Taylor-Green vortex box: fully periodic N^3, rho = 1,
+  u = (u0 sin kx cos ky cos kz, -u0 cos kx sin ky cos kz, 0), k = 2 pi / N,
+  populations at local equilibrium (reference tests/m0_taylor_green.cpp:57-80
+  seeds its open-domain variant the same way).
+* Wind tunnel around an extruded NACA 2412: buildCleanFoilFlags with the
+  default fractional anchors (0.3 nx, 0.5 ny), reference voxelizer.h:57-85.
+* The 23.6 M-cell wing: chord 256, 512 x 192 x 240, chord 1.2 m at 30 m/s in
+  air (tau clamped at 0.5005, units.h:192-219).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import solver as S
+
+
+@dataclass
+class Case:
+    name: str
+    dims: tuple
+    flags: np.ndarray
+    scaling: object
+    periodic_xy: bool = False
+    cs: float = 0.1733
+    ramp: bool = True
+    f0: np.ndarray | None = None  # optional canonical initial state (FP64)
+
+
+def equilibrium_field(rho, u, v, w) -> np.ndarray:
+    """Canonical SoA feq (FP64) of per-cell (rho, u, v, w) arrays."""
+    n = rho.size
+    usq = 1.5 * (u * u + v * v + w * w)
+    f = np.empty((S.Q, n), dtype=np.float64)
+    for q in range(S.Q):
+        cu = 3.0 * (S.CX[q] * u + S.CY[q] * v + S.CZ[q] * w)
+        f[q] = S.W[q] * rho * (1.0 + cu + 0.5 * cu * cu - usq)
+    return f.reshape(-1)
+
+
+def tgv_fields(n: int, u0: float = 0.05):
+    k = 2.0 * np.pi / n
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    x = x.reshape(-1).astype(np.float64)
+    y = y.reshape(-1).astype(np.float64)
+    z = z.reshape(-1).astype(np.float64)
+    u = u0 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z)
+    v = -u0 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z)
+    w = np.zeros_like(u)
+    return np.ones_like(u), u, v, w
+
+
+def taylor_green(n: int = 64, u0: float = 0.05, nu: float = 0.01) -> Case:
+    tau = 3.0 * nu + 0.5
+    sc = S.scaling_for_tau(tau, u0, n)
+    rho, u, v, w = tgv_fields(n, u0)
+    return Case(f"tgv{n}", (n, n, n), np.zeros(n ** 3, dtype=np.uint8), sc, periodic_xy=True,
+                cs=0.0, ramp=False, f0=equilibrium_field(rho, u, v, w))
+
+
+def tgv_kinetic_energy(u, v, w) -> float:
+    return 0.5 * float(np.sum(u.astype(np.float64) ** 2 + v.astype(np.float64) ** 2
+                              + w.astype(np.float64) ** 2))
+
+
+def naca_tunnel(nx: int, ny: int, nz: int, chord: int, aoa: float = 8.0, u_lat: float = 0.08,
+                cs: float = 0.1, chord_m: float = 1.2, airspeed: float = 30.0,
+                nu: float = 1.48e-5, digits: str = "2412", ramp: bool = True) -> Case:
+    pts, _ = S.generate_naca4(digits, 200)
+    layout = S.make_layout(nx, ny, nz, chord)
+    flags = S.build_clean_foil_flags(pts, aoa, layout)
+    sc = S.compute_scaling(chord_m, airspeed, nu, chord, u_lat)
+    return Case(f"naca{digits}_{nx}x{ny}x{nz}_c{chord}_a{aoa:g}", (nx, ny, nz), flags, sc,
+                cs=cs, ramp=ramp)
+
+
+def config2_tunnel(aoa: float = 8.0) -> Case:
+    """384 x 128 x 64, chord 128, u_lat 0.08, Cs 0.1."""
+    return naca_tunnel(384, 128, 64, 128, aoa=aoa, u_lat=0.08, cs=0.1)
+
+
+def config3_wing(aoa: float = 8.0) -> Case:
+    """512 x 192 x 240 (23.6 M cells), chord 256, TRT + LES (Cs 0.1733)."""
+    return naca_tunnel(512, 192, 240, 256, aoa=aoa, u_lat=0.08, cs=0.1733)
+
+
+def make_solver(case: Case, storage: str = "f32") -> S.LBMSolver:
+    s = S.LBMSolver()
+    s.set_startup_ramp_enabled(case.ramp)
+    s.init(case.dims, case.scaling, case.flags, storage=storage, periodic_xy=case.periodic_xy)
+    s.set_smagorinsky_cs(case.cs)
+    if case.f0 is not None:
+        s.upload_f(case.f0)
+    return s

@@ -1,0 +1,66 @@
+// Host-callable launchers of the FoilCFD HIP kernels. Plain C++ signatures
+// (raw device pointers, sizes, an opaque stream handle) so the host solver
+// can be compiled by the host compiler alone. Every launcher returns the
+// hipError_t of its launch as an int (0 = success).
+#pragma once
+
+#include <cstdint>
+
+#include "fc_layout.hpp"
+
+namespace fc {
+
+// Collision / boundary parameters of one fused step (the reference's
+// StepParams, src/sim/lbm_core.cuh:233-241, with Cs turned into the
+// precomputed Smagorinsky prefactor 18*sqrt(2)*Cs^2 exactly as the reference
+// launch wrapper forms it, src/sim/lbm_core.cu:785-787).
+struct StepArgs {
+    float tau0 = 0.6f;
+    float magicLambda = 3.0f / 16.0f;
+    float smagPrefactor = 0.0f;
+    float uInlet = 0.08f;
+    int writeMacro = 0;
+};
+
+// Device view of one lattice level (non-owning).
+struct LatticeView {
+    void* f = nullptr;                 // kQ * cells populations, storage format below
+    int storage = kStoreF32;           // Storage enum
+    const std::uint8_t* kind = nullptr;  // cells kind bytes
+    Dims dims;
+    int periodicX = 0, periodicY = 0;  // z is always periodic
+    void* feed = nullptr;              // outlet feed planes [2][kQ][ny*nz], storage format
+    float* rho = nullptr;              // optional macro outputs (cells each)
+    float* u = nullptr;
+    float* v = nullptr;
+    float* w = nullptr;
+    const std::uint16_t* uwx = nullptr;  // optional wall-model slip (half bits, cells each)
+    const std::uint16_t* uwy = nullptr;
+    const std::uint16_t* uwz = nullptr;
+};
+
+int launchInitEquilibrium(const LatticeView& lat, float uInit, void* stream);
+int launchSpanwisePerturb(const LatticeView& lat, float amplitude, float phase1, float phase2,
+                          float phase3, void* stream);
+// Canonical FP32 / FP64 unpadded SoA f[q*cells + cell] <-> device layout.
+int launchUploadCanonical(const LatticeView& lat, const void* canon, int canonIsF64,
+                          void* stream);
+int launchDownloadCanonical(const LatticeView& lat, int layout, void* canon, int canonIsF64,
+                            void* stream);
+// One fused stream-collide step reading layout `layout` and writing the other.
+int launchStep(const LatticeView& lat, int layout, const StepArgs& args, void* stream);
+// Momentum-exchange force over a sparse list of fluid cells with solid links.
+// partials: device scratch of 3 * partialSlots doubles; out3: device double[3].
+int launchForce(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks,
+                int count, double* partials, int partialSlots, double* out3, void* stream);
+int forcePartialSlots(int count);
+// Global FP64 mass over Fluid cells; partials sized by massPartialSlots().
+int launchMass(const LatticeView& lat, int layout, double* partials, int partialSlots,
+               double* out1, void* stream);
+int massPartialSlots(const Dims& d);
+// NaN / runaway watchdog (strided sample), verdict written into *flag.
+int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, int flagBase,
+                   int* flag, void* stream);
+int launchFillFloat(float* dst, long long count, float value, void* stream);
+
+}  // namespace fc

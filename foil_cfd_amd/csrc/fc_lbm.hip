@@ -1,0 +1,854 @@
+// FoilCFD D3Q19 kernels for gfx950: fused TRT + Smagorinsky stream-collide
+// on an in-place (AA) population store, equilibrium init, spanwise speck
+// perturbation, canonical upload/download, FP64 momentum-exchange force,
+// FP64 global mass and the NaN watchdog.
+//
+// Behaviour mirrors the reference kernels (src/sim/lbm_core.cu) cell for cell;
+// the implementation is organised for CDNA4 instead:
+//  * one population array updated in place (AA pattern) instead of a ping-pong
+//    pair: half the footprint, and a step reads + writes each population once;
+//  * one 64-lane wavefront per 64-cell x segment of a (y,z) row, so all row /
+//    plane offsets (including the periodic z wrap, which replaces the
+//    reference's ghost planes and their refresh kernel) are wave-uniform
+//    scalars and only the x index lives in a vector register;
+//  * workgroups remapped so each XCD (private L2) streams one contiguous slab;
+//  * collision in pair (symmetric / antisymmetric) form on scaled deviations
+//    H = K (f - w), ~250 VALU ops per cell, so FP16 storage stays HBM bound;
+//  * the zero-gradient outlet, which needs the previous post-collision state
+//    of the x-1 column (overwritten in place), reads a 2 x 19 x ny*nz feed
+//    plane that the x-1 cells publish every step.
+#include <hip/hip_runtime.h>
+
+#include "fc_device.hpp"
+#include "fc_kernels.hpp"
+
+namespace fc {
+namespace {
+
+// Kernel arguments of one lattice level (POD, passed by value).
+struct KArgs {
+    void* f;
+    const std::uint8_t* kind;
+    void* feed;
+    float* rho;
+    float* u;
+    float* v;
+    float* w;
+    const std::uint16_t* uwx;
+    const std::uint16_t* uwy;
+    const std::uint16_t* uwz;
+    long long ncells;
+    int nx, ny, nz;
+    int nseg;     // 64-cell segments per row
+    int nwaves;   // nseg * ny * nz
+    int nblocks;  // ceil(nwaves / kWavesPerBlock)
+    int chunk;    // blocks per XCD after the remap
+    float tau0, lam, pre, uIn;
+    int writeMacro;
+};
+
+KArgs makeArgs(const LatticeView& lat) {
+    KArgs a{};
+    a.f = lat.f;
+    a.kind = lat.kind;
+    a.feed = lat.feed;
+    a.rho = lat.rho;
+    a.u = lat.u;
+    a.v = lat.v;
+    a.w = lat.w;
+    a.uwx = lat.uwx;
+    a.uwy = lat.uwy;
+    a.uwz = lat.uwz;
+    a.ncells = lat.dims.cells();
+    a.nx = lat.dims.nx;
+    a.ny = lat.dims.ny;
+    a.nz = lat.dims.nz;
+    a.nseg = (a.nx + kWave - 1) / kWave;
+    a.nwaves = a.nseg * a.ny * a.nz;
+    a.nblocks = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+    a.chunk = (a.nblocks + kXcds - 1) / kXcds;
+    return a;
+}
+
+// Position of this lane in the wave-row decomposition plus the 3 x 3 table
+// of row start indices for y-1..y+1, z-1..z+1 (z periodic; y and x periodic
+// only when requested). Row starts of rows outside a non-periodic y range
+// are flagged by yOk so boundary cells can skip out-of-domain scatter writes.
+struct RowPos {
+    int x, y, z;
+    int xm, xp;
+    int R[3][3];
+    bool yOk[3];
+};
+
+template <bool PX, bool PY>
+__device__ inline bool rowPos(const KArgs& a, RowPos& p) {
+    const int hb = blockIdx.x;
+    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
+    if (lb >= a.nblocks) return false;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int gw = lb * kWavesPerBlock + wv;
+    if (gw >= a.nwaves) return false;
+    const int row = gw / a.nseg;
+    const int seg = gw - row * a.nseg;
+    const int z = row / a.ny;
+    const int y = row - z * a.ny;
+    const int x = seg * kWave + static_cast<int>(threadIdx.x & (kWave - 1));
+    if (x >= a.nx) return false;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.xm = x - 1;
+    p.xp = x + 1;
+    if (PX) {
+        if (p.xm < 0) p.xm += a.nx;
+        if (p.xp >= a.nx) p.xp -= a.nx;
+    }
+    int ys[3] = {y - 1, y, y + 1};
+    p.yOk[0] = p.yOk[1] = p.yOk[2] = true;
+    if (PY) {
+        if (ys[0] < 0) ys[0] += a.ny;
+        if (ys[2] >= a.ny) ys[2] -= a.ny;
+    } else {
+        p.yOk[0] = ys[0] >= 0;
+        p.yOk[2] = ys[2] < a.ny;
+        if (!p.yOk[0]) ys[0] = 0;
+        if (!p.yOk[2]) ys[2] = a.ny - 1;
+    }
+    const int zs[3] = {z == 0 ? a.nz - 1 : z - 1, z, z == a.nz - 1 ? 0 : z + 1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) p.R[i][j] = (ys[i] + a.ny * zs[j]) * a.nx;
+    return true;
+}
+
+// Cell index of (x + ex, y + ey, z + ez) for compile-time offsets.
+__device__ inline unsigned nb(const RowPos& p, int ex, int ey, int ez) {
+    const int xx = ex < 0 ? p.xm : (ex > 0 ? p.xp : p.x);
+    return static_cast<unsigned>(xx + p.R[ey + 1][ez + 1]);
+}
+
+// True when (x + ex, y + ey) is inside a non-periodic domain (z wraps).
+template <bool PX>
+__device__ inline bool inDomain(const KArgs& a, const RowPos& p, int ex, int ey) {
+    bool ok = p.yOk[ey + 1];
+    if (!PX) {
+        const int xx = p.x + ex;
+        ok = ok && xx >= 0 && xx < a.nx;
+    }
+    return ok;
+}
+
+// Scaled equilibrium deviation K * (feq_q(rho, u) - w_q) for one direction,
+// via the same constants the collision uses.
+template <class C>
+__device__ inline C heqOf(int q, C K, C drho, C rho, C ux, C uy, C uz, C usq) {
+    const C w = wOf<C>(q);
+    const C cu = C(kCx[q]) * ux + C(kCy[q]) * uy + C(kCz[q]) * uz;
+    return K * w * (drho + rho * (C(3) * cu + C(4.5) * cu * cu - usq));
+}
+
+// ---------------------------------------------------------------------------
+// TRT + Smagorinsky collision of one cell on scaled deviations H (in place).
+// Same operator as the reference (src/sim/lbm_core.cu:385-475): moments,
+// density floor 0.05, |u| limiter 0.30, second-order equilibrium, tau_eff
+// from |Pi_neq| (Hou et al.), TRT with tau- from the magic parameter. The
+// order of operations differs (pair form, moments from pair sums, Pi_neq from
+// the closed-form equilibrium moments), so FP32 results agree to rounding.
+// ---------------------------------------------------------------------------
+template <class C>
+struct Macro {
+    C rho, ux, uy, uz;
+};
+
+template <class C>
+__device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
+    const C invK = C(1) / K;
+    C s[9], d[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        s[p] = H[2 * p + 1] + H[2 * p + 2];
+        d[p] = H[2 * p + 1] - H[2 * p + 2];
+    }
+    // Pairs: 0 x, 1 y, 2 z, 3 (+x+y), 4 (+x+z), 5 (+y+z), 6 (+x-y), 7 (+x-z), 8 (+y-z).
+    const C R = H[0] + ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8];
+    const C jx = (d[0] + d[3]) + (d[4] + d[6]) + d[7];
+    const C jy = (d[1] + d[3]) + (d[5] - d[6]) + d[8];
+    const C jz = (d[2] + d[4]) + (d[5] - d[7]) - d[8];
+
+    C drho = R * invK;
+    C rho = C(1) + drho;
+    if (!(rho >= C(0.05))) {  // fmax(rho, 0.05) incl. the NaN scrub
+        rho = C(0.05);
+        drho = C(-0.95);
+    }
+    const C invRho = frcp(rho);
+    const C jscale = invK * invRho;
+    C ux = jx * jscale, uy = jy * jscale, uz = jz * jscale;
+    C u2 = ux * ux + uy * uy + uz * uz;
+    if (u2 > C(0.09)) {  // kMaxSimSpeed^2
+        const C sc = C(0.3) * frsqrt(u2);
+        ux *= sc;
+        uy *= sc;
+        uz *= sc;
+        u2 = ux * ux + uy * uy + uz * uz;
+    }
+    const C usq = C(1.5) * u2;
+
+    // Smagorinsky effective relaxation time.
+    C tauEff = tau0;
+    if (pre > C(0)) {
+        const C third = drho * C(1.0 / 3.0);
+        const C rux = rho * ux, ruy = rho * uy, ruz = rho * uz;
+        const C nxx = ((s[0] + s[3]) + (s[4] + s[6]) + s[7]) * invK - third - rux * ux;
+        const C nyy = ((s[1] + s[3]) + (s[5] + s[6]) + s[8]) * invK - third - ruy * uy;
+        const C nzz = ((s[2] + s[4]) + (s[5] + s[7]) + s[8]) * invK - third - ruz * uz;
+        const C nxy = (s[3] - s[6]) * invK - rux * uy;
+        const C nxz = (s[4] - s[7]) * invK - rux * uz;
+        const C nyz = (s[5] - s[8]) * invK - ruy * uz;
+        const C q2 = nxx * nxx + nyy * nyy + nzz * nzz + C(2) * (nxy * nxy + nxz * nxz + nyz * nyz);
+        const C qbar = fsqrt(q2);
+        tauEff = C(0.5) * (tau0 + fsqrt(tau0 * tau0 + pre * qbar * invRho));
+    }
+    const C omP = frcp(tauEff);
+    const C omM = frcp(C(0.5) + lam * frcp(tauEff - C(0.5)));
+    const C P = C(0.5) * (C(1) - omP);
+    const C M = C(0.5) * (C(1) - omM);
+
+    // Equilibrium constants per weight class (w = 1/18 axis, 1/36 diagonal):
+    // sym part A + B cu^2, antisym part Cc cu, pre-multiplied by omega.
+    const C base = drho - rho * usq;
+    const C a1 = omP * K * C(1.0 / 18.0) * base;
+    const C b1 = omP * K * C(4.5 / 18.0) * rho;
+    const C c1 = omM * K * C(3.0 / 18.0) * rho;
+    const C a2 = C(0.5) * a1, b2 = C(0.5) * b1, c2 = C(0.5) * c1;
+    const C a0 = omP * K * C(1.0 / 3.0) * base;
+
+    H[0] = (C(1) - omP) * H[0] + a0;
+    const C cu[9] = {ux, uy, uz, ux + uy, ux + uz, uy + uz, ux - uy, ux - uz, uy - uz};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const C A = p < 3 ? a1 : a2;
+        const C B = p < 3 ? b1 : b2;
+        const C Cc = p < 3 ? c1 : c2;
+        const C c = cu[p];
+        const C Sp = P * s[p] + (A + B * (c * c));
+        const C Dp = M * d[p] + Cc * c;
+        H[2 * p + 1] = Sp + Dp;
+        H[2 * p + 2] = Sp - Dp;
+    }
+    return Macro<C>{rho, ux, uy, uz};
+}
+
+// ---------------------------------------------------------------------------
+// The fused step. FROM_L: the array is in layout L and this step writes
+// layout S (scatter writes); otherwise it reads layout S and writes L (fully
+// local). WM compiles in the moving-wall (Ladd) slip term on solid links.
+// ---------------------------------------------------------------------------
+template <class S, bool FROM_L, bool WM, bool PX, bool PY>
+__global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<PX, PY>(a, p)) return;
+    T* const f = static_cast<T*>(a.f);
+    const long long n = a.ncells;
+    const unsigned c = nb(p, 0, 0, 0);
+    const std::uint8_t k = a.kind[c];
+    const std::uint8_t flag = k & kKindFlagMask;
+    const C K = C(S::K);
+    const int newLayout = FROM_L ? kLayoutS : kLayoutL;
+    const unsigned yz = static_cast<unsigned>(p.y + a.ny * p.z);
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+
+    // Store f*_q of this cell into the output layout (scatter writes are
+    // bounds checked only for boundary cells; fluid cells never leave the
+    // domain because every non-periodic face plane is boundary-flagged).
+    auto storeOut = [&](int q, C val, bool check) {
+        if (FROM_L) {
+            if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
+            S::store(f + static_cast<long long>(q) * n + nb(p, kCx[q], kCy[q], kCz[q]), val);
+        } else {
+            S::store(f + static_cast<long long>(kOpp[q]) * n + c, val);
+        }
+    };
+
+    if (flag != kFluid) {
+        if (flag == kInlet) {  // equilibrium Dirichlet, reference lbm_core.cu:242-253
+            const C uIn = C(a.uIn);
+            const C usq = C(1.5) * uIn * uIn;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q)
+                storeOut(q, heqOf<C>(q, K, C(0), C(1), uIn, C(0), C(0), usq), true);
+            if (a.writeMacro) {
+                a.rho[c] = 1.0f;
+                a.u[c] = a.uIn;
+                a.v[c] = 0.0f;
+                a.w[c] = 0.0f;
+            }
+        } else if (flag == kOutlet) {  // zero-gradient copy, lbm_core.cu:254-272
+            const T* src = static_cast<const T*>(a.feed) + (FROM_L ? 0 : 1) * kQ * plane;
+            C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const C h = S::load(src + q * plane + yz);
+                storeOut(q, h, true);
+                r += h;
+                jx += C(kCx[q]) * h;
+                jy += C(kCy[q]) * h;
+                jz += C(kCz[q]) * h;
+            }
+            if (a.writeMacro) {
+                const C rr = C(1) + r / K;
+                const C inv = C(1) / rr;
+                a.rho[c] = static_cast<float>(rr);
+                a.u[c] = static_cast<float>(jx / K * inv);
+                a.v[c] = static_cast<float>(jy / K * inv);
+                a.w[c] = static_cast<float>(jz / K * inv);
+            }
+        } else if (a.writeMacro) {  // solid / slip / interface: no population write
+            a.rho[c] = 1.0f;
+            a.u[c] = 0.0f;
+            a.v[c] = 0.0f;
+            a.w[c] = 0.0f;
+        }
+        return;
+    }
+
+    // ---- fluid: pull the 19 incoming populations ---------------------------
+    C H[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        if (FROM_L)
+            H[q] = S::load(f + static_cast<long long>(kOpp[q]) * n + nb(p, -kCx[q], -kCy[q], -kCz[q]));
+        else
+            H[q] = S::load(f + static_cast<long long>(q) * n + c);
+    }
+    if (k & kKindSpecial) {
+        // Per-link source select by the upstream neighbour's flag, as the
+        // reference (lbm_core.cu:306-331), mapped onto the AA slots.
+        C uwx = 0, uwy = 0, uwz = 0;
+        if (WM && (k & kKindWall)) {
+            uwx = C(halfBits(a.uwx[c]));
+            uwy = C(halfBits(a.uwy[c]));
+            uwz = C(halfBits(a.uwz[c]));
+        }
+#pragma unroll
+        for (int q = 1; q < kQ; ++q) {
+            const int cx = kCx[q], cy = kCy[q], cz = kCz[q];
+            const std::uint8_t nf = a.kind[nb(p, -cx, -cy, -cz)] & kKindFlagMask;
+            if (nf == kSolid) {
+                // half-way bounce-back: own f*_opp(q)
+                if (FROM_L)
+                    H[q] = S::load(f + static_cast<long long>(q) * n + c);
+                else
+                    H[q] = S::load(f + static_cast<long long>(kOpp[q]) * n + nb(p, -cx, -cy, -cz));
+                if (WM) {
+                    const C cuw = C(cx) * uwx + C(cy) * uwy + C(cz) * uwz;
+                    H[q] += K * C(6) * wOf<C>(q) * cuw;
+                }
+            } else if (nf == kSlipTop || nf == kSlipBottom) {
+                const int m = kMirY[q];
+                if (FROM_L)
+                    H[q] = S::load(f + static_cast<long long>(kOpp[m]) * n + nb(p, -cx, 0, -cz));
+                else
+                    H[q] = S::load(f + static_cast<long long>(m) * n + nb(p, 0, -cy, 0));
+            } else if (nf == kSlipFront || nf == kSlipBack) {
+                const int m = kMirZ[q];
+                if (FROM_L)
+                    H[q] = S::load(f + static_cast<long long>(kOpp[m]) * n + nb(p, -cx, -cy, 0));
+                else
+                    H[q] = S::load(f + static_cast<long long>(m) * n + nb(p, 0, 0, -cz));
+            }
+        }
+    }
+
+    const Macro<C> mac = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
+
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) storeOut(q, H[q], false);
+    if (k & kKindFeed) {
+        T* dstFeed = static_cast<T*>(a.feed) + newLayout * kQ * plane;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) S::store(dstFeed + q * plane + yz, H[q]);
+    }
+    if (a.writeMacro) {
+        a.rho[c] = static_cast<float>(mac.rho);
+        a.u[c] = static_cast<float>(mac.ux);
+        a.v[c] = static_cast<float>(mac.uy);
+        a.w[c] = static_cast<float>(mac.uz);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Init: f = feq(1, (u0,0,0)) (solid cells at rest) written in layout L, plus
+// both halves of the outlet feed planes (reference initEquilibriumKernel,
+// lbm_core.cu:102-118).
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(kBlock) initKernel(KArgs a, float u0) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<false, false>(a, p)) return;
+    T* const f = static_cast<T*>(a.f);
+    const unsigned c = nb(p, 0, 0, 0);
+    const std::uint8_t k = a.kind[c];
+    const C ux = ((k & kKindFlagMask) == kSolid) ? C(0) : C(u0);
+    const C K = C(S::K);
+    const C usq = C(1.5) * ux * ux;
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+    const unsigned yz = static_cast<unsigned>(p.y + a.ny * p.z);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const C h = heqOf<C>(q, K, C(0), C(1), ux, C(0), C(0), usq);
+        S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
+        if (k & kKindFeed) {
+            T* fd = static_cast<T*>(a.feed);
+            S::store(fd + q * plane + yz, h);
+            S::store(fd + (kQ + q) * plane + yz, h);
+        }
+    }
+}
+
+// Spanwise speck perturbation on fluid cells, layout L (reference
+// spanwisePerturbKernel, lbm_core.cu:130-160, same fast sine intrinsic).
+template <class S>
+__global__ void __launch_bounds__(kBlock) speckKernel(KArgs a, float amp, float ph1, float ph2,
+                                                      float ph3) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<false, false>(a, p)) return;
+    T* const f = static_cast<T*>(a.f);
+    const unsigned c = nb(p, 0, 0, 0);
+    if ((a.kind[c] & kKindFlagMask) != kFluid) return;
+    const float tz = 6.2831853f * (static_cast<float>(p.z) + 0.5f) / a.nz;
+    const float tx = 6.2831853f * (static_cast<float>(p.x) + 0.5f) / a.nx;
+    const float wz = amp * (0.6f * __sinf(3.0f * tz + ph1) + 0.4f * __sinf(7.0f * tz + ph2)) *
+                     (1.0f + 0.3f * __sinf(2.0f * tx + ph3));
+    const C K = C(S::K);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        if (kCz[q] == 0) continue;
+        T* slot = f + static_cast<long long>(kOpp[q]) * a.ncells + c;
+        S::store(slot, S::load(slot) + K * C(3) * wOf<C>(q) * C(kCz[q]) * C(wz));
+    }
+}
+
+// Canonical unpadded SoA (FP32 or FP64) -> layout L, and the feed planes.
+template <class S, class CT>
+__global__ void __launch_bounds__(kBlock) uploadKernel(KArgs a, const CT* canon) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<false, false>(a, p)) return;
+    T* const f = static_cast<T*>(a.f);
+    const unsigned c = nb(p, 0, 0, 0);
+    const std::uint8_t k = a.kind[c];
+    const C K = C(S::K);
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+    const unsigned yz = static_cast<unsigned>(p.y + a.ny * p.z);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const C h = K * (C(canon[static_cast<long long>(q) * a.ncells + c]) - wOf<C>(q));
+        S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
+        if (k & kKindFeed) {
+            T* fd = static_cast<T*>(a.feed);
+            S::store(fd + q * plane + yz, h);
+            S::store(fd + (kQ + q) * plane + yz, h);
+        }
+    }
+}
+
+// Layout L or S -> canonical SoA. Slots of layout S that fall outside a
+// non-periodic domain are unrecoverable and come back as NaN.
+template <class S, class CT, bool PX, bool PY>
+__global__ void __launch_bounds__(kBlock) downloadKernel(KArgs a, int layout, CT* canon) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<PX, PY>(a, p)) return;
+    const T* const f = static_cast<const T*>(a.f);
+    const unsigned c = nb(p, 0, 0, 0);
+    const C invK = C(1) / C(S::K);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        CT out;
+        if (layout == kLayoutL) {
+            out = CT(wOf<C>(q) + S::load(f + static_cast<long long>(kOpp[q]) * a.ncells + c) * invK);
+        } else if (inDomain<PX>(a, p, kCx[q], kCy[q])) {
+            out = CT(wOf<C>(q) +
+                     S::load(f + static_cast<long long>(q) * a.ncells + nb(p, kCx[q], kCy[q], kCz[q])) *
+                         invK);
+        } else {
+            out = CT(__builtin_nan(""));
+        }
+        canon[static_cast<long long>(q) * a.ncells + c] = out;
+    }
+}
+
+// Post-collision population f*_q of cell (x,y,z) in the given layout, as a
+// deviation H (unscaled by K). Used by the readout kernels.
+template <class S>
+__device__ inline typename S::C postH(const KArgs& a, int layout, int q, int x, int y, int z) {
+    using T = typename S::T;
+    const T* const f = static_cast<const T*>(a.f);
+    if (layout == kLayoutL) {
+        const long long c = x + static_cast<long long>(a.nx) * (y + static_cast<long long>(a.ny) * z);
+        return S::load(f + static_cast<long long>(kOpp[q]) * a.ncells + c);
+    }
+    int xx = x + kCx[q], yy = y + kCy[q], zz = z + kCz[q];
+    if (xx < 0) xx += a.nx;
+    if (xx >= a.nx) xx -= a.nx;
+    if (yy < 0) yy += a.ny;
+    if (yy >= a.ny) yy -= a.ny;
+    if (zz < 0) zz += a.nz;
+    if (zz >= a.nz) zz -= a.nz;
+    const long long c = xx + static_cast<long long>(a.nx) * (yy + static_cast<long long>(a.ny) * zz);
+    return S::load(f + static_cast<long long>(q) * a.ncells + c);
+}
+
+// Wave64 FP64 sum via cross-lane shuffles.
+__device__ inline double waveSum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+
+// Deterministic block reduction of NV doubles per thread (block = kBlock).
+template <int NV>
+__device__ inline void blockSum(double (&v)[NV], double* out) {
+    __shared__ double sm[NV][kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double s = waveSum(v[i]);
+        if (lane == 0) sm[i][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double t = 0;
+            for (int j = 0; j < kWavesPerBlock; ++j) t += sm[i][j];
+            out[i] = t;
+        }
+    }
+}
+
+// Momentum exchange (reference forceReductionKernel<WM>, lbm_core.cu:573-637):
+// F = sum over fluid->solid links of c_q (2 f*_q - 6 w_q c_q.u_w), per-link
+// terms in the compute type, sums in FP64 in a fixed order.
+template <class S>
+__global__ void __launch_bounds__(kBlock) forceKernel(KArgs a, int layout, const int* cells,
+                                                      const std::uint32_t* masks, int count,
+                                                      double* partials) {
+    using C = typename S::C;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double acc[3] = {0, 0, 0};
+    if (i < count) {
+        const int cell = cells[i];
+        const std::uint32_t mask = masks[i];
+        const int z = cell / (a.nx * a.ny);
+        const int rem = cell - z * a.nx * a.ny;
+        const int y = rem / a.nx;
+        const int x = rem - y * a.nx;
+        C uwx = 0, uwy = 0, uwz = 0;
+        if (a.uwx) {
+            uwx = C(halfBits(a.uwx[cell]));
+            uwy = C(halfBits(a.uwy[cell]));
+            uwz = C(halfBits(a.uwz[cell]));
+        }
+        const C invK = C(1) / C(S::K);
+        C fx = 0, fy = 0, fz = 0;
+#pragma unroll
+        for (int q = 1; q < kQ; ++q) {
+            if (!(mask & (1u << q))) continue;
+            const C fq = wOf<C>(q) + postH<S>(a, layout, q, x, y, z) * invK;
+            const C corr = C(6) * wOf<C>(q) * (C(kCx[q]) * uwx + C(kCy[q]) * uwy + C(kCz[q]) * uwz);
+            const C m = C(2) * fq - corr;
+            fx += C(kCx[q]) * m;
+            fy += C(kCy[q]) * m;
+            fz += C(kCz[q]) * m;
+        }
+        acc[0] = static_cast<double>(fx);
+        acc[1] = static_cast<double>(fy);
+        acc[2] = static_cast<double>(fz);
+    }
+    blockSum<3>(acc, partials + 3 * blockIdx.x);
+}
+
+// Fixed-order final sum of NV-wide partials by one block.
+template <int NV>
+__global__ void __launch_bounds__(kBlock) finalSumKernel(const double* partials, int slots,
+                                                         double* out) {
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0;
+    for (int s = threadIdx.x; s < slots; s += kBlock)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += partials[NV * s + i];
+    blockSum<NV>(acc, out);
+}
+
+// Global mass sum_{fluid cells} sum_q f*_q in FP64 (grid-stride, fixed order).
+template <class S>
+__global__ void __launch_bounds__(kBlock) massKernel(KArgs a, int layout, double* partials) {
+    using C = typename S::C;
+    const C invK = C(1) / C(S::K);
+    double acc[1] = {0};
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long cell = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; cell < a.ncells;
+         cell += stride) {
+        if ((a.kind[cell] & kKindFlagMask) != kFluid) continue;
+        const int z = static_cast<int>(cell / (static_cast<long long>(a.nx) * a.ny));
+        const int rem = static_cast<int>(cell - static_cast<long long>(z) * a.nx * a.ny);
+        const int y = rem / a.nx;
+        const int x = rem - y * a.nx;
+        C s = 0;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) s += postH<S>(a, layout, q, x, y, z);
+        acc[0] += 1.0 + static_cast<double>(s * invK);
+    }
+    blockSum<1>(acc, partials + blockIdx.x);
+}
+
+// NaN watchdog (reference nanWatchdogKernel, lbm_core.cu:522-559).
+template <class S>
+__global__ void __launch_bounds__(kBlock) watchdogKernel(KArgs a, int layout, long long offset,
+                                                         int flagBase, int* flag) {
+    const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long cell = t * 4093 + offset;
+    if (cell >= a.ncells) return;
+    const int z = static_cast<int>(cell / (static_cast<long long>(a.nx) * a.ny));
+    const int rem = static_cast<int>(cell - static_cast<long long>(z) * a.nx * a.ny);
+    const int y = rem / a.nx;
+    const int x = rem - y * a.nx;
+    const float invK = 1.0f / static_cast<float>(S::K);
+    float s = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float fq = wOf<float>(q) + static_cast<float>(postH<S>(a, layout, q, x, y, z)) * invK;
+        s += fq;
+        jx += kCx[q] * fq;
+        jy += kCy[q] * fq;
+        jz += kCz[q] * fq;
+    }
+    if (!__builtin_isfinite(s)) {
+        *flag = flagBase + 1;
+    } else {
+        const float speed2 = (jx * jx + jy * jy + jz * jz) / fmaxf(s * s, 1e-12f);
+        const float cap = 0.95f * 0.30f;
+        if (speed2 > cap * cap) *flag = flagBase + 2;
+    }
+}
+
+__global__ void fillFloatKernel(float* dst, long long n, float value) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = value;
+}
+
+unsigned rowGrid(const KArgs& a) { return static_cast<unsigned>(a.chunk * kXcds); }
+
+int lastError() { return static_cast<int>(hipGetLastError()); }
+
+template <class S>
+int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, hipStream_t st) {
+    const bool wm = lat.uwx && lat.uwy && lat.uwz;
+    const bool fromL = layout == kLayoutL;
+    const bool pxy = lat.periodicX || lat.periodicY;
+    const dim3 g(rowGrid(a)), b(kBlock);
+    // Periodic x / y are only used together (fully periodic box).
+    if (pxy) {
+        if (fromL)
+            stepKernel<S, true, false, true, true><<<g, b, 0, st>>>(a);
+        else
+            stepKernel<S, false, false, true, true><<<g, b, 0, st>>>(a);
+    } else if (wm) {
+        if (fromL)
+            stepKernel<S, true, true, false, false><<<g, b, 0, st>>>(a);
+        else
+            stepKernel<S, false, true, false, false><<<g, b, 0, st>>>(a);
+    } else {
+        if (fromL)
+            stepKernel<S, true, false, false, false><<<g, b, 0, st>>>(a);
+        else
+            stepKernel<S, false, false, false, false><<<g, b, 0, st>>>(a);
+    }
+    return lastError();
+}
+
+template <template <class> class F, class... Args>
+int byStorage(int storage, Args&&... args) {
+    switch (storage) {
+        case kStoreF32: return F<StoreF32>::run(args...);
+        case kStoreF16: return F<StoreF16>::run(args...);
+        case kStoreBF16: return F<StoreBF16>::run(args...);
+        case kStoreF64: return F<StoreF64>::run(args...);
+        default: return static_cast<int>(hipErrorInvalidValue);
+    }
+}
+
+template <class S>
+struct StepOp {
+    static int run(const LatticeView& lat, int layout, const StepArgs& sa, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        a.tau0 = sa.tau0;
+        a.lam = sa.magicLambda;
+        a.pre = sa.smagPrefactor;
+        a.uIn = sa.uInlet;
+        a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+        return dispatchStep<S>(a, lat, layout, st);
+    }
+};
+
+template <class S>
+struct InitOp {
+    static int run(const LatticeView& lat, float u0, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        initKernel<S><<<rowGrid(a), kBlock, 0, st>>>(a, u0);
+        return lastError();
+    }
+};
+
+template <class S>
+struct SpeckOp {
+    static int run(const LatticeView& lat, float amp, float p1, float p2, float p3, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        speckKernel<S><<<rowGrid(a), kBlock, 0, st>>>(a, amp, p1, p2, p3);
+        return lastError();
+    }
+};
+
+template <class S>
+struct UploadOp {
+    static int run(const LatticeView& lat, const void* canon, int isF64, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        if (isF64)
+            uploadKernel<S, double><<<rowGrid(a), kBlock, 0, st>>>(a, static_cast<const double*>(canon));
+        else
+            uploadKernel<S, float><<<rowGrid(a), kBlock, 0, st>>>(a, static_cast<const float*>(canon));
+        return lastError();
+    }
+};
+
+template <class S>
+struct DownloadOp {
+    static int run(const LatticeView& lat, int layout, void* canon, int isF64, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        const bool pxy = lat.periodicX || lat.periodicY;
+        if (isF64) {
+            if (pxy)
+                downloadKernel<S, double, true, true><<<rowGrid(a), kBlock, 0, st>>>(a, layout, static_cast<double*>(canon));
+            else
+                downloadKernel<S, double, false, false><<<rowGrid(a), kBlock, 0, st>>>(a, layout, static_cast<double*>(canon));
+        } else {
+            if (pxy)
+                downloadKernel<S, float, true, true><<<rowGrid(a), kBlock, 0, st>>>(a, layout, static_cast<float*>(canon));
+            else
+                downloadKernel<S, float, false, false><<<rowGrid(a), kBlock, 0, st>>>(a, layout, static_cast<float*>(canon));
+        }
+        return lastError();
+    }
+};
+
+template <class S>
+struct ForceOp {
+    static int run(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks,
+                   int count, double* partials, int slots, double* out3, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        const int blocks = (count + kBlock - 1) / kBlock;
+        if (blocks > slots) return static_cast<int>(hipErrorInvalidValue);
+        if (blocks > 0) forceKernel<S><<<blocks, kBlock, 0, st>>>(a, layout, cells, masks, count, partials);
+        finalSumKernel<3><<<1, kBlock, 0, st>>>(partials, blocks, out3);
+        return lastError();
+    }
+};
+
+template <class S>
+struct MassOp {
+    static int run(const LatticeView& lat, int layout, double* partials, int slots, double* out1,
+                   hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        massKernel<S><<<slots, kBlock, 0, st>>>(a, layout, partials);
+        finalSumKernel<1><<<1, kBlock, 0, st>>>(partials, slots, out1);
+        return lastError();
+    }
+};
+
+template <class S>
+struct WatchdogOp {
+    static int run(const LatticeView& lat, int layout, long long offset, int flagBase, int* flag,
+                   hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        const long long stride = 4093;
+        const long long off = ((offset % stride) + stride) % stride;
+        const long long samples = (a.ncells + stride - 1) / stride;
+        const unsigned blocks = static_cast<unsigned>((samples + kBlock - 1) / kBlock);
+        watchdogKernel<S><<<blocks, kBlock, 0, st>>>(a, layout, off, flagBase, flag);
+        return lastError();
+    }
+};
+
+}  // namespace
+
+int launchInitEquilibrium(const LatticeView& lat, float uInit, void* stream) {
+    return byStorage<InitOp>(lat.storage, lat, uInit, static_cast<hipStream_t>(stream));
+}
+
+int launchSpanwisePerturb(const LatticeView& lat, float amplitude, float phase1, float phase2,
+                          float phase3, void* stream) {
+    return byStorage<SpeckOp>(lat.storage, lat, amplitude, phase1, phase2, phase3,
+                              static_cast<hipStream_t>(stream));
+}
+
+int launchUploadCanonical(const LatticeView& lat, const void* canon, int canonIsF64, void* stream) {
+    return byStorage<UploadOp>(lat.storage, lat, canon, canonIsF64, static_cast<hipStream_t>(stream));
+}
+
+int launchDownloadCanonical(const LatticeView& lat, int layout, void* canon, int canonIsF64,
+                            void* stream) {
+    return byStorage<DownloadOp>(lat.storage, lat, layout, canon, canonIsF64,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int launchStep(const LatticeView& lat, int layout, const StepArgs& args, void* stream) {
+    return byStorage<StepOp>(lat.storage, lat, layout, args, static_cast<hipStream_t>(stream));
+}
+
+int forcePartialSlots(int count) { return (count + kBlock - 1) / kBlock + 1; }
+
+int launchForce(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks,
+                int count, double* partials, int partialSlots, double* out3, void* stream) {
+    return byStorage<ForceOp>(lat.storage, lat, layout, cells, masks, count, partials, partialSlots,
+                              out3, static_cast<hipStream_t>(stream));
+}
+
+int massPartialSlots(const Dims& d) {
+    const long long blocks = (d.cells() + kBlock - 1) / kBlock;
+    return static_cast<int>(blocks < 1024 ? (blocks < 1 ? 1 : blocks) : 1024);
+}
+
+int launchMass(const LatticeView& lat, int layout, double* partials, int partialSlots, double* out1,
+               void* stream) {
+    return byStorage<MassOp>(lat.storage, lat, layout, partials, partialSlots, out1,
+                             static_cast<hipStream_t>(stream));
+}
+
+int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, int flagBase,
+                   int* flag, void* stream) {
+    return byStorage<WatchdogOp>(lat.storage, lat, layout, sampleOffset, flagBase, flag,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int launchFillFloat(float* dst, long long count, float value, void* stream) {
+    if (!dst || count <= 0) return static_cast<int>(hipErrorInvalidValue);
+    const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
+    fillFloatKernel<<<blocks, kBlock, 0, static_cast<hipStream_t>(stream)>>>(dst, count, value);
+    return lastError();
+}
+
+}  // namespace fc

@@ -1,0 +1,1140 @@
+// FoilCFD host orchestrator + C ABI (include/foilcfd.h).
+//
+// Owns the device state of one lattice (one in-place population array in
+// the chosen storage format, kind bytes, outlet feed planes, macro fields,
+// force list) on one HIP stream, and sequences the steps exactly like the
+// reference's LBMSolver::stepN (src/sim/lbm_solver.cpp:1132-1390): ramped
+// tau / inlet speed per step, macro store on the batch's last step, NaN
+// watchdog every 200 steps with the 977-step sample rotation, one
+// momentum-exchange sample per batch folded asynchronously into the force
+// EMA and the trailing-average ring (lbm_solver.cpp:603-666, :1414-1509).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/foilcfd.h"
+#include "fc_geometry.hpp"
+#include "fc_kernels.hpp"
+#include "fc_units.hpp"
+
+namespace fc {
+namespace {
+
+constexpr int kWatchdogPeriodSteps = 200;     // lbm_solver.cpp:25
+constexpr float kForceGateFlowThroughs = 2.0f;  // lbm_solver.h:25
+constexpr float kWarmGateFlowThroughs = 0.5f;   // lbm_solver.cpp:30
+constexpr long long kCompactSettleSteps = 1000; // lbm_solver.cpp:34
+constexpr double kTimingEmaAlpha = 0.25;        // lbm_solver.cpp:41
+constexpr float kSpeckAmplitudeFrac = 1e-3f;    // lbm_solver.cpp:46
+constexpr unsigned kSpeckSeed = 0x5eedu;        // lbm_solver.cpp:49
+constexpr int kAvgRingSize = 256;               // lbm_solver.cpp:120
+
+// Phase of the speck perturbation from the seed (Weyl-style integer hash,
+// reference lbm_core.cu:763-767).
+float speckPhase(unsigned seed, unsigned salt) {
+    unsigned h = (seed + salt) * 2654435761u;
+    h ^= h >> 16;
+    return 6.2831853f * static_cast<float>(h & 0xFFFFFFu) / 16777216.0f;
+}
+
+template <class T>
+int hipAlloc(T** p, std::size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return FC_OK;
+    void* raw = nullptr;
+    if (hipMalloc(&raw, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return FC_ERR_ALLOC;
+    }
+    *p = static_cast<T*>(raw);
+    return FC_OK;
+}
+
+template <class T>
+void hipRelease(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+Scaling fromC(const fc_scaling& c) {
+    Scaling s;
+    s.chord_m = c.chord_m;
+    s.airspeed_ms = c.airspeed_ms;
+    s.nu_m2s = c.nu_m2s;
+    s.chordCells = c.chord_cells;
+    s.u_lat = c.u_lat;
+    s.dx = c.dx;
+    s.dt = c.dt;
+    s.nu_lat_target = c.nu_lat_target;
+    s.nu_lat = c.nu_lat;
+    s.tau = c.tau;
+    s.tauClamped = c.tau_clamped != 0;
+    s.reTarget = c.re_target;
+    s.reEffective = c.re_effective;
+    return s;
+}
+
+fc_scaling toC(const Scaling& s) {
+    fc_scaling c{};
+    c.chord_m = s.chord_m;
+    c.airspeed_ms = s.airspeed_ms;
+    c.nu_m2s = s.nu_m2s;
+    c.chord_cells = s.chordCells;
+    c.u_lat = s.u_lat;
+    c.dx = s.dx;
+    c.dt = s.dt;
+    c.nu_lat_target = s.nu_lat_target;
+    c.nu_lat = s.nu_lat;
+    c.tau = s.tau;
+    c.tau_clamped = s.tauClamped ? 1 : 0;
+    c.re_target = s.reTarget;
+    c.re_effective = s.reEffective;
+    return c;
+}
+
+DomainLayout fromC(const fc_layout& l) {
+    DomainLayout d;
+    d.dims = Dims{l.dims.nx, l.dims.ny, l.dims.nz};
+    d.chordCells = l.chord_cells;
+    d.anchorXFrac = l.anchor_x_frac;
+    d.anchorYFrac = l.anchor_y_frac;
+    d.anchorXCells = l.anchor_x_cells;
+    d.anchorYCells = l.anchor_y_cells;
+    d.anchorAbsolute = l.anchor_absolute != 0;
+    return d;
+}
+
+fc_layout toC(const DomainLayout& d) {
+    fc_layout l{};
+    l.dims = fc_dims{d.dims.nx, d.dims.ny, d.dims.nz};
+    l.chord_cells = d.chordCells;
+    l.anchor_x_frac = d.anchorXFrac;
+    l.anchor_y_frac = d.anchorYFrac;
+    l.anchor_x_cells = d.anchorXCells;
+    l.anchor_y_cells = d.anchorYCells;
+    l.anchor_absolute = d.anchorAbsolute ? 1 : 0;
+    return l;
+}
+
+Airfoil airfoilFrom(const float* xy, int npts) {
+    Airfoil a;
+    a.points.resize(static_cast<std::size_t>(std::max(npts, 0)));
+    for (int i = 0; i < npts; ++i) a.points[i] = Vec2{xy[2 * i], xy[2 * i + 1]};
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+class Solver {
+public:
+    ~Solver() { freeAll(); }
+
+    std::string error;
+
+    int init(const fc_dims& d, const fc_scaling& sc, const std::uint8_t* flagsIn,
+             const fc_init_options* opt) {
+        freeAll();
+        if (!flagsIn) return fail(FC_ERR_INVALID, "flags is null");
+        dims_ = Dims{d.nx, d.ny, d.nz};
+        if (dims_.nx < 1 || dims_.ny < 1 || dims_.nz < 1)
+            return fail(FC_ERR_INVALID, "grid dimensions must be positive");
+        if (dims_.cells() >= (1LL << 31))
+            return fail(FC_ERR_INVALID, "grid exceeds 2^31 cells");
+        storage_ = opt ? opt->storage : kStoreF32;
+        if (storage_ < kStoreF32 || storage_ > kStoreF64)
+            return fail(FC_ERR_INVALID, "unknown storage format");
+        periodicX_ = opt ? (opt->periodic_x != 0) : false;
+        periodicY_ = opt ? (opt->periodic_y != 0) : false;
+        if (periodicX_ != periodicY_)
+            return fail(FC_ERR_INVALID, "periodic x and y must be enabled together");
+        scaling_ = fromC(sc);
+        ncells_ = dims_.cells();
+        hostFlags_.assign(flagsIn, flagsIn + ncells_);
+        std::string why;
+        if (!validateFlags(hostFlags_, why)) return fail(FC_ERR_INVALID, why);
+
+        int devs = 0;
+        if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) {
+            (void)hipGetLastError();
+            return fail(FC_ERR_DEVICE, "no HIP device available");
+        }
+        if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "stream creation failed");
+
+        const std::size_t popBytes = static_cast<std::size_t>(kQ) * ncells_ * storageBytes(storage_);
+        const std::size_t plane = static_cast<std::size_t>(dims_.ny) * dims_.nz;
+        int rc = FC_OK;
+        rc |= hipAlloc(&f_, popBytes);
+        rc |= hipAlloc(&kind_, static_cast<std::size_t>(ncells_));
+        rc |= hipAlloc(&feed_, 2 * kQ * plane * storageBytes(storage_));
+        for (float** p : {&rho_, &u_, &v_, &w_}) rc |= hipAlloc(p, ncells_ * sizeof(float));
+        rc |= hipAlloc(&dForce_, 3 * sizeof(double));
+        rc |= hipAlloc(&dMass_, sizeof(double));
+        massSlots_ = massPartialSlots(dims_);
+        rc |= hipAlloc(&dMassPartials_, static_cast<std::size_t>(massSlots_) * sizeof(double));
+        rc |= hipAlloc(&dNan_, sizeof(int));
+        if (rc != FC_OK) {
+            freeAll();
+            return fail(FC_ERR_ALLOC, "device allocation failed (" + std::to_string(popBytes) +
+                                          " bytes of populations)");
+        }
+        if (hipHostMalloc(reinterpret_cast<void**>(&hForce_), 3 * sizeof(double), 0) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&hNan_), sizeof(int), 0) != hipSuccess) {
+            freeAll();
+            return fail(FC_ERR_ALLOC, "pinned readback allocation failed");
+        }
+        for (hipEvent_t* e : {&evT0_, &evT1_, &evForce_, &evNan_}) {
+            if (hipEventCreate(e) != hipSuccess) {
+                freeAll();
+                return fail(FC_ERR_DEVICE, "event creation failed");
+            }
+        }
+        devBytes_ = static_cast<long long>(popBytes + ncells_ + 2 * kQ * plane * storageBytes(storage_) +
+                                           4 * ncells_ * sizeof(float));
+        // Quiescent macro defaults (lbm_solver.cpp:864-874).
+        launchFillFloat(rho_, ncells_, 1.0f, stream_);
+        (void)hipMemsetAsync(u_, 0, ncells_ * sizeof(float), stream_);
+        (void)hipMemsetAsync(v_, 0, ncells_ * sizeof(float), stream_);
+        (void)hipMemsetAsync(w_, 0, ncells_ * sizeof(float), stream_);
+        if (int e = uploadFlags(); e != FC_OK) {
+            freeAll();
+            return e;
+        }
+        initialized_ = true;
+        if (int e = reset(); e != FC_OK) {
+            freeAll();
+            return e;
+        }
+        if (hipStreamSynchronize(stream_) != hipSuccess) {
+            freeAll();
+            return fail(FC_ERR_DEVICE, "initial field setup failed");
+        }
+        return FC_OK;
+    }
+
+    void freeAll() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        hipRelease(f_);
+        hipRelease(kind_);
+        hipRelease(feed_);
+        hipRelease(rho_);
+        hipRelease(u_);
+        hipRelease(v_);
+        hipRelease(w_);
+        hipRelease(dForce_);
+        hipRelease(dMass_);
+        hipRelease(dMassPartials_);
+        hipRelease(dNan_);
+        hipRelease(dForceCells_);
+        hipRelease(dForceMasks_);
+        hipRelease(dForcePartials_);
+        if (hForce_) (void)hipHostFree(hForce_);
+        if (hNan_) (void)hipHostFree(hNan_);
+        hForce_ = nullptr;
+        hNan_ = nullptr;
+        for (hipEvent_t* e : {&evT0_, &evT1_, &evForce_, &evNan_})
+            if (*e) {
+                (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
+        for (auto& pr : kEvents_) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        kEvents_.clear();
+        if (stream_) (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+        initialized_ = false;
+        forceCount_ = 0;
+        devBytes_ = 0;
+    }
+
+    bool initialized() const { return initialized_; }
+
+    int reset() {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        steps_ = 0;
+        layout_ = kLayoutL;
+        rampActive_ = true;
+        rampStartStep_ = 0;
+        gateOpenAtSteps_ = static_cast<long long>(kForceGateFlowThroughs * flowThroughSteps());
+        emaFx_ = emaFy_ = 0.0f;
+        emaSeeded_ = false;
+        forcePending_ = false;
+        stepsAtForceLaunch_ = stepsAtLastForceFold_ = 0;
+        ringHead_ = ringCount_ = 0;
+        nanLatched_ = false;
+        nanPending_ = false;
+        nanKind_ = 0;
+        stepsSinceWatchdog_ = 0;
+        (void)hipMemsetAsync(dNan_, 0, sizeof(int), stream_);
+        // Rest init when the ramp is on, freestream otherwise; then the
+        // spanwise speck (lbm_solver.cpp:919-936).
+        const float initU = rampEnabled_ ? 0.0f : scaling_.u_lat;
+        const LatticeView lat = view();
+        int e = launchInitEquilibrium(lat, initU, stream_);
+        if (e == 0)
+            e = launchSpanwisePerturb(lat, kSpeckAmplitudeFrac * scaling_.u_lat,
+                                      speckPhase(kSpeckSeed, 1u), speckPhase(kSpeckSeed, 2u),
+                                      speckPhase(kSpeckSeed, 3u), stream_);
+        if (e != 0) return fail(FC_ERR_DEVICE, "init kernels failed");
+        return FC_OK;
+    }
+
+    int setFlags(const std::uint8_t* flagsIn) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!flagsIn) return fail(FC_ERR_INVALID, "flags is null");
+        std::vector<std::uint8_t> nf(flagsIn, flagsIn + ncells_);
+        std::string why;
+        if (!validateFlags(nf, why)) return fail(FC_ERR_INVALID, why);
+        hostFlags_.swap(nf);
+        if (int e = uploadFlags(); e != FC_OK) return e;
+        return reset();
+    }
+
+    void setRampEnabled(bool on) { rampEnabled_ = on; }
+    void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
+    void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
+
+    int stepN(int n) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (n <= 0) return FC_OK;
+        pollAsync();
+        if (nanLatched_) return FC_OK;  // paused, reference lbm_solver.cpp:1137
+        const bool timeThisBatch = !timingPending_;
+        if (timeThisBatch) (void)hipEventRecord(evT0_, stream_);
+        const LatticeView lat = view();
+        for (int i = 0; i < n; ++i) {
+            StepArgs a;
+            a.tau0 = effectiveTau();
+            a.magicLambda = kTRTMagicLambda;
+            a.smagPrefactor = smagPrefactor(cs_);
+            a.uInlet = effectiveU();
+            a.writeMacro = (i == n - 1) ? 1 : 0;
+            std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+            if (kernelTiming_) {
+                ev = nextKernelEvents();
+                if (ev) (void)hipEventRecord(ev->first, stream_);
+            }
+            const int e = launchStep(lat, layout_, a, stream_);
+            if (ev) (void)hipEventRecord(ev->second, stream_);
+            if (e != 0) return fail(FC_ERR_DEVICE, std::string("step launch failed: ") +
+                                                       hipGetErrorString(static_cast<hipError_t>(e)));
+            layout_ = 1 - layout_;
+            ++steps_;
+            if (++stepsSinceWatchdog_ >= kWatchdogPeriodSteps && !nanPending_) {
+                stepsSinceWatchdog_ = 0;
+                (void)hipMemsetAsync(dNan_, 0, sizeof(int), stream_);
+                launchWatchdog(lat, layout_, steps_ * 977, 0, dNan_, stream_);
+                (void)hipMemcpyAsync(hNan_, dNan_, sizeof(int), hipMemcpyDeviceToHost, stream_);
+                (void)hipEventRecord(evNan_, stream_);
+                nanPending_ = true;
+            }
+        }
+        if (timeThisBatch) {
+            (void)hipEventRecord(evT1_, stream_);
+            timingPending_ = true;
+            timingBatchN_ = n;
+        }
+        // One momentum-exchange sample per batch (lbm_solver.cpp:1337-1354).
+        if (!forcePending_) {
+            if (launchForceInto(dForce_) == FC_OK) {
+                (void)hipMemcpyAsync(hForce_, dForce_, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                     stream_);
+                (void)hipEventRecord(evForce_, stream_);
+                forcePending_ = true;
+                stepsAtForceLaunch_ = steps_;
+            }
+        }
+        return FC_OK;
+    }
+
+    int synchronize() {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        const hipError_t e = hipStreamSynchronize(stream_);
+        pollAsync();
+        if (e != hipSuccess) return fail(FC_ERR_DEVICE, std::string("stream error: ") + hipGetErrorString(e));
+        return FC_OK;
+    }
+
+    int forces(fc_force_readout* r) {
+        if (!r) return fail(FC_ERR_INVALID, "null output");
+        *r = fc_force_readout{};
+        if (!initialized_) return FC_OK;
+        pollAsync();
+        r->flow_throughs = flowThroughsCompleted();
+        const int span = zSlipWalls_ ? std::max(1, dims_.nz - 2) : dims_.nz;
+        r->cd = scaling_.forceToCoefficient(emaFx_, span);
+        r->cl = scaling_.forceToCoefficient(emaFy_, span);
+        r->lift_to_drag = (std::fabs(r->cd) > 1e-6f) ? r->cl / r->cd : 0.0f;
+        r->valid = (emaSeeded_ && steps_ >= gateOpenAtSteps_) ? 1 : 0;
+        if (r->valid && ringCount_ > 0) {
+            const long long oneFT = static_cast<long long>(flowThroughSteps());
+            std::vector<float> cl, cd, ld;
+            for (int i = 0; i < ringCount_; ++i) {
+                const int idx = (ringHead_ - 1 - i + kAvgRingSize) % kAvgRingSize;
+                const Sample& smp = ring_[idx];
+                if (smp.step < oneFT) continue;
+                const float c1 = scaling_.forceToCoefficient(smp.fy, span);
+                const float c2 = scaling_.forceToCoefficient(smp.fx, span);
+                cl.push_back(c1);
+                cd.push_back(c2);
+                ld.push_back((std::fabs(c2) > 1e-6f) ? c1 / c2 : 0.0f);
+            }
+            if (!cl.empty()) {
+                auto stats = [](std::vector<float> v, float& avg, float& mn, float& mx, float& med) {
+                    float sum = 0.0f;
+                    for (float x : v) sum += x;
+                    avg = sum / static_cast<float>(v.size());
+                    mn = *std::min_element(v.begin(), v.end());
+                    mx = *std::max_element(v.begin(), v.end());
+                    std::sort(v.begin(), v.end());
+                    const std::size_t m = v.size() / 2;
+                    med = (v.size() % 2 == 0) ? 0.5f * (v[m - 1] + v[m]) : v[m];
+                };
+                stats(cl, r->cl_avg, r->cl_min, r->cl_max, r->cl_median);
+                stats(cd, r->cd_avg, r->cd_min, r->cd_max, r->cd_median);
+                stats(ld, r->ld_avg, r->ld_min, r->ld_max, r->ld_median);
+            }
+        }
+        return FC_OK;
+    }
+
+    int forceRaw(double* out3, long long* atStep) {
+        if (!out3) return fail(FC_ERR_INVALID, "null output");
+        pollAsync();
+        out3[0] = lastRaw_[0];
+        out3[1] = lastRaw_[1];
+        out3[2] = lastRaw_[2];
+        if (atStep) *atStep = lastRawStep_;
+        return FC_OK;
+    }
+
+    int forceNow(double* out3) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!out3) return fail(FC_ERR_INVALID, "null output");
+        double* d = nullptr;
+        if (hipAlloc(&d, 3 * sizeof(double)) != FC_OK) return fail(FC_ERR_ALLOC, "force scratch");
+        int rc = launchForceInto(d);
+        if (rc == FC_OK && (hipMemcpyAsync(out3, d, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_) !=
+                                hipSuccess ||
+                            hipStreamSynchronize(stream_) != hipSuccess))
+            rc = fail(FC_ERR_DEVICE, "force readback failed");
+        hipRelease(d);
+        return rc;
+    }
+
+    int mass(double* out) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!out) return fail(FC_ERR_INVALID, "null output");
+        if (launchMass(view(), layout_, dMassPartials_, massSlots_, dMass_, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "mass kernel failed");
+        if (hipMemcpyAsync(out, dMass_, sizeof(double), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "mass readback failed");
+        return FC_OK;
+    }
+
+    int readMacro(float* r, float* u, float* v, float* w) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        const std::size_t bytes = static_cast<std::size_t>(ncells_) * sizeof(float);
+        const std::pair<float*, float*> pairs[] = {{r, rho_}, {u, u_}, {v, v_}, {w, w_}};
+        for (const auto& pr : pairs)
+            if (pr.first && hipMemcpyAsync(pr.first, pr.second, bytes, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "macro readback failed");
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "macro readback failed");
+        return FC_OK;
+    }
+
+    int readFlags(std::uint8_t* out) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!out) return fail(FC_ERR_INVALID, "null output");
+        std::memcpy(out, hostFlags_.data(), hostFlags_.size());
+        return FC_OK;
+    }
+
+    int downloadF(void* host, bool asF64) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!host) return fail(FC_ERR_INVALID, "null output");
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * ncells_ * (asF64 ? 8 : 4);
+        void* d = nullptr;
+        if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "download scratch");
+        int rc = FC_OK;
+        if (launchDownloadCanonical(view(), layout_, d, asF64 ? 1 : 0, stream_) != 0 ||
+            hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            rc = fail(FC_ERR_DEVICE, "download failed");
+        hipRelease(d);
+        return rc;
+    }
+
+    int uploadF(const void* host, bool isF64) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!host) return fail(FC_ERR_INVALID, "null input");
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * ncells_ * (isF64 ? 8 : 4);
+        void* d = nullptr;
+        if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "upload scratch");
+        int rc = FC_OK;
+        if (hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            launchUploadCanonical(view(), d, isF64 ? 1 : 0, stream_) != 0 ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            rc = fail(FC_ERR_DEVICE, "upload failed");
+        hipRelease(d);
+        if (rc != FC_OK) return rc;
+        layout_ = kLayoutL;
+        notifySnapshotRestored(true);
+        return FC_OK;
+    }
+
+    // lbm_solver.cpp:1750-1771
+    void notifySnapshotRestored(bool fullState) {
+        if (!initialized_) return;
+        rampActive_ = false;
+        emaSeeded_ = false;
+        forcePending_ = false;
+        stepsAtLastForceFold_ = steps_;
+        nanLatched_ = false;
+        nanPending_ = false;
+        if (fullState)
+            gateOpenAtSteps_ = steps_ + static_cast<long long>(0.25f * flowThroughSteps());
+        else
+            gateOpenAtSteps_ = steps_ + kCompactSettleSteps +
+                               static_cast<long long>(kWarmGateFlowThroughs * flowThroughSteps());
+    }
+
+    int nanDetected() {
+        pollAsync();
+        return nanLatched_ ? 1 : 0;
+    }
+    int nanKind() {
+        pollAsync();
+        return nanKind_;
+    }
+
+    int perfStats(fc_perf_stats* p) {
+        if (!p) return fail(FC_ERR_INVALID, "null output");
+        pollAsync();
+        *p = fc_perf_stats{};
+        p->last_steps_per_frame = timingBatchN_;
+        p->last_step_ms = lastStepMs_;
+        if (lastStepMs_ > 0.0) p->mlups = static_cast<double>(ncells_) / (lastStepMs_ * 1000.0);
+        p->population_bytes_per_step = 2.0 * kQ * storageBytes(storage_) * static_cast<double>(ncells_);
+        return FC_OK;
+    }
+
+    long long stepCount() const { return steps_; }
+    float currentTau() const { return effectiveTau(); }
+    float flowThroughsCompleted() const {
+        const float ft = flowThroughSteps();
+        return ft > 0.0f ? static_cast<float>(steps_) / ft : 0.0f;
+    }
+    Dims dims() const { return dims_; }
+    const Scaling& scaling() const { return scaling_; }
+    int storage() const { return storage_; }
+    long long deviceBytes() const { return devBytes_; }
+    void deviceFields(void** r, void** u, void** v, void** w, void** fl) const {
+        if (r) *r = rho_;
+        if (u) *u = u_;
+        if (v) *v = v_;
+        if (w) *w = w_;
+        if (fl) *fl = kind_;
+    }
+
+    int setKernelTiming(bool on) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        (void)hipStreamSynchronize(stream_);
+        kernelTiming_ = on;
+        kEventsUsed_ = 0;
+        kTimedMs_ = 0.0;
+        kTimedLaunches_ = 0;
+        return FC_OK;
+    }
+
+    int kernelTiming(int* launches, double* totalMs) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "stream error");
+        harvestKernelEvents();
+        if (launches) *launches = static_cast<int>(kTimedLaunches_);
+        if (totalMs) *totalMs = kTimedMs_;
+        return FC_OK;
+    }
+
+    int timeSteps(int calls, int perCall, double* ms) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!ms || calls <= 0 || perCall <= 0) return fail(FC_ERR_INVALID, "bad arguments");
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "event creation failed");
+        (void)hipEventRecord(a, stream_);
+        int rc = FC_OK;
+        for (int i = 0; i < calls && rc == FC_OK; ++i) rc = stepN(perCall);
+        (void)hipEventRecord(b, stream_);
+        if (hipEventSynchronize(b) != hipSuccess) rc = fail(FC_ERR_DEVICE, "timing sync failed");
+        float t = 0.0f;
+        if (rc == FC_OK) (void)hipEventElapsedTime(&t, a, b);
+        *ms = t;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return rc;
+    }
+
+private:
+    struct Sample {
+        long long step = 0;
+        float fx = 0, fy = 0;
+    };
+
+    int fail(int code, const std::string& why) {
+        error = why;
+        return code;
+    }
+
+    float flowThroughSteps() const { return scaling_.flowThroughSteps(dims_.nx); }
+    float effectiveTau() const {
+        return (rampActive_ && rampEnabled_) ? rampedTau(scaling_, steps_ - rampStartStep_, dims_.nx)
+                                             : scaling_.tau;
+    }
+    float effectiveU() const {
+        return (rampActive_ && rampEnabled_) ? rampedU(scaling_, steps_ - rampStartStep_, dims_.nx)
+                                             : scaling_.u_lat;
+    }
+
+    LatticeView view() const {
+        LatticeView l;
+        l.f = f_;
+        l.storage = storage_;
+        l.kind = kind_;
+        l.dims = dims_;
+        l.periodicX = periodicX_ ? 1 : 0;
+        l.periodicY = periodicY_ ? 1 : 0;
+        l.feed = feed_;
+        l.rho = rho_;
+        l.u = u_;
+        l.v = v_;
+        l.w = w_;
+        return l;
+    }
+
+    std::size_t at(int x, int y, int z) const {
+        return static_cast<std::size_t>(x) +
+               static_cast<std::size_t>(dims_.nx) * (static_cast<std::size_t>(y) +
+                                                     static_cast<std::size_t>(dims_.ny) * static_cast<std::size_t>(z));
+    }
+
+    // Neighbour coordinate along an axis (z always periodic); -1 outside.
+    static int stepAxis(int v, int d, int n, bool periodic) {
+        int r = v + d;
+        if (r < 0) return periodic ? r + n : -1;
+        if (r >= n) return periodic ? r - n : -1;
+        return r;
+    }
+
+    // Reject flag fields the pull stencil cannot execute without leaving the
+    // array: fluid cells on a non-periodic face, outlets at x = 0, flag
+    // values outside the CellFlag range.
+    bool validateFlags(const std::vector<std::uint8_t>& fl, std::string& why) const {
+        for (int z = 0; z < dims_.nz; ++z)
+            for (int y = 0; y < dims_.ny; ++y)
+                for (int x = 0; x < dims_.nx; ++x) {
+                    const std::uint8_t v = fl[at(x, y, z)];
+                    if (v > kInterface) {
+                        why = "flag value " + std::to_string(v) + " out of range";
+                        return false;
+                    }
+                    if (v == kFluid) {
+                        if (!periodicX_ && (x == 0 || x == dims_.nx - 1)) {
+                            why = "fluid cell on a non-periodic x face";
+                            return false;
+                        }
+                        if (!periodicY_ && (y == 0 || y == dims_.ny - 1)) {
+                            why = "fluid cell on a non-periodic y face";
+                            return false;
+                        }
+                    }
+                    if (v == kOutlet && x == 0) {
+                        why = "outlet cell at x = 0 has no upstream column";
+                        return false;
+                    }
+                }
+        return true;
+    }
+
+    int uploadFlags() {
+        // Kind bytes: flag + special / feed bits.
+        std::vector<std::uint8_t> kind(static_cast<std::size_t>(ncells_));
+        std::vector<int> fcells;
+        std::vector<std::uint32_t> fmasks;
+        zSlipWalls_ = false;
+        for (long long i = 0; i < static_cast<long long>(dims_.nx) * dims_.ny && i < ncells_; ++i)
+            if (hostFlags_[static_cast<std::size_t>(i)] == kSlipFront) zSlipWalls_ = true;
+        for (int z = 0; z < dims_.nz; ++z)
+            for (int y = 0; y < dims_.ny; ++y)
+                for (int x = 0; x < dims_.nx; ++x) {
+                    const std::size_t c = at(x, y, z);
+                    const std::uint8_t fl = hostFlags_[c];
+                    std::uint8_t k = fl;
+                    if (fl == kFluid) {
+                        std::uint32_t solidMask = 0;
+                        for (int q = 1; q < kQ; ++q) {
+                            // pull neighbour c - e_q
+                            const int xn = stepAxis(x, -kCx[q], dims_.nx, periodicX_);
+                            const int yn = stepAxis(y, -kCy[q], dims_.ny, periodicY_);
+                            const int zn = stepAxis(z, -kCz[q], dims_.nz, true);
+                            const std::uint8_t nf = hostFlags_[at(xn, yn, zn)];
+                            if (nf == kSolid || (nf >= kSlipTop && nf <= kSlipBack)) k |= kKindSpecial;
+                            // push neighbour c + e_q for the momentum exchange
+                            const int xp = stepAxis(x, kCx[q], dims_.nx, periodicX_);
+                            const int yp = stepAxis(y, kCy[q], dims_.ny, periodicY_);
+                            const int zp = stepAxis(z, kCz[q], dims_.nz, true);
+                            if (hostFlags_[at(xp, yp, zp)] == kSolid) solidMask |= (1u << q);
+                        }
+                        if (solidMask) {
+                            fcells.push_back(static_cast<int>(c));
+                            fmasks.push_back(solidMask);
+                        }
+                    }
+                    const int xr = stepAxis(x, 1, dims_.nx, periodicX_);
+                    if (xr >= 0 && hostFlags_[at(xr, y, z)] == kOutlet) k |= kKindFeed;
+                    kind[c] = k;
+                }
+        if (hipMemcpyAsync(kind_, kind.data(), kind.size(), hipMemcpyHostToDevice, stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "flag upload failed");
+        hipRelease(dForceCells_);
+        hipRelease(dForceMasks_);
+        hipRelease(dForcePartials_);
+        forceCount_ = static_cast<int>(fcells.size());
+        forceSlots_ = forcePartialSlots(forceCount_);
+        int rc = hipAlloc(&dForceCells_, fcells.size() * sizeof(int));
+        rc |= hipAlloc(&dForceMasks_, fmasks.size() * sizeof(std::uint32_t));
+        rc |= hipAlloc(&dForcePartials_, static_cast<std::size_t>(forceSlots_) * 3 * sizeof(double));
+        if (rc != FC_OK) return fail(FC_ERR_ALLOC, "force list allocation failed");
+        if (forceCount_ > 0 &&
+            (hipMemcpyAsync(dForceCells_, fcells.data(), fcells.size() * sizeof(int), hipMemcpyHostToDevice,
+                            stream_) != hipSuccess ||
+             hipMemcpyAsync(dForceMasks_, fmasks.data(), fmasks.size() * sizeof(std::uint32_t),
+                            hipMemcpyHostToDevice, stream_) != hipSuccess))
+            return fail(FC_ERR_DEVICE, "force list upload failed");
+        // The host vectors die at return: finish the copies first.
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "flag upload failed");
+        return FC_OK;
+    }
+
+    int launchForceInto(double* dOut) {
+        if (launchForce(view(), layout_, dForceCells_, dForceMasks_, forceCount_, dForcePartials_,
+                        forceSlots_, dOut, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "force kernel failed");
+        return FC_OK;
+    }
+
+    static bool eventDone(hipEvent_t ev) {
+        const hipError_t st = hipEventQuery(ev);
+        if (st == hipErrorNotReady) (void)hipGetLastError();
+        return st == hipSuccess;
+    }
+
+    // Fold finished asynchronous readbacks (lbm_solver.cpp:616-666).
+    void pollAsync() {
+        if (!initialized_) return;
+        if (timingPending_ && eventDone(evT1_)) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, evT0_, evT1_) == hipSuccess && timingBatchN_ > 0) {
+                lastStepMs_ = static_cast<double>(ms) / timingBatchN_;
+                emaStepMs_ = emaStepMs_ > 0.0 ? emaStepMs_ + kTimingEmaAlpha * (lastStepMs_ - emaStepMs_)
+                                              : lastStepMs_;
+            }
+            timingPending_ = false;
+        }
+        if (forcePending_ && eventDone(evForce_)) {
+            const float windowSteps = std::max(1.0f, emaWindowFlowThroughs_ * flowThroughSteps());
+            const float dSteps = static_cast<float>(stepsAtForceLaunch_ - stepsAtLastForceFold_);
+            const float alpha = 1.0f - std::exp(-dSteps / windowSteps);
+            const float fx = static_cast<float>(hForce_[0]);
+            const float fy = static_cast<float>(hForce_[1]);
+            if (!emaSeeded_) {
+                emaFx_ = fx;
+                emaFy_ = fy;
+                emaSeeded_ = true;
+            } else {
+                emaFx_ += alpha * (fx - emaFx_);
+                emaFy_ += alpha * (fy - emaFy_);
+            }
+            ring_[ringHead_] = Sample{stepsAtForceLaunch_, fx, fy};
+            ringHead_ = (ringHead_ + 1) % kAvgRingSize;
+            if (ringCount_ < kAvgRingSize) ++ringCount_;
+            lastRaw_[0] = hForce_[0];
+            lastRaw_[1] = hForce_[1];
+            lastRaw_[2] = hForce_[2];
+            lastRawStep_ = stepsAtForceLaunch_;
+            stepsAtLastForceFold_ = stepsAtForceLaunch_;
+            forcePending_ = false;
+        }
+        if (nanPending_ && eventDone(evNan_)) {
+            if (*hNan_ != 0) {
+                nanLatched_ = true;
+                nanKind_ = *hNan_;
+            }
+            nanPending_ = false;
+        }
+    }
+
+    std::pair<hipEvent_t, hipEvent_t>* nextKernelEvents() {
+        if (kEventsUsed_ >= 4096) {
+            (void)hipStreamSynchronize(stream_);
+            harvestKernelEvents();
+        }
+        if (kEventsUsed_ >= kEvents_.size()) {
+            std::pair<hipEvent_t, hipEvent_t> pr{nullptr, nullptr};
+            if (hipEventCreate(&pr.first) != hipSuccess || hipEventCreate(&pr.second) != hipSuccess)
+                return nullptr;
+            kEvents_.push_back(pr);
+        }
+        return &kEvents_[kEventsUsed_++];
+    }
+
+    void harvestKernelEvents() {
+        for (std::size_t i = 0; i < kEventsUsed_; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, kEvents_[i].first, kEvents_[i].second) == hipSuccess) {
+                kTimedMs_ += ms;
+                ++kTimedLaunches_;
+            }
+        }
+        kEventsUsed_ = 0;
+    }
+
+    Dims dims_{};
+    Scaling scaling_{};
+    long long ncells_ = 0;
+    int storage_ = kStoreF32;
+    bool periodicX_ = false, periodicY_ = false;
+    bool zSlipWalls_ = false;
+    std::vector<std::uint8_t> hostFlags_;
+    hipStream_t stream_ = nullptr;
+    bool initialized_ = false;
+    long long devBytes_ = 0;
+
+    void* f_ = nullptr;
+    std::uint8_t* kind_ = nullptr;
+    void* feed_ = nullptr;
+    float *rho_ = nullptr, *u_ = nullptr, *v_ = nullptr, *w_ = nullptr;
+    double* dForce_ = nullptr;
+    double* dMass_ = nullptr;
+    double* dMassPartials_ = nullptr;
+    int massSlots_ = 0;
+    int* dNan_ = nullptr;
+    int* dForceCells_ = nullptr;
+    std::uint32_t* dForceMasks_ = nullptr;
+    double* dForcePartials_ = nullptr;
+    int forceCount_ = 0, forceSlots_ = 0;
+    double* hForce_ = nullptr;
+    int* hNan_ = nullptr;
+    hipEvent_t evT0_ = nullptr, evT1_ = nullptr, evForce_ = nullptr, evNan_ = nullptr;
+
+    int layout_ = kLayoutL;
+    long long steps_ = 0;
+    bool rampActive_ = true, rampEnabled_ = true;
+    long long rampStartStep_ = 0;
+    float cs_ = kSmagorinskyCs;
+    long long gateOpenAtSteps_ = 0;
+    float emaWindowFlowThroughs_ = 1.0f;
+    float emaFx_ = 0, emaFy_ = 0;
+    bool emaSeeded_ = false, forcePending_ = false;
+    long long stepsAtForceLaunch_ = 0, stepsAtLastForceFold_ = 0;
+    Sample ring_[kAvgRingSize]{};
+    int ringHead_ = 0, ringCount_ = 0;
+    double lastRaw_[3] = {0, 0, 0};
+    long long lastRawStep_ = -1;
+    bool nanLatched_ = false, nanPending_ = false;
+    int nanKind_ = 0;
+    int stepsSinceWatchdog_ = 0;
+    bool timingPending_ = false;
+    int timingBatchN_ = 0;
+    double emaStepMs_ = 0.0, lastStepMs_ = 0.0;
+
+    bool kernelTiming_ = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kEvents_;
+    std::size_t kEventsUsed_ = 0;
+    double kTimedMs_ = 0.0;
+    long long kTimedLaunches_ = 0;
+};
+
+void copyErr(const std::string& msg, char* err, int errlen) {
+    if (!err || errlen <= 0) return;
+    const std::size_t n = std::min(msg.size(), static_cast<std::size_t>(errlen - 1));
+    std::memcpy(err, msg.data(), n);
+    err[n] = '\0';
+}
+
+}  // namespace
+}  // namespace fc
+
+struct fc_solver {
+    fc::Solver impl;
+};
+
+using fc::Solver;
+
+extern "C" {
+
+const char* fc_version(void) { return "foilcfd-mi355x 0.1"; }
+
+int fc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int fc_compute_scaling(float chord_m, float airspeed_ms, float nu_m2s, int chord_cells, float u_lat,
+                       fc_scaling* out) {
+    if (!out || chord_cells <= 0 || airspeed_ms == 0.0f) return FC_ERR_INVALID;
+    *out = fc::toC(fc::computeScaling(chord_m, airspeed_ms, nu_m2s, chord_cells, u_lat));
+    return FC_OK;
+}
+
+float fc_max_effective_re(int chord_cells, float u_lat) { return fc::maxEffectiveRe(chord_cells, u_lat); }
+
+int fc_refined_scaling(const fc_scaling* parent, int factor, fc_scaling* out) {
+    if (!parent || !out || factor < 1) return FC_ERR_INVALID;
+    *out = fc::toC(fc::refinedScaling(fc::fromC(*parent), factor));
+    return FC_OK;
+}
+
+float fc_fine_tau_for(float tau_coarse, int factor) { return fc::fineTauFor(tau_coarse, factor); }
+
+float fc_ramped_tau(const fc_scaling* s, long long step, int nx) {
+    return s ? fc::rampedTau(fc::fromC(*s), step, nx) : 0.0f;
+}
+
+float fc_ramped_u(const fc_scaling* s, long long step, int nx) {
+    return s ? fc::rampedU(fc::fromC(*s), step, nx) : 0.0f;
+}
+
+float fc_force_to_coefficient(const fc_scaling* s, float force_lat, int span_cells) {
+    return s ? fc::fromC(*s).forceToCoefficient(force_lat, span_cells) : 0.0f;
+}
+
+float fc_flow_through_steps(const fc_scaling* s, int nx) {
+    return s ? fc::fromC(*s).flowThroughSteps(nx) : 0.0f;
+}
+
+int fc_generate_naca4(const char* digits, int points_per_surface, float* xy, int cap, int* le_index,
+                      char* err, int errlen) {
+    if (!digits) return -1;
+    fc::Airfoil a;
+    std::string why;
+    if (!fc::generateNACA4(digits, points_per_surface, a, &why)) {
+        fc::copyErr(why, err, errlen);
+        return -1;
+    }
+    const int n = static_cast<int>(a.points.size());
+    if (xy) {
+        if (cap < n) {
+            fc::copyErr("output capacity too small", err, errlen);
+            return -1;
+        }
+        for (int i = 0; i < n; ++i) {
+            xy[2 * i] = a.points[i].x;
+            xy[2 * i + 1] = a.points[i].y;
+        }
+    }
+    if (le_index) *le_index = a.leIndex;
+    return n;
+}
+
+int fc_build_boundary_flags(fc_dims dims, uint8_t* flags) {
+    if (!flags || dims.nx < 1 || dims.ny < 1 || dims.nz < 1) return FC_ERR_INVALID;
+    const auto v = fc::buildBoundaryFlags(fc::Dims{dims.nx, dims.ny, dims.nz});
+    std::memcpy(flags, v.data(), v.size());
+    return FC_OK;
+}
+
+int fc_voxelize_airfoil(const float* xy, int npts, float aoa_deg, const fc_layout* layout,
+                        uint8_t* flags) {
+    if (!xy || !layout || !flags || npts < 0) return FC_ERR_INVALID;
+    const fc::DomainLayout l = fc::fromC(*layout);
+    std::vector<std::uint8_t> v(flags, flags + l.dims.cells());
+    fc::voxelizeAirfoil(fc::airfoilFrom(xy, npts), aoa_deg, l, v);
+    std::memcpy(flags, v.data(), v.size());
+    return FC_OK;
+}
+
+int fc_close_trailing_edge_gaps(fc_dims dims, uint8_t* flags) {
+    if (!flags) return -1;
+    const fc::Dims d{dims.nx, dims.ny, dims.nz};
+    std::vector<std::uint8_t> v(flags, flags + d.cells());
+    const int n = fc::closeTrailingEdgeGaps(d, v);
+    std::memcpy(flags, v.data(), v.size());
+    return n;
+}
+
+int fc_build_clean_foil_flags(const float* xy, int npts, float aoa_deg, const fc_layout* layout,
+                              uint8_t* flags) {
+    if (!xy || !layout || !flags || npts < 0) return FC_ERR_INVALID;
+    const fc::DomainLayout l = fc::fromC(*layout);
+    const auto v = fc::buildCleanFoilFlags(fc::airfoilFrom(xy, npts), aoa_deg, l);
+    std::memcpy(flags, v.data(), v.size());
+    return FC_OK;
+}
+
+int fc_make_fine_layout(const fc_layout* coarse, const fc_patch_box* box, int factor, fc_layout* out) {
+    if (!coarse || !box || !out || factor < 1) return FC_ERR_INVALID;
+    const fc::PatchBox b{box->x0, box->y0, box->x1, box->y1};
+    *out = fc::toC(fc::makeFineLayout(fc::fromC(*coarse), b, factor));
+    return FC_OK;
+}
+
+int fc_fine_dims(const fc_patch_box* box, fc_dims coarse, int factor, fc_dims* out) {
+    if (!box || !out || factor < 1) return FC_ERR_INVALID;
+    const fc::Dims d =
+        fc::fineDimsFor(fc::PatchBox{box->x0, box->y0, box->x1, box->y1}, fc::Dims{coarse.nx, coarse.ny, coarse.nz},
+                        factor);
+    *out = fc_dims{d.nx, d.ny, d.nz};
+    return FC_OK;
+}
+
+int fc_stamp_interface_shell(fc_dims dims, uint8_t* flags) {
+    if (!flags) return FC_ERR_INVALID;
+    const fc::Dims d{dims.nx, dims.ny, dims.nz};
+    std::vector<std::uint8_t> v(flags, flags + d.cells());
+    fc::stampInterfaceShell(d, v);
+    std::memcpy(flags, v.data(), v.size());
+    return FC_OK;
+}
+
+int fc_build_fine_patch_flags(const float* xy, int npts, float aoa_deg, const fc_layout* coarse,
+                              const fc_patch_box* box, int factor, uint8_t* flags) {
+    if (!xy || !coarse || !box || !flags || factor < 1) return FC_ERR_INVALID;
+    const auto v = fc::buildFinePatchFlags(fc::airfoilFrom(xy, npts), aoa_deg, fc::fromC(*coarse),
+                                           fc::PatchBox{box->x0, box->y0, box->x1, box->y1}, factor);
+    std::memcpy(flags, v.data(), v.size());
+    return FC_OK;
+}
+
+int fc_derive_patch_box(fc_dims dims, const uint8_t* flags, int margin_x0, int margin_x1, int margin_y0,
+                        int margin_y1, int clearance, fc_patch_box* out) {
+    if (!flags || !out) return FC_ERR_INVALID;
+    const fc::Dims d{dims.nx, dims.ny, dims.nz};
+    const std::vector<std::uint8_t> v(flags, flags + d.cells());
+    const fc::PatchBox b = fc::derivePatchBox(d, v, margin_x0, margin_x1, margin_y0, margin_y1, clearance);
+    *out = fc_patch_box{b.x0, b.y0, b.x1, b.y1};
+    return FC_OK;
+}
+
+fc_solver* fc_create(void) { return new (std::nothrow) fc_solver(); }
+
+void fc_destroy(fc_solver* s) { delete s; }
+
+int fc_init(fc_solver* s, fc_dims dims, const fc_scaling* scaling, const uint8_t* flags,
+            const fc_init_options* options, char* err, int errlen) {
+    if (!s || !scaling) return FC_ERR_INVALID;
+    const int rc = s->impl.init(dims, *scaling, flags, options);
+    if (rc != FC_OK) fc::copyErr(s->impl.error, err, errlen);
+    return rc;
+}
+
+void fc_shutdown(fc_solver* s) {
+    if (s) s->impl.freeAll();
+}
+
+int fc_reset(fc_solver* s) { return s ? s->impl.reset() : FC_ERR_INVALID; }
+int fc_set_flags(fc_solver* s, const uint8_t* flags) { return s ? s->impl.setFlags(flags) : FC_ERR_INVALID; }
+
+int fc_set_startup_ramp(fc_solver* s, int enabled) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.setRampEnabled(enabled != 0);
+    return FC_OK;
+}
+
+int fc_set_force_ema_window(fc_solver* s, float flow_throughs) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.setEmaWindow(flow_throughs);
+    return FC_OK;
+}
+
+int fc_set_smagorinsky_cs(fc_solver* s, float cs) {
+    if (!s || cs < 0.0f) return FC_ERR_INVALID;
+    s->impl.setCs(cs);
+    return FC_OK;
+}
+
+int fc_step(fc_solver* s, int n) { return s ? s->impl.stepN(n) : FC_ERR_INVALID; }
+int fc_synchronize(fc_solver* s) { return s ? s->impl.synchronize() : FC_ERR_INVALID; }
+int fc_forces(fc_solver* s, fc_force_readout* out) { return s ? s->impl.forces(out) : FC_ERR_INVALID; }
+
+int fc_force_raw(fc_solver* s, double* out3, long long* at_step) {
+    return s ? s->impl.forceRaw(out3, at_step) : FC_ERR_INVALID;
+}
+
+int fc_force_now(fc_solver* s, double* out3) { return s ? s->impl.forceNow(out3) : FC_ERR_INVALID; }
+int fc_mass(fc_solver* s, double* out) { return s ? s->impl.mass(out) : FC_ERR_INVALID; }
+
+int fc_read_macro(fc_solver* s, float* rho, float* u, float* v, float* w) {
+    return s ? s->impl.readMacro(rho, u, v, w) : FC_ERR_INVALID;
+}
+
+int fc_read_flags(fc_solver* s, uint8_t* flags) { return s ? s->impl.readFlags(flags) : FC_ERR_INVALID; }
+
+int fc_download_f(fc_solver* s, void* host, int as_f64) {
+    return s ? s->impl.downloadF(host, as_f64 != 0) : FC_ERR_INVALID;
+}
+
+int fc_upload_f(fc_solver* s, const void* host, int is_f64) {
+    return s ? s->impl.uploadF(host, is_f64 != 0) : FC_ERR_INVALID;
+}
+
+int fc_notify_snapshot_restored(fc_solver* s, int full_state) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.notifySnapshotRestored(full_state != 0);
+    return FC_OK;
+}
+
+int fc_nan_detected(fc_solver* s) { return s ? s->impl.nanDetected() : 0; }
+int fc_nan_kind(fc_solver* s) { return s ? s->impl.nanKind() : 0; }
+int fc_get_perf_stats(fc_solver* s, fc_perf_stats* out) { return s ? s->impl.perfStats(out) : FC_ERR_INVALID; }
+long long fc_step_count(fc_solver* s) { return s ? s->impl.stepCount() : 0; }
+float fc_current_tau(fc_solver* s) { return s ? s->impl.currentTau() : 0.0f; }
+float fc_flow_throughs_completed(fc_solver* s) { return s ? s->impl.flowThroughsCompleted() : 0.0f; }
+
+int fc_get_dims(fc_solver* s, fc_dims* out) {
+    if (!s || !out) return FC_ERR_INVALID;
+    const fc::Dims d = s->impl.dims();
+    *out = fc_dims{d.nx, d.ny, d.nz};
+    return FC_OK;
+}
+
+int fc_get_scaling(fc_solver* s, fc_scaling* out) {
+    if (!s || !out) return FC_ERR_INVALID;
+    *out = fc::toC(s->impl.scaling());
+    return FC_OK;
+}
+
+int fc_storage(fc_solver* s) { return s ? s->impl.storage() : -1; }
+
+int fc_device_fields(fc_solver* s, void** rho, void** u, void** v, void** w, void** flags) {
+    if (!s || !s->impl.initialized()) return FC_ERR_STATE;
+    s->impl.deviceFields(rho, u, v, w, flags);
+    return FC_OK;
+}
+
+long long fc_device_bytes(fc_solver* s) { return s ? s->impl.deviceBytes() : 0; }
+
+int fc_set_kernel_timing(fc_solver* s, int enabled) {
+    return s ? s->impl.setKernelTiming(enabled != 0) : FC_ERR_INVALID;
+}
+
+int fc_kernel_timing(fc_solver* s, int* launches, double* total_ms) {
+    return s ? s->impl.kernelTiming(launches, total_ms) : FC_ERR_INVALID;
+}
+
+int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms) {
+    return s ? s->impl.timeSteps(n, steps_per_call, ms) : FC_ERR_INVALID;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""Python mirror of the reference's solver interface over the C ABI.
+
+``LBMSolver`` follows ``class LBMSolver`` (src/sim/lbm_solver.h:133-514):
+init -> [reset] -> step_n, force / field readouts, exact-restore upload.
+Units follow src/sim/units.h and the flag builders src/geom/voxelizer.h.
+All device work happens in the HIP library; this module only marshals
+host arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FcDims, FcForceReadout, FcInitOptions, FcLayout, FcPatchBox, FcPerfStats,
+                   FcScaling, check, load)
+
+Q = 19
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0], dtype=np.int64)
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1], dtype=np.int64)
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1], dtype=np.int64)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+FLUID, SOLID, INLET, OUTLET = 0, 1, 2, 3
+SLIP_TOP, SLIP_BOTTOM, SLIP_FRONT, SLIP_BACK, INTERFACE = 4, 5, 6, 7, 8
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _f32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ---------------------------------------------------------------- units ----
+def compute_scaling(chord_m=1.2, airspeed_ms=30.0, nu_m2s=1.48e-5, chord_cells=256,
+                    u_lat=0.08) -> FcScaling:
+    """computeScaling (units.h:192-219)."""
+    s = FcScaling()
+    check(load().fc_compute_scaling(chord_m, airspeed_ms, nu_m2s, chord_cells, u_lat,
+                                    C.byref(s)), "fc_compute_scaling")
+    return s
+
+
+def scaling_for_tau(tau: float, u_lat: float, chord_cells: int) -> FcScaling:
+    """A self-consistent scaling whose lattice viscosity gives exactly ``tau``
+    (chord 1 m, airspeed = u_lat m/s so dx = 1/Nc and dt = dx)."""
+    nu_lat = (tau - 0.5) / 3.0
+    dx = 1.0 / chord_cells
+    nu_phys = nu_lat * dx  # nu = nu_lat dx^2 / dt with dt = dx
+    s = compute_scaling(1.0, u_lat, nu_phys, chord_cells, u_lat)
+    s.tau = tau
+    s.nu_lat = nu_lat
+    s.nu_lat_target = nu_lat
+    return s
+
+
+def refined_scaling(parent: FcScaling, factor: int) -> FcScaling:
+    out = FcScaling()
+    check(load().fc_refined_scaling(C.byref(parent), factor, C.byref(out)), "fc_refined_scaling")
+    return out
+
+
+def ramped_tau(s: FcScaling, step: int, nx: int) -> float:
+    return load().fc_ramped_tau(C.byref(s), step, nx)
+
+
+def ramped_u(s: FcScaling, step: int, nx: int) -> float:
+    return load().fc_ramped_u(C.byref(s), step, nx)
+
+
+def force_to_coefficient(s: FcScaling, force: float, span: int) -> float:
+    return load().fc_force_to_coefficient(C.byref(s), force, span)
+
+
+# ------------------------------------------------------------- geometry ----
+def generate_naca4(digits: str, points_per_surface: int = 200):
+    """generateNACA4 (airfoil.cpp:177-266) -> (points (n,2) float32, le_index)."""
+    lib = load()
+    cap = 2 * max(points_per_surface, 20)
+    xy = np.zeros((cap, 2), dtype=np.float32)
+    le = C.c_int(0)
+    err = C.create_string_buffer(256)
+    n = lib.fc_generate_naca4(digits.encode(), points_per_surface, _f32(xy), cap, C.byref(le),
+                              err, 256)
+    if n < 0:
+        raise ValueError(err.value.decode())
+    return xy[:n].copy(), le.value
+
+
+def make_layout(nx, ny, nz, chord_cells, anchor_x_frac=0.3, anchor_y_frac=0.5,
+                anchor_x_cells=-1.0, anchor_y_cells=-1.0, anchor_absolute=False) -> FcLayout:
+    return FcLayout(FcDims(nx, ny, nz), chord_cells, anchor_x_frac, anchor_y_frac,
+                    anchor_x_cells, anchor_y_cells, 1 if anchor_absolute else 0)
+
+
+def build_boundary_flags(nx, ny, nz) -> np.ndarray:
+    out = np.zeros(nx * ny * nz, dtype=np.uint8)
+    check(load().fc_build_boundary_flags(FcDims(nx, ny, nz), _u8(out)), "fc_build_boundary_flags")
+    return out
+
+
+def voxelize_airfoil(points: np.ndarray, aoa_deg: float, layout: FcLayout,
+                     flags: np.ndarray) -> None:
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    check(load().fc_voxelize_airfoil(_f32(pts), len(pts), aoa_deg, C.byref(layout), _u8(flags)),
+          "fc_voxelize_airfoil")
+
+
+def close_trailing_edge_gaps(nx, ny, nz, flags: np.ndarray) -> int:
+    return load().fc_close_trailing_edge_gaps(FcDims(nx, ny, nz), _u8(flags))
+
+
+def build_clean_foil_flags(points: np.ndarray, aoa_deg: float, layout: FcLayout) -> np.ndarray:
+    d = layout.dims
+    out = np.zeros(d.nx * d.ny * d.nz, dtype=np.uint8)
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    check(load().fc_build_clean_foil_flags(_f32(pts), len(pts), aoa_deg, C.byref(layout),
+                                           _u8(out)), "fc_build_clean_foil_flags")
+    return out
+
+
+def derive_patch_box(nx, ny, nz, flags, mx0, mx1, my0, my1, clearance=3) -> FcPatchBox:
+    b = FcPatchBox()
+    check(load().fc_derive_patch_box(FcDims(nx, ny, nz), _u8(flags), mx0, mx1, my0, my1,
+                                     clearance, C.byref(b)), "fc_derive_patch_box")
+    return b
+
+
+def make_fine_layout(coarse: FcLayout, box: FcPatchBox, factor: int) -> FcLayout:
+    out = FcLayout()
+    check(load().fc_make_fine_layout(C.byref(coarse), C.byref(box), factor, C.byref(out)),
+          "fc_make_fine_layout")
+    return out
+
+
+def build_fine_patch_flags(points, aoa_deg, coarse: FcLayout, box: FcPatchBox,
+                           factor: int) -> np.ndarray:
+    fl = make_fine_layout(coarse, box, factor)
+    d = fl.dims
+    out = np.zeros(d.nx * d.ny * d.nz, dtype=np.uint8)
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    check(load().fc_build_fine_patch_flags(_f32(pts), len(pts), aoa_deg, C.byref(coarse),
+                                           C.byref(box), factor, _u8(out)),
+          "fc_build_fine_patch_flags")
+    return out
+
+
+def stamp_interface_shell(nx, ny, nz, flags: np.ndarray) -> None:
+    check(load().fc_stamp_interface_shell(FcDims(nx, ny, nz), _u8(flags)),
+          "fc_stamp_interface_shell")
+
+
+# --------------------------------------------------------------- solver ----
+@dataclass
+class ForceReadout:
+    cl: float
+    cd: float
+    lift_to_drag: float
+    valid: bool
+    flow_throughs: float
+    cl_avg: float
+    cd_avg: float
+
+
+class LBMSolver:
+    """Owner of one device lattice (reference LBMSolver)."""
+
+    def __init__(self):
+        self._lib = load()
+        self._h = self._lib.fc_create()
+        if not self._h:
+            raise MemoryError("fc_create failed")
+        self.dims = None
+        self.ncells = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.fc_destroy(h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    # lifecycle -------------------------------------------------------------
+    def init(self, dims, scaling: FcScaling, flags: np.ndarray, storage="f32",
+             periodic_xy=False) -> None:
+        nx, ny, nz = dims
+        flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        if flags.size != nx * ny * nz:
+            raise ValueError("flag field size does not match grid dimensions")
+        st = _abi.STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
+        opt = FcInitOptions(st, 1 if periodic_xy else 0, 1 if periodic_xy else 0)
+        err = C.create_string_buffer(512)
+        rc = self._lib.fc_init(self._h, FcDims(nx, ny, nz), C.byref(scaling), _u8(flags),
+                               C.byref(opt), err, 512)
+        check(rc, "fc_init", err.value.decode())
+        self.dims = (nx, ny, nz)
+        self.ncells = nx * ny * nz
+        self.storage = st
+
+    def reset(self):
+        check(self._lib.fc_reset(self._h), "fc_reset")
+
+    def set_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        check(self._lib.fc_set_flags(self._h, _u8(f)), "fc_set_flags")
+
+    def set_startup_ramp_enabled(self, on: bool):
+        check(self._lib.fc_set_startup_ramp(self._h, 1 if on else 0), "fc_set_startup_ramp")
+
+    def set_smagorinsky_cs(self, cs: float):
+        check(self._lib.fc_set_smagorinsky_cs(self._h, cs), "fc_set_smagorinsky_cs")
+
+    def set_force_ema_window(self, ft: float):
+        check(self._lib.fc_set_force_ema_window(self._h, ft), "fc_set_force_ema_window")
+
+    # stepping ---------------------------------------------------------------
+    def step_n(self, n: int):
+        check(self._lib.fc_step(self._h, int(n)), "fc_step")
+
+    def synchronize(self):
+        check(self._lib.fc_synchronize(self._h), "fc_synchronize")
+
+    # readouts ---------------------------------------------------------------
+    def forces(self) -> ForceReadout:
+        r = FcForceReadout()
+        check(self._lib.fc_forces(self._h, C.byref(r)), "fc_forces")
+        return ForceReadout(r.cl, r.cd, r.lift_to_drag, bool(r.valid), r.flow_throughs,
+                            r.cl_avg, r.cd_avg)
+
+    def force_raw(self):
+        out = (C.c_double * 3)()
+        step = C.c_longlong(0)
+        check(self._lib.fc_force_raw(self._h, out, C.byref(step)), "fc_force_raw")
+        return np.array(out[:]), step.value
+
+    def force_now(self) -> np.ndarray:
+        out = (C.c_double * 3)()
+        check(self._lib.fc_force_now(self._h, out), "fc_force_now")
+        return np.array(out[:])
+
+    def mass(self) -> float:
+        out = C.c_double(0)
+        check(self._lib.fc_mass(self._h, C.byref(out)), "fc_mass")
+        return out.value
+
+    def read_macro(self):
+        n = self.ncells
+        rho, u, v, w = (np.empty(n, dtype=np.float32) for _ in range(4))
+        check(self._lib.fc_read_macro(self._h, _f32(rho), _f32(u), _f32(v), _f32(w)),
+              "fc_read_macro")
+        return rho, u, v, w
+
+    def download_f(self, f64=False) -> np.ndarray:
+        out = np.empty(Q * self.ncells, dtype=np.float64 if f64 else np.float32)
+        check(self._lib.fc_download_f(self._h, out.ctypes.data_as(C.c_void_p), 1 if f64 else 0),
+              "fc_download_f")
+        return out
+
+    def upload_f(self, f: np.ndarray):
+        f = np.ascontiguousarray(f).reshape(-1)
+        if f.size != Q * self.ncells:
+            raise ValueError("population array size mismatch")
+        is64 = f.dtype == np.float64
+        if not is64:
+            f = f.astype(np.float32, copy=False)
+        check(self._lib.fc_upload_f(self._h, f.ctypes.data_as(C.c_void_p), 1 if is64 else 0),
+              "fc_upload_f")
+
+    def nan_detected(self) -> bool:
+        return bool(self._lib.fc_nan_detected(self._h))
+
+    def nan_kind(self) -> int:
+        return self._lib.fc_nan_kind(self._h)
+
+    def perf_stats(self) -> FcPerfStats:
+        p = FcPerfStats()
+        check(self._lib.fc_get_perf_stats(self._h, C.byref(p)), "fc_get_perf_stats")
+        return p
+
+    def step_count(self) -> int:
+        return self._lib.fc_step_count(self._h)
+
+    def current_tau(self) -> float:
+        return self._lib.fc_current_tau(self._h)
+
+    def device_bytes(self) -> int:
+        return self._lib.fc_device_bytes(self._h)
+
+    # measurement --------------------------------------------------------------
+    def set_kernel_timing(self, on: bool):
+        check(self._lib.fc_set_kernel_timing(self._h, 1 if on else 0), "fc_set_kernel_timing")
+
+    def kernel_timing(self):
+        n = C.c_int(0)
+        ms = C.c_double(0)
+        check(self._lib.fc_kernel_timing(self._h, C.byref(n), C.byref(ms)), "fc_kernel_timing")
+        return n.value, ms.value
+
+    def time_steps(self, calls: int, steps_per_call: int) -> float:
+        ms = C.c_double(0)
+        check(self._lib.fc_time_steps(self._h, calls, steps_per_call, C.byref(ms)),
+              "fc_time_steps")
+        return ms.value

@@ -1,0 +1,218 @@
+/*
+ * FoilCFD MI355X solver core -- C ABI.
+ *
+ * The drop-in boundary behind the reference's src/sim solver interface
+ * (class LBMSolver, src/sim/lbm_solver.h:133-514), its unit conversions
+ * (src/sim/units.h), the flag builders it is fed from (src/geom/voxelizer.h,
+ * src/geom/airfoil.h) and the particle entry points (src/render/particles.cuh).
+ * Plain C: opaque handle, raw host pointers and sizes, integer status codes,
+ * no exceptions across the boundary, no HIP or torch types in the signatures.
+ *
+ * Index convention for every host array: cell = x + nx*(y + ny*z), x fastest
+ * (reference GridDims, src/sim/lbm_core.cuh:137-158); population arrays are
+ * canonical SoA f[q*cells + cell] in the frozen D3Q19 ordering
+ * (src/sim/lbm_core.cuh:20-68), independent of the device storage format.
+ */
+#ifndef FOILCFD_H
+#define FOILCFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FC_OK 0
+#define FC_ERR_INVALID 1   /* bad argument (reference: cudaErrorInvalidValue) */
+#define FC_ERR_ALLOC 2     /* device / pinned allocation failed */
+#define FC_ERR_DEVICE 3    /* a HIP runtime call or launch failed */
+#define FC_ERR_STATE 4     /* call not valid in the current state (e.g. before init) */
+
+/* ---- population storage formats --------------------------------------- */
+#define FC_STORAGE_F32 0   /* FP32 shifted deviations f - w */
+#define FC_STORAGE_F16 1   /* IEEE FP16 of 2^15 (f - w); FP32 arithmetic */
+#define FC_STORAGE_BF16 2  /* BF16 of (f - w); FP32 arithmetic */
+#define FC_STORAGE_F64 3   /* FP64 storage and arithmetic (reference-quality runs) */
+
+/* ---- plain data ---------------------------------------------------------- */
+typedef struct fc_dims { int nx, ny, nz; } fc_dims;  /* GridDims, lbm_core.cuh:139-158 */
+
+/* LatticeScaling, units.h:115-174 */
+typedef struct fc_scaling {
+    float chord_m, airspeed_ms, nu_m2s;
+    int chord_cells;
+    float u_lat;
+    float dx, dt;
+    float nu_lat_target, nu_lat, tau;
+    int tau_clamped;
+    float re_target, re_effective;
+} fc_scaling;
+
+/* DomainLayout, voxelizer.h:57-85 */
+typedef struct fc_layout {
+    fc_dims dims;
+    int chord_cells;
+    float anchor_x_frac, anchor_y_frac;
+    float anchor_x_cells, anchor_y_cells;
+    int anchor_absolute;
+} fc_layout;
+
+/* PatchBox, lbm_refine.cuh:54-61 (coarse cells, [x0,x1) x [y0,y1) x full z) */
+typedef struct fc_patch_box { int x0, y0, x1, y1; } fc_patch_box;
+
+/* ForceReadout, lbm_solver.h:28-52 */
+typedef struct fc_force_readout {
+    float cl, cd, lift_to_drag;
+    int valid;
+    float flow_throughs;
+    float cl_avg, cl_min, cl_max, cl_median;
+    float cd_avg, cd_min, cd_max, cd_median;
+    float ld_avg, ld_min, ld_max, ld_median;
+} fc_force_readout;
+
+/* SolverPerfStats, lbm_solver.h:68-74 (+ bytes moved per step) */
+typedef struct fc_perf_stats {
+    int last_steps_per_frame;
+    double last_step_ms;
+    double mlups;
+    double population_bytes_per_step;
+} fc_perf_stats;
+
+/* Options of fc_init (extensions over the reference's init, which has
+ * neither a storage choice nor periodic x / y). */
+typedef struct fc_init_options {
+    int storage;     /* FC_STORAGE_* */
+    int periodic_x;  /* wrap x (fully periodic boxes; reference: never) */
+    int periodic_y;  /* wrap y */
+} fc_init_options;
+
+/* ---- library ------------------------------------------------------------- */
+const char* fc_version(void);
+/* Number of HIP devices visible (0 without a GPU); no other side effect. */
+int fc_device_count(void);
+
+/* ---- units (src/sim/units.h) ----------------------------------------------- */
+/* computeScaling, units.h:192-219 */
+int fc_compute_scaling(float chord_m, float airspeed_ms, float nu_m2s, int chord_cells,
+                       float u_lat, fc_scaling* out);
+/* maxEffectiveRe, units.h:224-227 */
+float fc_max_effective_re(int chord_cells, float u_lat);
+/* refinedScaling, units.h:242-254 */
+int fc_refined_scaling(const fc_scaling* parent, int factor, fc_scaling* out);
+/* fineTauFor, lbm_refine.cuh:99-101 */
+float fc_fine_tau_for(float tau_coarse, int factor);
+/* rampedTau / rampedU, units.h:274-283 / :310-318 */
+float fc_ramped_tau(const fc_scaling* s, long long step, int nx);
+float fc_ramped_u(const fc_scaling* s, long long step, int nx);
+/* LatticeScaling::forceToCoefficient / flowThroughSteps, units.h:158-173 */
+float fc_force_to_coefficient(const fc_scaling* s, float force_lat, int span_cells);
+float fc_flow_through_steps(const fc_scaling* s, int nx);
+
+/* ---- geometry -> flags (src/geom/airfoil.h, src/geom/voxelizer.h) -------- */
+/* generateNACA4, airfoil.h:73. Writes up to cap (x,y) pairs into xy; returns
+ * the point count (2*n-1), or -1 with a reason in err. */
+int fc_generate_naca4(const char* digits, int points_per_surface, float* xy, int cap,
+                      int* le_index, char* err, int errlen);
+/* buildBoundaryFlags, voxelizer.h:97 (flags: dims cells bytes) */
+int fc_build_boundary_flags(fc_dims dims, uint8_t* flags);
+/* voxelizeAirfoil, voxelizer.h:115 (xy: npts (x,y) pairs of a Selig loop) */
+int fc_voxelize_airfoil(const float* xy, int npts, float aoa_deg, const fc_layout* layout,
+                        uint8_t* flags);
+/* closeTrailingEdgeGaps, voxelizer.h:125; returns the closed-cell count */
+int fc_close_trailing_edge_gaps(fc_dims dims, uint8_t* flags);
+/* buildCleanFoilFlags, voxelizer.h:135 */
+int fc_build_clean_foil_flags(const float* xy, int npts, float aoa_deg, const fc_layout* layout,
+                              uint8_t* flags);
+/* makeFineLayout / fineDimsFor / stampInterfaceShell / buildFinePatchFlags /
+ * derivePatchBox, voxelizer.h:155-199, lbm_refine.cuh:87-94 */
+int fc_make_fine_layout(const fc_layout* coarse, const fc_patch_box* box, int factor,
+                        fc_layout* out);
+int fc_fine_dims(const fc_patch_box* box, fc_dims coarse, int factor, fc_dims* out);
+int fc_stamp_interface_shell(fc_dims dims, uint8_t* flags);
+int fc_build_fine_patch_flags(const float* xy, int npts, float aoa_deg, const fc_layout* coarse,
+                              const fc_patch_box* box, int factor, uint8_t* flags);
+int fc_derive_patch_box(fc_dims dims, const uint8_t* flags, int margin_x0, int margin_x1,
+                        int margin_y0, int margin_y1, int clearance, fc_patch_box* out);
+
+/* ---- solver lifecycle (LBMSolver, lbm_solver.h:133-514) ------------------ */
+typedef struct fc_solver fc_solver;
+
+fc_solver* fc_create(void);                 /* LBMSolver::LBMSolver */
+void fc_destroy(fc_solver* s);              /* LBMSolver::~LBMSolver */
+/* LBMSolver::init (lbm_solver.h:152-154): allocate, upload flags, reset.
+ * flags: dims cells bytes (CellFlag values). options may be NULL (FP32,
+ * z-periodic only). Fails with a reason in err, leaving nothing allocated. */
+int fc_init(fc_solver* s, fc_dims dims, const fc_scaling* scaling, const uint8_t* flags,
+            const fc_init_options* options, char* err, int errlen);
+void fc_shutdown(fc_solver* s);             /* LBMSolver::shutdown */
+int fc_reset(fc_solver* s);                 /* LBMSolver::reset */
+int fc_set_flags(fc_solver* s, const uint8_t* flags);  /* LBMSolver::setFlags */
+int fc_set_startup_ramp(fc_solver* s, int enabled);    /* setStartupRampEnabled, :269 */
+int fc_set_force_ema_window(fc_solver* s, float flow_throughs);  /* setForceEmaWindow, :490 */
+/* Smagorinsky constant (reference: fixed 0.1733 inside stepN, lbm_solver.cpp:1148);
+ * 0 disables the eddy-viscosity term. Default 0.1733. */
+int fc_set_smagorinsky_cs(fc_solver* s, float cs);
+
+/* ---- stepping ------------------------------------------------------------ */
+/* LBMSolver::stepN (lbm_solver.h:354): n fused steps with the startup ramps,
+ * watchdog every 200 steps, one force sample + wall-model update per call.
+ * Asynchronous on the solver's stream. */
+int fc_step(fc_solver* s, int n);
+/* Wait for the solver's stream and fold finished readbacks. */
+int fc_synchronize(fc_solver* s);
+
+/* ---- readouts ------------------------------------------------------------ */
+int fc_forces(fc_solver* s, fc_force_readout* out);   /* LBMSolver::forces, :371 */
+/* Last folded raw momentum-exchange sample (lattice units, FP64 sum) and the
+ * step it was taken at. */
+int fc_force_raw(fc_solver* s, double* out3, long long* at_step);
+/* Synchronous momentum-exchange force of the current state (launchForceReduction). */
+int fc_force_now(fc_solver* s, double* out3);
+/* Synchronous FP64 global mass sum over Fluid cells of sum_q f_q. */
+int fc_mass(fc_solver* s, double* out);
+/* Host copies of the macroscopic fields (velocityField/deviceRho, :425-429);
+ * written on the last step of each fc_step call. Any pointer may be NULL. */
+int fc_read_macro(fc_solver* s, float* rho, float* u, float* v, float* w);
+/* Host copy of the cell flags (hostFlags, :485). */
+int fc_read_flags(fc_solver* s, uint8_t* flags);
+/* Canonical post-collision populations (the unpadded FP32/FP64 form of
+ * activeDeviceF, :454-458). Populations of cells that do not own their
+ * storage (solid / slip / interface) are unspecified; slots that left a
+ * non-periodic domain read NaN. */
+int fc_download_f(fc_solver* s, void* host, int as_f64);
+/* Replace the state (tests' uploadF, tests/lbm_test_util.h:116-147) and run
+ * the exact-restore bookkeeping (notifySnapshotRestored(true), :465). */
+int fc_upload_f(fc_solver* s, const void* host, int is_f64);
+int fc_notify_snapshot_restored(fc_solver* s, int full_state);
+int fc_nan_detected(fc_solver* s);          /* nanDetected, :411 (1 = latched) */
+int fc_nan_kind(fc_solver* s);              /* watchdog verdict of the trip */
+int fc_get_perf_stats(fc_solver* s, fc_perf_stats* out);   /* perfStats, :419 */
+long long fc_step_count(fc_solver* s);      /* stepCount, :501 */
+float fc_current_tau(fc_solver* s);         /* currentTau, :507 */
+float fc_flow_throughs_completed(fc_solver* s);  /* flowThroughsCompleted, :504 */
+int fc_get_dims(fc_solver* s, fc_dims* out);     /* dims, :495 */
+int fc_get_scaling(fc_solver* s, fc_scaling* out);  /* scaling, :498 */
+int fc_storage(fc_solver* s);
+/* Device pointers of the macro fields and flags (velocityField, deviceRho,
+ * deviceFlags): valid until the next init/shutdown. */
+int fc_device_fields(fc_solver* s, void** rho, void** u, void** v, void** w, void** flags);
+/* Bytes of device memory owned by the handle. */
+long long fc_device_bytes(fc_solver* s);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Per-launch HIP-event timing of the stream-collide kernel (on the solver's
+ * stream); fc_kernel_timing returns launches and summed milliseconds since
+ * the last enable, after synchronizing. */
+int fc_set_kernel_timing(fc_solver* s, int enabled);
+int fc_kernel_timing(fc_solver* s, int* launches, double* total_ms);
+/* Wall time of n fc_step calls of one step each, bracketed by HIP events on
+ * the solver's stream (ms). */
+int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOILCFD_H */

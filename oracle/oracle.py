@@ -1,0 +1,230 @@
+"""FoilCFD CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of oracle/liboracle.so (the CPU restatement of the
+reference step, lbm_oracle.cpp) and of oracle/_ref/libfcref.so (the
+reference's own units.h + NACA generator, built only where /root/reference
+exists). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg may import this module; the product never does.
+
+Parity status of the oracle functions (what pins each one) is listed in
+DESIGN.md; every function here checked by at least one known answer is
+exercised in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libfcref.so")
+
+Q = 19
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0], dtype=np.int64)
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1], dtype=np.int64)
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1], dtype=np.int64)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+_lib = None
+_ref = None
+
+_vp = C.c_void_p
+_u8p = C.POINTER(C.c_uint8)
+_u16p = C.POINTER(C.c_uint16)
+_u32p = C.POINTER(C.c_uint32)
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"oracle not built: {ORACLE_LIB} (run `make -C oracle`)")
+        L = C.CDLL(ORACLE_LIB)
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_half_to_float.restype = C.c_float
+        L.orc_half_to_float.argtypes = [C.c_uint16]
+        L.orc_equilibrium_f64.argtypes = [C.c_double] * 4 + [_dp]
+        L.orc_equilibrium_f32.argtypes = [C.c_float] * 4 + [_fp]
+        L.orc_init_equilibrium_f32.argtypes = [_fp, _u8p, C.c_int, C.c_int, C.c_int, C.c_float]
+        L.orc_init_equilibrium_f64.argtypes = [_dp, _u8p, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_spanwise_perturb_f32.argtypes = [_fp, _u8p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                               C.c_uint32]
+        L.orc_spanwise_perturb_f64.argtypes = [_dp, _u8p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                               C.c_uint32]
+        L.orc_speck_phase_bits.restype = C.c_uint32
+        L.orc_speck_phase_bits.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_smag_prefactor.restype = C.c_float
+        L.orc_smag_prefactor.argtypes = [C.c_float]
+        common = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_stream_collide_f32.argtypes = [_fp, _fp] + common + [C.c_float] * 4 + [C.c_int] + \
+            [_fp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
+        L.orc_stream_collide_f64.argtypes = [_dp, _dp] + common + [C.c_double] * 4 + [C.c_int] + \
+            [_dp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
+        L.orc_run_f32.argtypes = [_fp, _fp] + common + [C.c_int, _fp, _fp, C.c_float, C.c_float] + \
+            [_fp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
+        L.orc_run_f64.argtypes = [_dp, _dp] + common + [C.c_int, _dp, _dp, C.c_double,
+                                                        C.c_double] + \
+            [_dp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
+        L.orc_force_f32.argtypes = [_fp] + common + [_u16p] * 3 + [_dp]
+        L.orc_force_f64.argtypes = [_dp] + common + [_u16p] * 3 + [_dp]
+        L.orc_watchdog_f32.restype = C.c_int
+        L.orc_watchdog_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_int]
+        L.orc_moments_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
+        L.orc_moments_f64.argtypes = [_dp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own units.h / generateNACA4 (None when not built)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_LIB):
+            return None
+        R = C.CDLL(REF_LIB)
+        sc = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_float]
+        R.ref_compute_scaling.argtypes = sc + [_fp]
+        R.ref_refined_scaling.argtypes = sc + [C.c_int, _fp]
+        R.ref_ramped_tau.restype = C.c_float
+        R.ref_ramped_tau.argtypes = sc + [C.c_longlong, C.c_int]
+        R.ref_ramped_u.restype = C.c_float
+        R.ref_ramped_u.argtypes = sc + [C.c_longlong, C.c_int]
+        R.ref_max_effective_re.restype = C.c_float
+        R.ref_max_effective_re.argtypes = [C.c_int, C.c_float]
+        R.ref_force_to_coefficient.restype = C.c_float
+        R.ref_force_to_coefficient.argtypes = sc + [C.c_float, C.c_int]
+        R.ref_generate_naca4.restype = C.c_int
+        R.ref_generate_naca4.argtypes = [C.c_char_p, C.c_int, _fp, C.c_int, C.POINTER(C.c_int)]
+        _ref = R
+    return _ref
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _ptr_for(a):
+    return _p(a, _dp if a.dtype == np.float64 else _fp)
+
+
+def equilibrium(rho, ux, uy, uz, f64=True):
+    out = np.zeros(Q, dtype=np.float64 if f64 else np.float32)
+    if f64:
+        lib().orc_equilibrium_f64(rho, ux, uy, uz, _p(out, _dp))
+    else:
+        lib().orc_equilibrium_f32(rho, ux, uy, uz, _p(out, _fp))
+    return out
+
+
+def init_equilibrium(flags, dims, u_inlet, f64=False):
+    nx, ny, nz = dims
+    f = np.zeros(Q * nx * ny * nz, dtype=np.float64 if f64 else np.float32)
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    if f64:
+        lib().orc_init_equilibrium_f64(_p(f, _dp), _p(fl, _u8p), nx, ny, nz, u_inlet)
+    else:
+        lib().orc_init_equilibrium_f32(_p(f, _fp), _p(fl, _u8p), nx, ny, nz, u_inlet)
+    return f
+
+
+def spanwise_perturb(f, flags, dims, amplitude, seed=0x5EED):
+    nx, ny, nz = dims
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    if f.dtype == np.float64:
+        lib().orc_spanwise_perturb_f64(_p(f, _dp), _p(fl, _u8p), nx, ny, nz, amplitude, seed)
+    else:
+        lib().orc_spanwise_perturb_f32(_p(f, _fp), _p(fl, _u8p), nx, ny, nz, amplitude, seed)
+
+
+def smag_prefactor(cs):
+    return lib().orc_smag_prefactor(cs)
+
+
+def run(f, flags, dims, taus, uins, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False,
+        slip=None, qfrac=None, ffmask=None, macro=True):
+    """Advance ``f`` (canonical SoA, modified in place) by len(taus) steps of the
+    reference step sequence; returns (rho, u, v, w) of the last step."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    f64 = f.dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    tmp = np.empty_like(f)
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    taus = np.ascontiguousarray(taus, dtype=dt)
+    uins = np.ascontiguousarray(uins, dtype=dt)
+    m = [np.zeros(n, dtype=dt) for _ in range(4)] if macro else [None] * 4
+    uw = [None] * 3 if slip is None else [np.ascontiguousarray(s, dtype=np.uint16) for s in slip]
+    pre = float(smag_prefactor(cs))
+    fn = lib().orc_run_f64 if f64 else lib().orc_run_f32
+    pt = _dp if f64 else _fp
+    fn(_p(f, pt), _p(tmp, pt), _p(fl, _u8p), nx, ny, nz, int(periodic_xy), int(periodic_xy),
+       len(taus), _p(taus, pt), _p(uins, pt), lam, pre,
+       *[_p(a, pt) for a in m], *[_p(a, _u16p) for a in uw],
+       _p(None if qfrac is None else np.ascontiguousarray(qfrac, np.uint8), _u8p),
+       _p(None if ffmask is None else np.ascontiguousarray(ffmask, np.uint32), _u32p))
+    return tuple(m)
+
+
+def force(f, flags, dims, periodic_xy=False, slip=None):
+    nx, ny, nz = dims
+    out = np.zeros(3, dtype=np.float64)
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    uw = [None] * 3 if slip is None else [np.ascontiguousarray(s, dtype=np.uint16) for s in slip]
+    fn = lib().orc_force_f64 if f.dtype == np.float64 else lib().orc_force_f32
+    fn(_ptr_for(f), _p(fl, _u8p), nx, ny, nz, int(periodic_xy), int(periodic_xy),
+       *[_p(a, _u16p) for a in uw], _p(out, _dp))
+    return out
+
+
+def watchdog(f, dims, offset, flag_base=0):
+    nx, ny, nz = dims
+    return lib().orc_watchdog_f32(_p(np.ascontiguousarray(f, np.float32), _fp), nx, ny, nz,
+                                  offset, flag_base)
+
+
+def moments(f, dims):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    out = [np.zeros(n) for _ in range(4)]
+    fn = lib().orc_moments_f64 if f.dtype == np.float64 else lib().orc_moments_f32
+    fn(_ptr_for(f), nx, ny, nz, *[_p(a, _dp) for a in out])
+    return tuple(out)
+
+
+def set_threads(n):
+    lib().orc_set_num_threads(int(n))
+
+
+def threads():
+    return lib().orc_num_threads()
+
+
+# ---- ramps, restated in FP32 for the step schedules the oracle runs with ----
+def ramp_schedule(tau, nu_lat, u_lat, nx, steps, start=0, ramp=True):
+    """Per-step (tau, u_in) of the solver's startup ramps (units.h:274-318)
+    in FP32, as the solver computes them."""
+    f32 = np.float32
+    taus = np.empty(steps, dtype=np.float32)
+    uins = np.empty(steps, dtype=np.float32)
+    ramp_steps = 2 * nx
+    vramp = int(f32(0.5) * f32(2.0) * f32(nx))
+    for i in range(steps):
+        s = start + i
+        if not ramp or s >= ramp_steps:
+            taus[i] = tau
+        else:
+            t = f32(s) / f32(ramp_steps)
+            nu_now = f32(nu_lat) * (f32(4.0) + t * (f32(1.0) - f32(4.0)))
+            taus[i] = f32(3.0) * nu_now + f32(0.5)
+        if not ramp or vramp <= 0 or s >= vramp:
+            uins[i] = u_lat
+        else:
+            t = f32(s) / f32(vramp)
+            sm = t * t * (f32(3.0) - f32(2.0) * t)
+            uins[i] = f32(u_lat) * sm
+    return taus, uins

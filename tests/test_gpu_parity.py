@@ -1,0 +1,269 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same inputs, plus size-independent properties at the full workload size.
+
+Tolerances: FP32 fields within 1e-5 relative L2 of the FP32
+oracle after the stated step counts; FP64 storage within 1e-12; FP16 / BF16
+shifted storage stated against the FP64 run of the same case.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import foil_cfd_amd as fc
+from foil_cfd_amd import cases
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+Q = fc.Q
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a, b = a[..., mask], b[..., mask]
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def schedule(case, steps):
+    s = case.scaling
+    return orc.ramp_schedule(s.tau, s.nu_lat, s.u_lat, case.dims[0], steps, ramp=case.ramp)
+
+
+def writer_mask(flags):
+    # cells that own their populations in both schemes (fluid, inlet, outlet)
+    return np.isin(flags, (fc.FLUID, fc.INLET, fc.OUTLET))
+
+
+# ------------------------------------------------------------------------------
+def test_upload_download_roundtrip(gpu):
+    say("case: roundtrip tgv16 f32/f64")
+    case = cases.taylor_green(16)
+    for st, tol in (("f32", 2e-7), ("f64", 1e-15)):
+        s = cases.make_solver(case, st)
+        got = s.download_f(f64=True)
+        assert np.abs(got - case.f0).max() < tol, st
+        s.close()
+
+
+@pytest.mark.parametrize("steps", [1, 2, 7])
+def test_tgv_step_parity_small(gpu, steps):
+    say(f"case: tgv16 f32 steps={steps}")
+    case = cases.taylor_green(16)
+    s = cases.make_solver(case, "f32")
+    s.step_n(steps)
+    s.synchronize()
+    got = s.download_f(f64=True)
+    f = case.f0.astype(np.float32)
+    taus, uins = schedule(case, steps)
+    rho, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs, periodic_xy=True)
+    assert rel_l2(got, f) < 1e-6
+    r2, u2, v2, w2 = s.read_macro()
+    assert rel_l2(u2, u) < 1e-5 and rel_l2(v2, v) < 1e-5
+    s.close()
+
+
+def test_tgv64_config1_parity_and_decay(gpu):
+    """Config 1: TGV 64^3, TRT (3/16), no LES, u0 0.05, nu 0.01, 2000 steps."""
+    say("case: tgv64 f32 2000 steps vs oracle f32")
+    case = cases.taylor_green(64, 0.05, 0.01)
+    s = cases.make_solver(case, "f32")
+    ke = []
+    for _ in range(4):
+        s.step_n(500)
+        s.synchronize()
+        _, u, v, w = s.read_macro()
+        ke.append(cases.tgv_kinetic_energy(u, v, w))
+    f = case.f0.astype(np.float32)
+    taus, uins = schedule(case, 2000)
+    rho, uo, vo, wo = orc.run(f, case.flags, case.dims, taus, uins, cs=0.0, periodic_xy=True)
+    _, u, v, w = s.read_macro()
+    err = max(rel_l2(u, uo), rel_l2(v, vo))
+    say(f"tgv64 rel L2 vs oracle after 2000 steps: {err:.3e}")
+    assert err < 1e-5
+    assert all(b < a for a, b in zip(ke, ke[1:]))
+    s.close()
+
+
+def test_tgv_linear_decay_rate_gpu(gpu):
+    """Small-amplitude TGV on the GPU decays at the analytic exp(-6 nu k^2 t)."""
+    say("case: tgv48 linear decay f32")
+    n, nu = 48, 0.02
+    case = cases.taylor_green(n, 1e-3, nu)
+    s = cases.make_solver(case, "f32")
+    s.step_n(20)
+    s.synchronize()
+    _, u, v, w = s.read_macro()
+    ke0 = cases.tgv_kinetic_energy(u, v, w)
+    s.step_n(400)
+    s.synchronize()
+    _, u, v, w = s.read_macro()
+    ke1 = cases.tgv_kinetic_energy(u, v, w)
+    k = 2 * np.pi / n
+    rate = -np.log(ke1 / ke0) / 400.0
+    assert rate == pytest.approx(6 * nu * k * k, rel=0.02)
+    s.close()
+
+
+def tunnel(nx=128, ny=48, nz=16, chord=40, aoa=8.0, cs=0.1):
+    return cases.naca_tunnel(nx, ny, nz, chord, aoa=aoa, cs=cs)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 10, 101])
+def test_tunnel_step_parity(gpu, steps):
+    """Inlet / outlet / slip / bounce-back / LES, ramps on, against the FP32 oracle."""
+    say(f"case: tunnel 128x48x16 f32 steps={steps}")
+    case = tunnel()
+    s = cases.make_solver(case, "f32")
+    f = s.download_f(f64=False)  # the solver's own rest + speck initial state
+    mask = writer_mask(case.flags)
+    f0 = orc.init_equilibrium(case.flags, case.dims, 0.0)
+    orc.spanwise_perturb(f0, case.flags, case.dims, 1e-3 * case.scaling.u_lat)
+    fm = f.reshape(Q, -1)[:, mask]
+    assert np.abs(fm - f0.reshape(Q, -1)[:, mask]).max() < 1e-8
+    s.step_n(steps)
+    s.synchronize()
+    got = s.download_f(f64=True).reshape(Q, -1)
+    taus, uins = schedule(case, steps)
+    rho, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs)
+    fluid = case.flags == fc.FLUID
+    assert rel_l2(got[:, fluid], f.reshape(Q, -1)[:, fluid]) < 1e-6
+    r2, u2, v2, w2 = s.read_macro()
+    for a, b in ((r2, rho), (u2, u), (v2, v)):
+        assert rel_l2(a, b) < 1e-5
+    F = s.force_now()
+    Fo = orc.force(f, case.flags, case.dims)
+    assert np.abs(F - Fo).max() < 1e-4 * max(np.abs(Fo).max(), 1e-3)
+    s.close()
+
+
+def test_config2_tunnel_parity(gpu):
+    """Config 2 geometry (384 x 128 x 64, chord 128, AoA 8, Cs 0.1, u_lat 0.08)
+    for 300 steps against the FP32 oracle."""
+    say("case: config2 tunnel 384x128x64 f32 300 steps")
+    case = cases.config2_tunnel()
+    s = cases.make_solver(case, "f32")
+    f = s.download_f(f64=False)
+    s.step_n(300)
+    s.synchronize()
+    taus, uins = schedule(case, 300)
+    rho, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs)
+    r2, u2, v2, w2 = s.read_macro()
+    err = max(rel_l2(u2, u), rel_l2(v2, v), rel_l2(r2, rho))
+    say(f"config2 rel L2 after 300 steps: {err:.3e}")
+    assert err < 1e-5
+    F, Fo = s.force_now(), orc.force(f, case.flags, case.dims)
+    assert np.abs(F - Fo).max() < 1e-4 * np.abs(Fo).max()
+    s.close()
+
+
+def test_fp64_storage_matches_fp64_oracle(gpu):
+    say("case: tunnel f64 50 steps")
+    case = tunnel()
+    s = cases.make_solver(case, "f64")
+    f = s.download_f(f64=True)
+    s.step_n(50)
+    s.synchronize()
+    got = s.download_f(f64=True).reshape(Q, -1)
+    taus, uins = schedule(case, 50)
+    orc.run(f, case.flags, case.dims, taus.astype(np.float64), uins.astype(np.float64),
+            cs=case.cs)
+    fluid = case.flags == fc.FLUID
+    assert rel_l2(got[:, fluid], f.reshape(Q, -1)[:, fluid]) < 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-3), ("bf16", 2e-2)])
+def test_reduced_storage_vs_fp64(gpu, storage, tol):
+    """Error of each storage format against the FP64 run of the same case."""
+    say(f"case: tunnel {storage} vs f64, 400 steps")
+    case = tunnel()
+    ref = cases.make_solver(case, "f64")
+    ref.step_n(400)
+    ref.synchronize()
+    _, ur, vr, _ = ref.read_macro()
+    s = cases.make_solver(case, storage)
+    s.step_n(400)
+    s.synchronize()
+    _, u, v, _ = s.read_macro()
+    err = rel_l2(np.stack([u, v]), np.stack([ur, vr]))
+    say(f"{storage} rel L2 (u,v) vs f64 after 400 steps: {err:.3e}")
+    assert err < tol
+    F, Fr = s.force_now(), ref.force_now()
+    say(f"{storage} force {F} vs f64 {Fr}")
+    assert np.abs(F[:2] - Fr[:2]).max() < 20 * tol * np.abs(Fr[:2]).max()
+    s.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
+def test_mass_conservation_periodic(gpu, storage):
+    say(f"case: tgv32 mass {storage}")
+    case = cases.taylor_green(32)
+    s = cases.make_solver(case, storage)
+    m0 = s.mass()
+    s.step_n(500)
+    s.synchronize()
+    m1 = s.mass()
+    assert abs(m0 - 32 ** 3) < 1e-3
+    tol = {"f32": 1e-6, "f16": 1e-4, "f64": 1e-12}[storage]
+    assert abs(m1 - m0) / m0 < tol
+    s.close()
+
+
+def test_watchdog_trips_on_nan_plane(gpu):
+    say("case: watchdog nan plane")
+    case = tunnel(96, 64, 8, 24)
+    s = cases.make_solver(case, "f32")
+    f = s.download_f(f64=False).reshape(Q, 8, 64, 96)
+    f[:, 3, 1:-1, 1:-1] = np.nan
+    s.upload_f(f.reshape(-1))
+    s.step_n(200)
+    s.synchronize()
+    s.step_n(1)
+    s.synchronize()
+    assert s.nan_detected()
+    assert s.nan_kind() in (1, 2)
+    s.close()
+
+
+def test_force_readout_pipeline(gpu):
+    say("case: force readout pipeline")
+    case = tunnel()
+    s = cases.make_solver(case, "f32")
+    s.step_n(64)
+    s.synchronize()
+    raw, at = s.force_raw()
+    assert at == 64
+    assert np.allclose(raw, s.force_now(), rtol=1e-12, atol=0)
+    r = s.forces()
+    assert np.isfinite(r.cl) and np.isfinite(r.cd)
+    s.close()
+
+
+def test_full_wing_grid_properties(gpu):
+    """The 23.6 M-cell wing (512 x 192 x 240) in FP16 and FP32: finite, the
+    inlet follows the ramp, mass consistent between formats."""
+    say("case: config3 wing 512x192x240 f16/f32 100 steps")
+    case = cases.config3_wing()
+    out = {}
+    for st in ("f16", "f32"):
+        s = cases.make_solver(case, st)
+        s.step_n(100)
+        s.synchronize()
+        rho, u, v, w = s.read_macro()
+        assert np.isfinite(u).all() and np.isfinite(rho).all()
+        assert not s.nan_detected()
+        uin = fc.ramped_u(case.scaling, 99, case.dims[0])
+        g = u.reshape(240, 192, 512)
+        assert np.allclose(g[:, 1:-1, 0], uin, rtol=1e-6)
+        out[st] = (s.mass(), u)
+        s.close()
+    assert abs(out["f16"][0] - out["f32"][0]) / out["f32"][0] < 1e-5
+    assert rel_l2(out["f16"][1], out["f32"][1]) < 5e-3

@@ -1,0 +1,293 @@
+"""Pins for the CPU oracle (oracle/lbm_oracle.cpp) from what the mathematics
+fixes: equilibrium moments, conservation, the momentum-exchange balance,
+mirror-image equivalence of the free-slip walls, the Dirichlet inlet, the
+one-step-stale outlet copy, linear Taylor-Green viscous decay, the watchdog
+verdicts and the exact half-float decode. Each check is chosen so that a
+plausible slip (wrong sign, index, table entry, dropped term) fails it."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+Q, CX, CY, CZ, W = orc.Q, orc.CX, orc.CY, orc.CZ, orc.W
+
+
+def fluid_flags(nx, ny, nz):
+    return np.zeros(nx * ny * nz, dtype=np.uint8)
+
+
+def eq_field(rho, u, v, w):
+    usq = 1.5 * (u * u + v * v + w * w)
+    f = np.empty((Q, rho.size))
+    for q in range(Q):
+        cu = 3.0 * (CX[q] * u + CY[q] * v + CZ[q] * w)
+        f[q] = W[q] * rho * (1 + cu + 0.5 * cu * cu - usq)
+    return f.reshape(-1)
+
+
+def totals(f, n, mask=None):
+    g = f.reshape(Q, n)
+    if mask is not None:
+        g = g[:, mask]
+    return np.array([g.sum(), (CX[:, None] * g).sum(), (CY[:, None] * g).sum(),
+                     (CZ[:, None] * g).sum()])
+
+
+def test_equilibrium_moments_exact():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        rho = 0.5 + rng.random()
+        u = (rng.random(3) - 0.5) * 0.3
+        f = orc.equilibrium(rho, *u)
+        assert abs(f.sum() - rho) < 1e-14
+        j = np.array([(CX * f).sum(), (CY * f).sum(), (CZ * f).sum()])
+        assert np.allclose(j, rho * u, atol=1e-14)
+        c = np.stack([CX, CY, CZ])
+        P = np.einsum("aq,bq,q->ab", c, c, f)
+        assert np.allclose(P, rho / 3 * np.eye(3) + rho * np.outer(u, u), atol=1e-14)
+    # tables: rest weights, opposite pairs consecutive
+    f0 = orc.equilibrium(1.0, 0.0, 0.0, 0.0)
+    assert np.allclose(f0, W)
+    for a in range(1, Q, 2):
+        assert (CX[a], CY[a], CZ[a]) == (-CX[a + 1], -CY[a + 1], -CZ[a + 1])
+
+
+def random_state(n, rng, amp=0.02):
+    rho = 1 + amp * (rng.random(n) - 0.5)
+    u, v, w = (amp * (rng.random(n) - 0.5) for _ in range(3))
+    f = eq_field(rho, u, v, w)
+    f += 1e-3 * amp * (rng.random(f.size) - 0.5)  # non-equilibrium part
+    return f
+
+
+@pytest.mark.parametrize("cs", [0.0, 0.1733])
+def test_periodic_conservation(cs):
+    d = (12, 10, 8)
+    n = np.prod(d)
+    f = random_state(n, np.random.default_rng(2))
+    before = totals(f, n)
+    taus = np.full(5, 0.52)
+    orc.run(f, fluid_flags(*d), d, taus, np.zeros(5), cs=cs, periodic_xy=True)
+    after = totals(f, n)
+    assert np.allclose(after, before, atol=1e-12, rtol=0)
+
+
+def test_uniform_flow_is_fixed_point():
+    d = (8, 8, 8)
+    n = np.prod(d)
+    f = eq_field(np.ones(n), np.full(n, 0.07), np.full(n, -0.02), np.full(n, 0.01))
+    f0 = f.copy()
+    orc.run(f, fluid_flags(*d), d, np.full(4, 0.6), np.zeros(4), cs=0.1733, periodic_xy=True)
+    assert np.abs(f - f0).max() < 1e-15
+
+
+def block_flags(d, x0, x1, y0, y1):
+    f = fluid_flags(*d).reshape(d[2], d[1], d[0])
+    f[:, y0:y1, x0:x1] = 1
+    return f.reshape(-1)
+
+
+def test_momentum_exchange_balance():
+    """J*(t+1) = J*(t) - F(t) over fluid cells, mass conserved (half-way BB)."""
+    d = (20, 14, 6)
+    n = np.prod(d)
+    flags = block_flags(d, 7, 11, 5, 9)
+    fl = flags == 0
+    rng = np.random.default_rng(3)
+    f = random_state(n, rng, amp=0.05)
+    f += eq_field(np.zeros(n), np.full(n, 0.04), np.zeros(n), np.zeros(n)) * 0  # keep shape
+    for _ in range(3):
+        F = orc.force(f, flags, d, periodic_xy=True)
+        t0 = totals(f, n, fl)
+        orc.run(f, flags, d, [0.55], [0.0], cs=0.1733, periodic_xy=True)
+        t1 = totals(f, n, fl)
+        assert abs(t1[0] - t0[0]) < 1e-12
+        assert np.allclose(t1[1:], t0[1:] - F, atol=1e-12, rtol=0)
+    assert np.abs(F).max() > 1e-6  # the balance is not vacuous
+
+
+def test_force_single_link_sign():
+    """A lone fluid cell next to a wall at +x: F = 2 c_q f*_q over the 5 links
+    that point into the wall (+x components only)."""
+    d = (6, 5, 3)
+    n = np.prod(d)
+    flags = block_flags(d, 3, 6, 0, 5)  # x-periodic: fluid x = 0..2, wall on both sides
+    f = np.zeros((Q, d[2], d[1], d[0]))
+    f[:] = W[:, None, None, None]
+    plus = CX > 0
+    f[plus, :, :, 2] *= 2.0  # cells at x=2 push twice the rest population into the +x wall
+    F = orc.force(f.reshape(-1), flags, d, periodic_xy=True)
+    # x=2 cells: 2 * sum_{cx=+1} c_x f = 2 * (2/18 + 4 * 2/36) = 2/3 per cell;
+    # x=0 cells (wall at -x through the wrap): 2 * -(1/18 + 4/36) = -1/3 per cell.
+    cells = d[1] * d[2]
+    assert F[0] == pytest.approx(cells / 3.0, rel=1e-14)
+    assert abs(F[1]) < 1e-14 and abs(F[2]) < 1e-14
+
+
+@pytest.mark.parametrize("axis", ["y", "z"])
+def test_free_slip_equals_mirror_image(axis):
+    nx, m, nk = 10, 6, 4   # m fluid rows between the slip planes
+    rng = np.random.default_rng(4)
+    if axis == "y":
+        dS = (nx, m + 2, nk)
+        dP = (nx, 2 * m, nk)
+    else:
+        dS = (nx, nk, m + 2)
+        dP = (nx, nk, 2 * m)
+    nS, nP = np.prod(dS), np.prod(dP)
+    flagsS = fluid_flags(*dS).reshape(dS[2], dS[1], dS[0])
+    if axis == "y":
+        flagsS[:, 0, :] = 5
+        flagsS[:, -1, :] = 4
+    else:
+        flagsS[0] = 6
+        flagsS[-1] = 7
+    flagsS = flagsS.reshape(-1)
+    mirror = np.array([0, 1, 2, 4, 3, 5, 6, 13, 14, 9, 10, 18, 17, 7, 8, 15, 16, 12, 11]) \
+        if axis == "y" else np.array([0, 1, 2, 3, 4, 6, 5, 7, 8, 15, 16, 17, 18, 13, 14, 9, 10,
+                                      11, 12])
+    # random state on the fluid rows, mirrored into the periodic box
+    fP = np.zeros((Q,) + dP[::-1])
+    fS = np.zeros((Q,) + dS[::-1])
+    base = random_state(m * nx * nk, rng, amp=0.05).reshape(Q, -1)
+    if axis == "y":
+        blk = base.reshape(Q, nk, m, nx)
+        fS[:, :, 1:m + 1, :] = blk
+        fP[:, :, :m, :] = blk
+        fP[:, :, m:, :] = blk[mirror][:, :, ::-1, :]
+    else:
+        blk = base.reshape(Q, m, nk, nx)
+        fS[:, 1:m + 1] = blk
+        fP[:, :m] = blk
+        fP[:, m:] = blk[mirror][:, ::-1]
+    fS = fS.reshape(-1).copy()
+    fP = fP.reshape(-1).copy()
+    steps = 6
+    taus, uins = np.full(steps, 0.56), np.zeros(steps)
+    # x periodic only for the slip box; x and y periodic for the image box
+    import ctypes as C
+    L = orc.lib()
+    slip_py = 0 if axis == "y" else 1  # the slip box wraps every axis except the walled one
+    for f, d, fl, px, py in ((fS, dS, flagsS, 1, slip_py), (fP, dP, fluid_flags(*dP), 1, 1)):
+        tmp = np.empty_like(f)
+        t = np.ascontiguousarray(taus)
+        u = np.ascontiguousarray(uins)
+        P = C.POINTER(C.c_double)
+        L.orc_run_f64(f.ctypes.data_as(P), tmp.ctypes.data_as(P),
+                      fl.ctypes.data_as(C.POINTER(C.c_uint8)), *d, px, py, steps,
+                      t.ctypes.data_as(P), u.ctypes.data_as(P), 3 / 16, float(orc.smag_prefactor(0.1733)),
+                      None, None, None, None, None, None, None, None, None)
+    gS = fS.reshape((Q,) + dS[::-1])
+    gP = fP.reshape((Q,) + dP[::-1])
+    if axis == "y":
+        assert np.abs(gS[:, :, 1:m + 1, :] - gP[:, :, :m, :]).max() < 1e-13
+    else:
+        assert np.abs(gS[:, 1:m + 1] - gP[:, :m]).max() < 1e-13
+
+
+def tunnel_flags(d):
+    nx, ny, nz = d
+    f = fluid_flags(*d).reshape(nz, ny, nx)
+    f[:, 0, :] = 5
+    f[:, -1, :] = 4
+    f[:, :, 0] = 2
+    f[:, 1:-1, -1] = 3
+    return f.reshape(-1)
+
+
+def test_freestream_tunnel_stays_uniform():
+    d = (24, 10, 4)
+    n = np.prod(d)
+    flags = tunnel_flags(d)
+    ui = 0.05
+    f = eq_field(np.ones(n), np.full(n, ui), np.zeros(n), np.zeros(n))
+    rho, u, v, w = orc.run(f, flags, d, np.full(30, 0.53), np.full(30, ui), cs=0.1733)
+    fl = flags == 0
+    assert np.abs(u[fl] - ui).max() < 1e-13 and np.abs(rho[fl] - 1).max() < 1e-13
+    assert np.abs(v[fl]).max() < 1e-13
+
+
+def test_inlet_dirichlet_and_outlet_stale_copy():
+    d = (16, 8, 4)
+    nx, ny, nz = d
+    n = np.prod(d)
+    flags = tunnel_flags(d)
+    f = random_state(n, np.random.default_rng(5), amp=0.05)
+    prev = f.copy().reshape(Q, nz, ny, nx)
+    orc.run(f, flags, d, [0.6], [0.04], cs=0.0)
+    g = f.reshape(Q, nz, ny, nx)
+    feq = orc.equilibrium(1.0, 0.04, 0.0, 0.0)
+    assert np.abs(g[:, :, :, 0] - feq[:, None, None]).max() < 1e-16
+    # outlet cells (x = nx-1, 1 <= y <= ny-2) copy the PREVIOUS x-1 column
+    assert np.array_equal(g[:, :, 1:-1, -1], prev[:, :, 1:-1, -2])
+
+
+def test_taylor_green_linear_decay_rate():
+    """Small-amplitude TGV in a periodic box decays as exp(-6 nu k^2 t)."""
+    N, u0, nu = 24, 1e-3, 0.02
+    k = 2 * np.pi / N
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    u = (u0 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z)).reshape(-1)
+    v = (-u0 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z)).reshape(-1)
+    f = eq_field(np.ones(N ** 3), u, v, np.zeros(N ** 3))
+    d = (N, N, N)
+    ke = []
+    tau = 3 * nu + 0.5
+    for _ in range(4):
+        rho, uu, vv, ww = orc.run(f, fluid_flags(*d), d, np.full(50, tau), np.zeros(50), cs=0.0,
+                                  periodic_xy=True)
+        ke.append(0.5 * np.sum(uu ** 2 + vv ** 2 + ww ** 2))
+    rate = -np.log(ke[-1] / ke[0]) / 150.0
+    assert rate == pytest.approx(6 * nu * k * k, rel=0.02)
+
+
+def test_watchdog_verdicts():
+    d = (80, 64, 4)  # z plane of 5120 cells > the 4093 sampling stride
+    n = np.prod(d)
+    f = eq_field(np.ones(n), np.full(n, 0.05), np.zeros(n), np.zeros(n)).astype(np.float32)
+    assert all(orc.watchdog(f, d, off) == 0 for off in (0, 977, 5000))
+    g = f.reshape(Q, 4, 64, 80).copy()
+    g[3, 2] = np.nan  # a whole z plane: the prime-stride sample must see it
+    assert orc.watchdog(g.reshape(-1), d, 0) == 1
+    assert orc.watchdog(g.reshape(-1), d, 0, flag_base=2) == 3
+    h = eq_field(np.ones(n), np.full(n, 0.29), np.zeros(n), np.zeros(n)).astype(np.float32)
+    assert orc.watchdog(h, d, 0) == 2
+
+
+def test_half_decode_exact():
+    L = orc.lib()
+    bits = np.arange(65536, dtype=np.uint16)
+    ref = bits.view(np.float16).astype(np.float32)
+    got = np.array([L.orc_half_to_float(int(b)) for b in bits[::97]], dtype=np.float32)
+    r = ref[::97]
+    same = (got == r) | (np.isnan(got) & np.isnan(r))
+    assert same.all()
+
+
+def test_speck_adds_z_momentum_only():
+    d = (16, 8, 12)
+    n = np.prod(d)
+    flags = fluid_flags(*d)
+    f = orc.init_equilibrium(flags, d, 0.0, f64=True)
+    f0 = f.copy()
+    orc.spanwise_perturb(f, flags, d, 1e-3)
+    df = (f - f0).reshape(Q, n)
+    assert np.abs(df.sum(axis=0)).max() < 1e-18          # density untouched
+    assert np.abs((CX[:, None] * df).sum(axis=0)).max() < 1e-18
+    wz = (CZ[:, None] * df).sum(axis=0)                   # w' exactly (sum w cz^2 = 1/3)
+    assert np.abs(wz).max() > 5e-4 and np.abs(wz).max() < 1.3e-3 * 1.0001
+    g = wz.reshape(d[2], d[1], d[0])
+    assert np.allclose(g[:, 0, :], g[:, 5, :])            # y independent
+
+
+def test_f32_oracle_tracks_f64():
+    d = (16, 12, 8)
+    n = np.prod(d)
+    flags = block_flags(d, 6, 9, 4, 8)
+    f64 = random_state(n, np.random.default_rng(6), amp=0.05)
+    f32 = f64.astype(np.float32)
+    taus = np.full(20, 0.51)
+    orc.run(f64, flags, d, taus, np.zeros(20), cs=0.1733, periodic_xy=True)
+    orc.run(f32, flags, d, taus.astype(np.float32), np.zeros(20, np.float32), cs=0.1733,
+            periodic_xy=True)
+    assert np.abs(f32 - f64).max() < 1e-6
