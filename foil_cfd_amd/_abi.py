@@ -64,6 +64,21 @@ class FcPerfStats(C.Structure):
                 ("mlups", C.c_double), ("population_bytes_per_step", C.c_double)]
 
 
+class FcWallModelReadout(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("mode", C.c_int), ("listed", C.c_int), ("cells", C.c_int),
+                ("mean_yplus", C.c_float), ("max_yplus", C.c_float), ("clamped_frac", C.c_float),
+                ("excluded_vg", C.c_int), ("degenerate", C.c_int), ("mlp_cells", C.c_int)]
+
+
+class FcMlpParams(C.Structure):
+    _fields_ = [("in_mean", C.c_float * 4), ("in_inv_std", C.c_float * 4),
+                ("env_lo", C.c_float * 4), ("env_hi", C.c_float * 4),
+                ("out_mean", C.c_float), ("out_std", C.c_float),
+                ("w1", C.c_uint16 * (32 * 3)), ("b1", C.c_float * 32),
+                ("w2", C.c_uint16 * (32 * 32)), ("b2", C.c_float * 32),
+                ("w3", C.c_uint16 * 32), ("b3", C.c_float)]
+
+
 class FcInitOptions(C.Structure):
     _fields_ = [("storage", C.c_int), ("periodic_x", C.c_int), ("periodic_y", C.c_int)]
 
@@ -134,6 +149,25 @@ SIGNATURES = {
     "fc_device_fields": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                    C.POINTER(_vp), C.POINTER(_vp)]),
     "fc_device_bytes": (C.c_longlong, [_vp]),
+    "fc_build_wall_cell_list": (C.c_int, [FcDims, _u8p, _u8p, C.c_int, _llp, _llp, _fp, _fp, _fp,
+                                          _fp, C.POINTER(C.c_uint32), C.c_int, _ip]),
+    "fc_set_wall_model": (C.c_int, [_vp, C.c_int]),
+    "fc_set_surface_reference": (C.c_int, [_vp, _u8p]),
+    "fc_get_wall_model_readout": (C.c_int, [_vp, C.POINTER(FcWallModelReadout)]),
+    "fc_read_wall_slip": (C.c_int, [_vp, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16),
+                                    C.POINTER(C.c_uint16), _fp]),
+    "fc_wall_update_now": (C.c_int, [_vp]),
+    "fc_wall_cell_count": (C.c_int, [_vp]),
+    "fc_set_wall_harvest": (C.c_int, [_vp, C.c_int]),
+    "fc_wall_harvest": (C.c_int, [_vp, _fp, C.c_int]),
+    "fc_set_wall_mlp": (C.c_int, [_vp, C.POINTER(FcMlpParams)]),
+    "fc_particles_seed": (C.c_int, [_vp, C.c_int, C.c_uint]),
+    "fc_particles_advect": (C.c_int, [_vp, C.c_float, C.c_uint]),
+    "fc_advect_particles": (C.c_int, [_vp, _vp, C.c_int, C.c_float, C.c_uint]),
+    "fc_particles_read": (C.c_int, [_vp, _fp]),
+    "fc_particles_write": (C.c_int, [_vp, _fp]),
+    "fc_particles_count": (C.c_int, [_vp]),
+    "fc_particles_set_params": (C.c_int, [_vp, C.c_float, C.c_float]),
     "fc_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

@@ -63,4 +63,76 @@ int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, i
                    int* flag, void* stream);
 int launchFillFloat(float* dst, long long count, float value, void* stream);
 
+// ---- wall model -------------------------------------------------------------
+// Device view of a level's wall-cell list (SoA, reference WallCellListView,
+// src/sim/lbm_wallmodel.cuh:47-57); uTau is the persistent EMA state.
+struct WallListView {
+    const long long* cellIdx = nullptr;
+    const long long* sampleIdx = nullptr;
+    const float* nx = nullptr;
+    const float* ny = nullptr;
+    const float* nz = nullptr;
+    const float* ys = nullptr;
+    const std::uint32_t* mask = nullptr;
+    float* uTau = nullptr;
+    int count = 0;
+};
+
+inline constexpr int kMlpIn = 3;
+inline constexpr int kMlpHidden = 32;
+
+// Learned u_tau network (BF16 weight bits + FP32 biases / normalisation).
+struct MlpParams {
+    float inMean[4];
+    float inInvStd[4];
+    float envLo[4];  // training envelope in log-feature space (ln ys, ln ut, ln nu, ln Re_y)
+    float envHi[4];
+    float outMean, outStd;
+    std::uint16_t w1[kMlpHidden * kMlpIn];
+    float b1[kMlpHidden];
+    std::uint16_t w2[kMlpHidden * kMlpHidden];
+    float b2[kMlpHidden];
+    std::uint16_t w3[kMlpHidden];
+    float b3;
+};
+
+enum WallMode : int { kWallOff = 0, kWallNewton = 1, kWallMlp = 2 };
+
+struct WallParams {
+    float nuLat = 0.05f;     // level viscosity incl. the startup ramp
+    float utCutoff = 1e-4f;  // 1e-3 * u_lat
+    float emaAlpha = 0.25f;  // lbm_wallmodel.cuh:35
+    int mode = kWallNewton;
+    const MlpParams* mlp = nullptr;  // device pointer (mode kWallMlp)
+};
+
+struct WallStats {
+    float sumYplus;
+    int maxYplusBits;
+    int activeCells;
+    int clampedCells;
+    int mlpCells;
+};
+
+// One wall-model update of a level (reads the post-collision state in
+// `layout`, writes the half-bit slip arrays). harvest (optional): 4 floats per
+// listed cell (y_s, |u_t|, nu, Newton u_tau or -1) for network training.
+int launchWallUpdate(const LatticeView& lat, int layout, const WallListView& wl, std::uint16_t* uwx,
+                     std::uint16_t* uwy, std::uint16_t* uwz, const WallParams& p, WallStats* stats,
+                     float* harvest, void* stream);
+
+// ---- tracer particles (reference src/render/particles.cuh:29-46) -----------
+struct ParticleParams {
+    float dtSteps = 1.0f;
+    float ageRate = 0.002f;
+    float inletFraction = 0.3f;
+    unsigned seed = 0;
+};
+
+// pos4: 4 floats per particle (x, y, z in lattice-cell space, age in [0, 1]).
+int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, void* stream);
+// kind: the solver's kind bytes (low nibble = cell flag) or null.
+int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
+                         const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, void* stream);
+
 }  // namespace fc

@@ -19,126 +19,10 @@
 //    plane that the x-1 cells publish every step.
 #include <hip/hip_runtime.h>
 
-#include "fc_device.hpp"
-#include "fc_kernels.hpp"
+#include "fc_kcommon.hpp"
 
 namespace fc {
 namespace {
-
-// Kernel arguments of one lattice level (POD, passed by value).
-struct KArgs {
-    void* f;
-    const std::uint8_t* kind;
-    void* feed;
-    float* rho;
-    float* u;
-    float* v;
-    float* w;
-    const std::uint16_t* uwx;
-    const std::uint16_t* uwy;
-    const std::uint16_t* uwz;
-    long long ncells;
-    int nx, ny, nz;
-    int nseg;     // 64-cell segments per row
-    int nwaves;   // nseg * ny * nz
-    int nblocks;  // ceil(nwaves / kWavesPerBlock)
-    int chunk;    // blocks per XCD after the remap
-    float tau0, lam, pre, uIn;
-    int writeMacro;
-};
-
-KArgs makeArgs(const LatticeView& lat) {
-    KArgs a{};
-    a.f = lat.f;
-    a.kind = lat.kind;
-    a.feed = lat.feed;
-    a.rho = lat.rho;
-    a.u = lat.u;
-    a.v = lat.v;
-    a.w = lat.w;
-    a.uwx = lat.uwx;
-    a.uwy = lat.uwy;
-    a.uwz = lat.uwz;
-    a.ncells = lat.dims.cells();
-    a.nx = lat.dims.nx;
-    a.ny = lat.dims.ny;
-    a.nz = lat.dims.nz;
-    a.nseg = (a.nx + kWave - 1) / kWave;
-    a.nwaves = a.nseg * a.ny * a.nz;
-    a.nblocks = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
-    a.chunk = (a.nblocks + kXcds - 1) / kXcds;
-    return a;
-}
-
-// Position of this lane in the wave-row decomposition plus the 3 x 3 table
-// of row start indices for y-1..y+1, z-1..z+1 (z periodic; y and x periodic
-// only when requested). Row starts of rows outside a non-periodic y range
-// are flagged by yOk so boundary cells can skip out-of-domain scatter writes.
-struct RowPos {
-    int x, y, z;
-    int xm, xp;
-    int R[3][3];
-    bool yOk[3];
-};
-
-template <bool PX, bool PY>
-__device__ inline bool rowPos(const KArgs& a, RowPos& p) {
-    const int hb = blockIdx.x;
-    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
-    if (lb >= a.nblocks) return false;
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int gw = lb * kWavesPerBlock + wv;
-    if (gw >= a.nwaves) return false;
-    const int row = gw / a.nseg;
-    const int seg = gw - row * a.nseg;
-    const int z = row / a.ny;
-    const int y = row - z * a.ny;
-    const int x = seg * kWave + static_cast<int>(threadIdx.x & (kWave - 1));
-    if (x >= a.nx) return false;
-    p.x = x;
-    p.y = y;
-    p.z = z;
-    p.xm = x - 1;
-    p.xp = x + 1;
-    if (PX) {
-        if (p.xm < 0) p.xm += a.nx;
-        if (p.xp >= a.nx) p.xp -= a.nx;
-    }
-    int ys[3] = {y - 1, y, y + 1};
-    p.yOk[0] = p.yOk[1] = p.yOk[2] = true;
-    if (PY) {
-        if (ys[0] < 0) ys[0] += a.ny;
-        if (ys[2] >= a.ny) ys[2] -= a.ny;
-    } else {
-        p.yOk[0] = ys[0] >= 0;
-        p.yOk[2] = ys[2] < a.ny;
-        if (!p.yOk[0]) ys[0] = 0;
-        if (!p.yOk[2]) ys[2] = a.ny - 1;
-    }
-    const int zs[3] = {z == 0 ? a.nz - 1 : z - 1, z, z == a.nz - 1 ? 0 : z + 1};
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) p.R[i][j] = (ys[i] + a.ny * zs[j]) * a.nx;
-    return true;
-}
-
-// Cell index of (x + ex, y + ey, z + ez) for compile-time offsets.
-__device__ inline unsigned nb(const RowPos& p, int ex, int ey, int ez) {
-    const int xx = ex < 0 ? p.xm : (ex > 0 ? p.xp : p.x);
-    return static_cast<unsigned>(xx + p.R[ey + 1][ez + 1]);
-}
-
-// True when (x + ex, y + ey) is inside a non-periodic domain (z wraps).
-template <bool PX>
-__device__ inline bool inDomain(const KArgs& a, const RowPos& p, int ex, int ey) {
-    bool ok = p.yOk[ey + 1];
-    if (!PX) {
-        const int xx = p.x + ex;
-        ok = ok && xx >= 0 && xx < a.nx;
-    }
-    return ok;
-}
 
 // Scaled equilibrium deviation K * (feq_q(rho, u) - w_q) for one direction,
 // via the same constants the collision uses.
@@ -486,56 +370,6 @@ __global__ void __launch_bounds__(kBlock) downloadKernel(KArgs a, int layout, CT
             out = CT(__builtin_nan(""));
         }
         canon[static_cast<long long>(q) * a.ncells + c] = out;
-    }
-}
-
-// Post-collision population f*_q of cell (x,y,z) in the given layout, as a
-// deviation H (unscaled by K). Used by the readout kernels.
-template <class S>
-__device__ inline typename S::C postH(const KArgs& a, int layout, int q, int x, int y, int z) {
-    using T = typename S::T;
-    const T* const f = static_cast<const T*>(a.f);
-    if (layout == kLayoutL) {
-        const long long c = x + static_cast<long long>(a.nx) * (y + static_cast<long long>(a.ny) * z);
-        return S::load(f + static_cast<long long>(kOpp[q]) * a.ncells + c);
-    }
-    int xx = x + kCx[q], yy = y + kCy[q], zz = z + kCz[q];
-    if (xx < 0) xx += a.nx;
-    if (xx >= a.nx) xx -= a.nx;
-    if (yy < 0) yy += a.ny;
-    if (yy >= a.ny) yy -= a.ny;
-    if (zz < 0) zz += a.nz;
-    if (zz >= a.nz) zz -= a.nz;
-    const long long c = xx + static_cast<long long>(a.nx) * (yy + static_cast<long long>(a.ny) * zz);
-    return S::load(f + static_cast<long long>(q) * a.ncells + c);
-}
-
-// Wave64 FP64 sum via cross-lane shuffles.
-__device__ inline double waveSum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-    return v;
-}
-
-// Deterministic block reduction of NV doubles per thread (block = kBlock).
-template <int NV>
-__device__ inline void blockSum(double (&v)[NV], double* out) {
-    __shared__ double sm[NV][kWavesPerBlock];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const double s = waveSum(v[i]);
-        if (lane == 0) sm[i][wv] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double t = 0;
-            for (int j = 0; j < kWavesPerBlock; ++j) t += sm[i][j];
-            out[i] = t;
-        }
     }
 }
 

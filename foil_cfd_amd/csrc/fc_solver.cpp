@@ -21,6 +21,7 @@
 #include "fc_geometry.hpp"
 #include "fc_kernels.hpp"
 #include "fc_units.hpp"
+#include "fc_wall.hpp"
 
 namespace fc {
 namespace {
@@ -232,6 +233,10 @@ public:
         hipRelease(dForceCells_);
         hipRelease(dForceMasks_);
         hipRelease(dForcePartials_);
+        wm_.free();
+        hipRelease(dMlp_);
+        hipRelease(dParticles_);
+        particleCount_ = 0;
         if (hForce_) (void)hipHostFree(hForce_);
         if (hNan_) (void)hipHostFree(hNan_);
         hForce_ = nullptr;
@@ -282,6 +287,14 @@ public:
                                       speckPhase(kSpeckSeed, 1u), speckPhase(kSpeckSeed, 2u),
                                       speckPhase(kSpeckSeed, 3u), stream_);
         if (e != 0) return fail(FC_ERR_DEVICE, "init kernels failed");
+        // Wall-model EMA state and slip field restart with the flow.
+        if (wm_.active) {
+            const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
+            (void)hipMemsetAsync(wm_.dUTau, 0, static_cast<std::size_t>(wm_.count) * sizeof(float), stream_);
+            (void)hipMemsetAsync(wm_.dUwx, 0, cb, stream_);
+            (void)hipMemsetAsync(wm_.dUwy, 0, cb, stream_);
+            (void)hipMemsetAsync(wm_.dUwz, 0, cb, stream_);
+        }
         return FC_OK;
     }
 
@@ -350,8 +363,172 @@ public:
                 stepsAtForceLaunch_ = steps_;
             }
         }
+        // One wall-model update per batch on the state the batch produced;
+        // the new slip field takes effect next batch (lbm_solver.cpp:1356-1367).
+        if (wm_.active) {
+            if (int e = wallUpdate(); e != FC_OK) return e;
+        }
         return FC_OK;
     }
+
+    // ---- tracer particles (reference particles.cu / viz.cpp) ---------------
+    int particlesSeed(int count, unsigned seed) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (count <= 0) return fail(FC_ERR_INVALID, "particle count must be positive");
+        if (count != particleCount_) {
+            hipRelease(dParticles_);
+            particleCount_ = 0;
+            if (hipAlloc(&dParticles_, static_cast<std::size_t>(count) * 4 * sizeof(float)) != FC_OK)
+                return fail(FC_ERR_ALLOC, "particle buffer");
+            particleCount_ = count;
+        }
+        if (launchParticleSeed(dParticles_, count, dims_, seed, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "particle seed failed");
+        return FC_OK;
+    }
+
+    int particlesAdvect(float dtSteps, unsigned seed) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!dParticles_) return fail(FC_ERR_STATE, "particles not seeded");
+        ParticleParams p = particleParams_;
+        p.dtSteps = dtSteps;
+        p.seed = seed;
+        if (launchParticleAdvect(dParticles_, particleCount_, u_, v_, w_, kind_, dims_, p, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "particle advection failed");
+        return FC_OK;
+    }
+
+    int particlesAdvectExternal(float* devPos4, int count, float dtSteps, unsigned seed) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        ParticleParams p = particleParams_;
+        p.dtSteps = dtSteps;
+        p.seed = seed;
+        if (launchParticleAdvect(devPos4, count, u_, v_, w_, kind_, dims_, p, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "particle advection failed");
+        return FC_OK;
+    }
+
+    int particlesCopy(float* host, bool toHost) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!dParticles_ || !host) return fail(FC_ERR_STATE, "particles not seeded");
+        const std::size_t b = static_cast<std::size_t>(particleCount_) * 4 * sizeof(float);
+        const hipError_t e = toHost ? hipMemcpyAsync(host, dParticles_, b, hipMemcpyDeviceToHost, stream_)
+                                    : hipMemcpyAsync(dParticles_, host, b, hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess || hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "particle copy failed");
+        return FC_OK;
+    }
+
+    int particleCount() const { return particleCount_; }
+    void setParticleParams(float ageRate, float inletFraction) {
+        particleParams_.ageRate = ageRate;
+        particleParams_.inletFraction = inletFraction;
+    }
+
+    // ---- wall model --------------------------------------------------------
+    int setWallModel(int mode) {
+        if (mode < kWallOff || mode > kWallMlp) return fail(FC_ERR_INVALID, "unknown wall-model mode");
+        if (mode == kWallMlp && !dMlp_) return fail(FC_ERR_STATE, "no network loaded for the MLP wall model");
+        wmMode_ = mode;
+        if (!initialized_) return FC_OK;
+        return rebuildWallModel();
+    }
+
+    int setSurfaceReference(const std::uint8_t* clean) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!clean) {
+            surfaceRef_.clear();
+        } else {
+            surfaceRef_.assign(clean, clean + ncells_);
+        }
+        return rebuildWallModel();
+    }
+
+    int setWallHarvest(bool on) {
+        harvestOn_ = on;
+        if (!initialized_) return FC_OK;
+        hipRelease(wm_.dHarvest);
+        if (on && wm_.count > 0 &&
+            hipAlloc(&wm_.dHarvest, static_cast<std::size_t>(wm_.count) * 4 * sizeof(float)) != FC_OK)
+            return fail(FC_ERR_ALLOC, "harvest buffer");
+        return FC_OK;
+    }
+
+    int wallHarvest(float* out, int cap) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!wm_.dHarvest) return 0;
+        const int n = std::min(cap, wm_.count);
+        if (out && n > 0 &&
+            (hipMemcpyAsync(out, wm_.dHarvest, static_cast<std::size_t>(n) * 4 * sizeof(float),
+                            hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+             hipStreamSynchronize(stream_) != hipSuccess))
+            return -fail(FC_ERR_DEVICE, "harvest readback failed");
+        return out ? n : wm_.count;
+    }
+
+    int setMlp(const MlpParams* host) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!host) {
+            hipRelease(dMlp_);
+            if (wmMode_ == kWallMlp) wmMode_ = kWallNewton;
+            return FC_OK;
+        }
+        if (!dMlp_ && hipAlloc(&dMlp_, sizeof(MlpParams)) != FC_OK) return fail(FC_ERR_ALLOC, "mlp params");
+        if (hipMemcpyAsync(dMlp_, host, sizeof(MlpParams), hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "mlp upload failed");
+        return FC_OK;
+    }
+
+    int wallReadout(fc_wall_model_readout* r) {
+        if (!r) return fail(FC_ERR_INVALID, "null output");
+        *r = fc_wall_model_readout{};
+        r->mode = wmMode_;
+        r->enabled = wmMode_ != kWallOff ? 1 : 0;
+        r->excluded_vg = wm_.stats.excludedVG;
+        r->degenerate = wm_.stats.degenerate;
+        r->listed = wm_.count;
+        if (!initialized_ || !wm_.active) return FC_OK;
+        WallStats st{};
+        if (hipMemcpyAsync(&st, wm_.dStats, sizeof st, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "wall stats readback failed");
+        r->cells = st.activeCells;
+        r->mlp_cells = st.mlpCells;
+        if (st.activeCells > 0) {
+            r->mean_yplus = st.sumYplus / static_cast<float>(st.activeCells);
+            float mx = 0.0f;
+            std::memcpy(&mx, &st.maxYplusBits, sizeof mx);
+            r->max_yplus = mx;
+            r->clamped_frac = static_cast<float>(st.clampedCells) / static_cast<float>(st.activeCells);
+        }
+        return FC_OK;
+    }
+
+    int readWallSlip(std::uint16_t* ux, std::uint16_t* uy, std::uint16_t* uz, float* utau) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!wm_.active) return fail(FC_ERR_STATE, "wall model inactive");
+        const std::size_t b = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
+        const std::pair<std::uint16_t*, std::uint16_t*> pairs[] = {{ux, wm_.dUwx}, {uy, wm_.dUwy}, {uz, wm_.dUwz}};
+        for (const auto& pr : pairs)
+            if (pr.first && hipMemcpyAsync(pr.first, pr.second, b, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "slip readback failed");
+        if (utau && hipMemcpyAsync(utau, wm_.dUTau, static_cast<std::size_t>(wm_.count) * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "u_tau readback failed");
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "slip readback failed");
+        return FC_OK;
+    }
+
+    // Force one wall-model update now (tests; the solver otherwise runs it
+    // once per fc_step call).
+    int wallUpdateNow() {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!wm_.active) return fail(FC_ERR_STATE, "wall model inactive");
+        return wallUpdate();
+    }
+
+    int wallCount() const { return wm_.count; }
 
     int synchronize() {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
@@ -616,7 +793,137 @@ private:
         l.u = u_;
         l.v = v_;
         l.w = w_;
+        if (wm_.active) {
+            l.uwx = wm_.dUwx;
+            l.uwy = wm_.dUwy;
+            l.uwz = wm_.dUwz;
+        }
         return l;
+    }
+
+    // Device mirror of one level's wall-cell list + slip field
+    // (reference WMLevel, lbm_solver.cpp:206-243).
+    struct WMLevel {
+        bool active = false;
+        int count = 0;
+        WallListStats stats;
+        long long *dCell = nullptr, *dSample = nullptr;
+        float *dNx = nullptr, *dNy = nullptr, *dNz = nullptr, *dYs = nullptr, *dUTau = nullptr;
+        std::uint32_t* dMask = nullptr;
+        std::uint16_t *dUwx = nullptr, *dUwy = nullptr, *dUwz = nullptr;
+        WallStats* dStats = nullptr;
+        float* dHarvest = nullptr;
+        std::vector<long long> hostCells;
+        WallListView view() const {
+            WallListView v;
+            v.cellIdx = dCell;
+            v.sampleIdx = dSample;
+            v.nx = dNx;
+            v.ny = dNy;
+            v.nz = dNz;
+            v.ys = dYs;
+            v.mask = dMask;
+            v.uTau = dUTau;
+            v.count = count;
+            return v;
+        }
+        void free() {
+            hipRelease(dCell);
+            hipRelease(dSample);
+            hipRelease(dNx);
+            hipRelease(dNy);
+            hipRelease(dNz);
+            hipRelease(dYs);
+            hipRelease(dUTau);
+            hipRelease(dMask);
+            hipRelease(dUwx);
+            hipRelease(dUwy);
+            hipRelease(dUwz);
+            hipRelease(dStats);
+            hipRelease(dHarvest);
+            active = false;
+            count = 0;
+            stats = WallListStats{};
+            hostCells.clear();
+        }
+    };
+
+    // (Re)build the wall list from the live flags and the clean surface
+    // reference; listed cells get the kind wall bit (lbm_solver.cpp:295-367).
+    int rebuildWallModel() {
+        // clear old wall bits
+        for (long long c : wm_.hostCells) hostKind_[static_cast<std::size_t>(c)] &= ~kKindWall;
+        wm_.free();
+        int rc = FC_OK;
+        if (wmMode_ != kWallOff) {
+            const std::vector<std::uint8_t>& clean = surfaceRef_.size() == hostFlags_.size() ? surfaceRef_ : hostFlags_;
+            WallList list = buildWallCellList(dims_, hostFlags_, clean, kWallSampleCellsCoarse, &wm_.stats);
+            const std::size_t n = list.size();
+            if (n > 0) {
+                const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
+                rc |= hipAlloc(&wm_.dCell, n * sizeof(long long));
+                rc |= hipAlloc(&wm_.dSample, n * sizeof(long long));
+                rc |= hipAlloc(&wm_.dNx, n * sizeof(float));
+                rc |= hipAlloc(&wm_.dNy, n * sizeof(float));
+                rc |= hipAlloc(&wm_.dNz, n * sizeof(float));
+                rc |= hipAlloc(&wm_.dYs, n * sizeof(float));
+                rc |= hipAlloc(&wm_.dUTau, n * sizeof(float));
+                rc |= hipAlloc(&wm_.dMask, n * sizeof(std::uint32_t));
+                rc |= hipAlloc(&wm_.dUwx, cb);
+                rc |= hipAlloc(&wm_.dUwy, cb);
+                rc |= hipAlloc(&wm_.dUwz, cb);
+                rc |= hipAlloc(&wm_.dStats, sizeof(WallStats));
+                if (rc != FC_OK) {
+                    wm_.free();
+                    return fail(FC_ERR_ALLOC, "wall-model allocation failed");
+                }
+                const std::pair<void*, std::pair<const void*, std::size_t>> ups[] = {
+                    {wm_.dCell, {list.cellIdx.data(), n * sizeof(long long)}},
+                    {wm_.dSample, {list.sampleIdx.data(), n * sizeof(long long)}},
+                    {wm_.dNx, {list.nx.data(), n * sizeof(float)}},
+                    {wm_.dNy, {list.ny.data(), n * sizeof(float)}},
+                    {wm_.dNz, {list.nz.data(), n * sizeof(float)}},
+                    {wm_.dYs, {list.sampleDist.data(), n * sizeof(float)}},
+                    {wm_.dMask, {list.linkMask.data(), n * sizeof(std::uint32_t)}},
+                };
+                for (const auto& u : ups)
+                    if (hipMemcpyAsync(u.first, u.second.first, u.second.second, hipMemcpyHostToDevice, stream_) !=
+                        hipSuccess)
+                        rc = FC_ERR_DEVICE;
+                (void)hipMemsetAsync(wm_.dUTau, 0, n * sizeof(float), stream_);
+                (void)hipMemsetAsync(wm_.dUwx, 0, cb, stream_);
+                (void)hipMemsetAsync(wm_.dUwy, 0, cb, stream_);
+                (void)hipMemsetAsync(wm_.dUwz, 0, cb, stream_);
+                (void)hipMemsetAsync(wm_.dStats, 0, sizeof(WallStats), stream_);
+                wm_.count = static_cast<int>(n);
+                wm_.hostCells = list.cellIdx;
+                for (long long c : wm_.hostCells) hostKind_[static_cast<std::size_t>(c)] |= kKindWall;
+                wm_.active = true;
+                if (harvestOn_) {
+                    if (hipAlloc(&wm_.dHarvest, n * 4 * sizeof(float)) != FC_OK) rc = FC_ERR_ALLOC;
+                }
+            }
+        }
+        if (hipMemcpyAsync(kind_, hostKind_.data(), hostKind_.size(), hipMemcpyHostToDevice, stream_) != hipSuccess)
+            rc = FC_ERR_DEVICE;
+        if (hipStreamSynchronize(stream_) != hipSuccess) rc = FC_ERR_DEVICE;
+        if (rc != FC_OK) {
+            wm_.free();
+            return fail(rc, "wall-model upload failed");
+        }
+        return FC_OK;
+    }
+
+    int wallUpdate() {
+        WallParams p;
+        p.nuLat = (effectiveTau() - 0.5f) / 3.0f;
+        p.utCutoff = 1e-3f * scaling_.u_lat;
+        p.mode = wmMode_;
+        p.mlp = dMlp_;
+        if (launchWallUpdate(view(), layout_, wm_.view(), wm_.dUwx, wm_.dUwy, wm_.dUwz, p, wm_.dStats,
+                             wm_.dHarvest, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "wall-model update failed");
+        return FC_OK;
     }
 
     std::size_t at(int x, int y, int z) const {
@@ -665,7 +972,8 @@ private:
 
     int uploadFlags() {
         // Kind bytes: flag + special / feed bits.
-        std::vector<std::uint8_t> kind(static_cast<std::size_t>(ncells_));
+        hostKind_.assign(static_cast<std::size_t>(ncells_), 0);
+        std::vector<std::uint8_t>& kind = hostKind_;
         std::vector<int> fcells;
         std::vector<std::uint32_t> fmasks;
         zSlipWalls_ = false;
@@ -720,6 +1028,8 @@ private:
             return fail(FC_ERR_DEVICE, "force list upload failed");
         // The host vectors die at return: finish the copies first.
         if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "flag upload failed");
+        // Geometry changed: the wall-cell list is keyed to the flags.
+        if (wmMode_ != kWallOff) return rebuildWallModel();
         return FC_OK;
     }
 
@@ -808,6 +1118,15 @@ private:
 
     Dims dims_{};
     Scaling scaling_{};
+    std::vector<std::uint8_t> hostKind_;
+    std::vector<std::uint8_t> surfaceRef_;
+    WMLevel wm_;
+    int wmMode_ = kWallOff;
+    bool harvestOn_ = false;
+    MlpParams* dMlp_ = nullptr;
+    float* dParticles_ = nullptr;
+    int particleCount_ = 0;
+    ParticleParams particleParams_;
     long long ncells_ = 0;
     int storage_ = kStoreF32;
     bool periodicX_ = false, periodicY_ = false;
@@ -1135,6 +1454,92 @@ int fc_kernel_timing(fc_solver* s, int* launches, double* total_ms) {
 
 int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms) {
     return s ? s->impl.timeSteps(n, steps_per_call, ms) : FC_ERR_INVALID;
+}
+
+}  // extern "C"
+
+// ---- wall model C ABI -----------------------------------------------------
+static_assert(sizeof(fc_mlp_params) == sizeof(fc::MlpParams), "fc_mlp_params layout");
+
+extern "C" {
+
+int fc_build_wall_cell_list(fc_dims dims, const uint8_t* active, const uint8_t* clean, int sample_cells,
+                            long long* cell_idx, long long* sample_idx, float* nx, float* ny, float* nz,
+                            float* sample_dist, uint32_t* link_mask, int cap, int* stats3) {
+    if (!active || dims.nx < 1 || dims.ny < 1 || dims.nz < 1) return -1;
+    const fc::Dims d{dims.nx, dims.ny, dims.nz};
+    const std::vector<std::uint8_t> a(active, active + d.cells());
+    const std::vector<std::uint8_t> c = clean ? std::vector<std::uint8_t>(clean, clean + d.cells()) : a;
+    fc::WallListStats st;
+    const fc::WallList l = fc::buildWallCellList(d, a, c, sample_cells, &st);
+    const int n = static_cast<int>(l.size());
+    if (stats3) {
+        stats3[0] = st.listed;
+        stats3[1] = st.excludedVG;
+        stats3[2] = st.degenerate;
+    }
+    const int m = std::min(n, std::max(cap, 0));
+    auto put = [m](auto* dst, const auto& src) {
+        if (dst) std::copy(src.begin(), src.begin() + m, dst);
+    };
+    put(cell_idx, l.cellIdx);
+    put(sample_idx, l.sampleIdx);
+    put(nx, l.nx);
+    put(ny, l.ny);
+    put(nz, l.nz);
+    put(sample_dist, l.sampleDist);
+    put(link_mask, l.linkMask);
+    return n;
+}
+
+int fc_set_wall_model(fc_solver* s, int mode) { return s ? s->impl.setWallModel(mode) : FC_ERR_INVALID; }
+
+int fc_set_surface_reference(fc_solver* s, const uint8_t* clean_flags) {
+    return s ? s->impl.setSurfaceReference(clean_flags) : FC_ERR_INVALID;
+}
+
+int fc_get_wall_model_readout(fc_solver* s, fc_wall_model_readout* out) {
+    return s ? s->impl.wallReadout(out) : FC_ERR_INVALID;
+}
+
+int fc_read_wall_slip(fc_solver* s, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz, float* utau) {
+    return s ? s->impl.readWallSlip(uwx, uwy, uwz, utau) : FC_ERR_INVALID;
+}
+
+int fc_wall_update_now(fc_solver* s) { return s ? s->impl.wallUpdateNow() : FC_ERR_INVALID; }
+int fc_wall_cell_count(fc_solver* s) { return s ? s->impl.wallCount() : 0; }
+int fc_set_wall_harvest(fc_solver* s, int enabled) { return s ? s->impl.setWallHarvest(enabled != 0) : FC_ERR_INVALID; }
+int fc_wall_harvest(fc_solver* s, float* out, int cap) { return s ? s->impl.wallHarvest(out, cap) : -FC_ERR_INVALID; }
+
+int fc_set_wall_mlp(fc_solver* s, const fc_mlp_params* params) {
+    if (!s) return FC_ERR_INVALID;
+    return s->impl.setMlp(reinterpret_cast<const fc::MlpParams*>(params));
+}
+
+}  // extern "C"
+
+// ---- particles C ABI --------------------------------------------------------
+extern "C" {
+
+int fc_particles_seed(fc_solver* s, int count, unsigned seed) {
+    return s ? s->impl.particlesSeed(count, seed) : FC_ERR_INVALID;
+}
+int fc_particles_advect(fc_solver* s, float dt_steps, unsigned seed) {
+    return s ? s->impl.particlesAdvect(dt_steps, seed) : FC_ERR_INVALID;
+}
+int fc_advect_particles(fc_solver* s, void* device_pos4, int count, float dt_steps, unsigned seed) {
+    if (!s || !device_pos4 || count <= 0) return FC_ERR_INVALID;
+    return s->impl.particlesAdvectExternal(static_cast<float*>(device_pos4), count, dt_steps, seed);
+}
+int fc_particles_read(fc_solver* s, float* pos4) { return s ? s->impl.particlesCopy(pos4, true) : FC_ERR_INVALID; }
+int fc_particles_write(fc_solver* s, const float* pos4) {
+    return s ? s->impl.particlesCopy(const_cast<float*>(pos4), false) : FC_ERR_INVALID;
+}
+int fc_particles_count(fc_solver* s) { return s ? s->impl.particleCount() : 0; }
+int fc_particles_set_params(fc_solver* s, float age_rate, float inlet_fraction) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.setParticleParams(age_rate, inlet_fraction);
+    return FC_OK;
 }
 
 }  // extern "C"

@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (FcDims, FcForceReadout, FcInitOptions, FcLayout, FcPatchBox, FcPerfStats,
-                   FcScaling, check, load)
+from ._abi import (FcDims, FcForceReadout, FcInitOptions, FcLayout, FcMlpParams, FcPatchBox,
+                   FcPerfStats, FcScaling, FcWallModelReadout, check, load)
 
 Q = 19
 CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0], dtype=np.int64)
@@ -152,6 +152,31 @@ def build_fine_patch_flags(points, aoa_deg, coarse: FcLayout, box: FcPatchBox,
 def stamp_interface_shell(nx, ny, nz, flags: np.ndarray) -> None:
     check(load().fc_stamp_interface_shell(FcDims(nx, ny, nz), _u8(flags)),
           "fc_stamp_interface_shell")
+
+
+def build_wall_cell_list(dims, active, clean=None, sample_cells=2):
+    """buildWallCellList (wall_model.cpp:169-238) -> (dict of SoA arrays, stats)."""
+    lib = load()
+    nx, ny, nz = dims
+    a = np.ascontiguousarray(active, dtype=np.uint8).reshape(-1)
+    c = None if clean is None else np.ascontiguousarray(clean, dtype=np.uint8).reshape(-1)
+    st = (C.c_int * 3)()
+    n = lib.fc_build_wall_cell_list(FcDims(nx, ny, nz), _u8(a), None if c is None else _u8(c),
+                                    sample_cells, None, None, None, None, None, None, None, 0, st)
+    out = {"cell": np.zeros(n, np.int64), "sample": np.zeros(n, np.int64),
+           "nx": np.zeros(n, np.float32), "ny": np.zeros(n, np.float32),
+           "nz": np.zeros(n, np.float32), "ys": np.zeros(n, np.float32),
+           "mask": np.zeros(n, np.uint32)}
+    ll = C.POINTER(C.c_longlong)
+    lib.fc_build_wall_cell_list(FcDims(nx, ny, nz), _u8(a), None if c is None else _u8(c),
+                                sample_cells, out["cell"].ctypes.data_as(ll),
+                                out["sample"].ctypes.data_as(ll), _f32(out["nx"]), _f32(out["ny"]),
+                                _f32(out["nz"]), _f32(out["ys"]),
+                                out["mask"].ctypes.data_as(C.POINTER(C.c_uint32)), n, st)
+    return out, {"listed": st[0], "excluded_vg": st[1], "degenerate": st[2]}
+
+
+WALL_OFF, WALL_NEWTON, WALL_MLP = 0, 1, 2
 
 
 # --------------------------------------------------------------- solver ----
@@ -291,6 +316,76 @@ class LBMSolver:
 
     def device_bytes(self) -> int:
         return self._lib.fc_device_bytes(self._h)
+
+    # wall model -------------------------------------------------------------
+    def set_wall_model(self, mode):
+        m = {"off": WALL_OFF, "newton": WALL_NEWTON, "mlp": WALL_MLP}.get(mode, mode)
+        check(self._lib.fc_set_wall_model(self._h, int(m)), "fc_set_wall_model")
+
+    def set_surface_reference(self, clean):
+        c = None if clean is None else np.ascontiguousarray(clean, dtype=np.uint8).reshape(-1)
+        check(self._lib.fc_set_surface_reference(self._h, None if c is None else _u8(c)),
+              "fc_set_surface_reference")
+
+    def wall_model_readout(self) -> FcWallModelReadout:
+        r = FcWallModelReadout()
+        check(self._lib.fc_get_wall_model_readout(self._h, C.byref(r)), "fc_get_wall_model_readout")
+        return r
+
+    def wall_cell_count(self) -> int:
+        return self._lib.fc_wall_cell_count(self._h)
+
+    def read_wall_slip(self):
+        n = self.ncells
+        u16 = C.POINTER(C.c_uint16)
+        ux, uy, uz = (np.zeros(n, np.uint16) for _ in range(3))
+        utau = np.zeros(max(self.wall_cell_count(), 1), np.float32)
+        check(self._lib.fc_read_wall_slip(self._h, ux.ctypes.data_as(u16), uy.ctypes.data_as(u16),
+                                          uz.ctypes.data_as(u16), _f32(utau)), "fc_read_wall_slip")
+        return ux, uy, uz, utau[: self.wall_cell_count()]
+
+    def wall_update_now(self):
+        check(self._lib.fc_wall_update_now(self._h), "fc_wall_update_now")
+
+    def set_wall_harvest(self, on: bool):
+        check(self._lib.fc_set_wall_harvest(self._h, 1 if on else 0), "fc_set_wall_harvest")
+
+    def wall_harvest(self) -> np.ndarray:
+        n = self._lib.fc_wall_harvest(self._h, None, 0)
+        if n < 0:
+            raise RuntimeError("fc_wall_harvest failed")
+        out = np.zeros((max(n, 1), 4), np.float32)
+        got = self._lib.fc_wall_harvest(self._h, _f32(out), n)
+        if got < 0:
+            raise RuntimeError("fc_wall_harvest failed")
+        return out[:got]
+
+    def set_wall_mlp(self, params):
+        check(self._lib.fc_set_wall_mlp(self._h, None if params is None else C.byref(params)),
+              "fc_set_wall_mlp")
+
+    # tracer particles ---------------------------------------------------------
+    def particles_seed(self, count: int, seed: int = 0xF01DCAFE):
+        check(self._lib.fc_particles_seed(self._h, int(count), seed & 0xFFFFFFFF),
+              "fc_particles_seed")
+
+    def particles_advect(self, dt_steps: float, seed: int):
+        check(self._lib.fc_particles_advect(self._h, dt_steps, seed & 0xFFFFFFFF),
+              "fc_particles_advect")
+
+    def particles_read(self) -> np.ndarray:
+        n = self._lib.fc_particles_count(self._h)
+        out = np.zeros((n, 4), np.float32)
+        check(self._lib.fc_particles_read(self._h, _f32(out)), "fc_particles_read")
+        return out
+
+    def particles_write(self, pos: np.ndarray):
+        pos = np.ascontiguousarray(pos, np.float32)
+        check(self._lib.fc_particles_write(self._h, _f32(pos)), "fc_particles_write")
+
+    def particles_set_params(self, age_rate=0.002, inlet_fraction=0.3):
+        check(self._lib.fc_particles_set_params(self._h, age_rate, inlet_fraction),
+              "fc_particles_set_params")
 
     # measurement --------------------------------------------------------------
     def set_kernel_timing(self, on: bool):

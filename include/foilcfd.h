@@ -201,6 +201,79 @@ int fc_device_fields(fc_solver* s, void** rho, void** u, void** v, void** w, voi
 /* Bytes of device memory owned by the handle. */
 long long fc_device_bytes(fc_solver* s);
 
+/* ---- iMEM wall model (lbm_wallmodel.cuh, wall_model.h) -------------------- */
+#define FC_WALL_OFF 0
+#define FC_WALL_NEWTON 1   /* Reichardt inversion by Newton (the reference) */
+#define FC_WALL_MLP 2      /* learned u_tau network, Newton outside its envelope */
+
+/* WallModelReadout, lbm_solver.h:82-91 (+ mode, listed and network-served cells) */
+typedef struct fc_wall_model_readout {
+    int enabled, mode;
+    int listed, cells;
+    float mean_yplus, max_yplus, clamped_frac;
+    int excluded_vg, degenerate;
+    int mlp_cells;
+} fc_wall_model_readout;
+
+/* Learned u_tau network: 3 -> 32 -> 32 -> 1 tanh MLP on normalised
+ * (ln y_s, ln |u_t|, ln nu), output ln(u_tau/|u_t|) (de-normalised by
+ * out_mean/out_std); BF16 weight bits, FP32 biases; envelope bounds on
+ * (ln y_s, ln |u_t|, ln nu, ln Re_y). No reference counterpart. */
+typedef struct fc_mlp_params {
+    float in_mean[4];
+    float in_inv_std[4];
+    float env_lo[4];
+    float env_hi[4];
+    float out_mean, out_std;
+    uint16_t w1[32 * 3];
+    float b1[32];
+    uint16_t w2[32 * 32];
+    float b2[32];
+    uint16_t w3[32];
+    float b3;
+} fc_mlp_params;
+
+/* buildWallCellList, wall_model.h:107-111 (host). Returns the listed count;
+ * arrays (each cap entries, may be NULL to query the count) receive the SoA
+ * list; stats3 (may be NULL) receives listed / excludedVG / degenerate. */
+int fc_build_wall_cell_list(fc_dims dims, const uint8_t* active, const uint8_t* clean,
+                            int sample_cells, long long* cell_idx, long long* sample_idx,
+                            float* nx, float* ny, float* nz, float* sample_dist,
+                            uint32_t* link_mask, int cap, int* stats3);
+/* setWallModelEnabled, lbm_solver.h:290 (mode FC_WALL_*) */
+int fc_set_wall_model(fc_solver* s, int mode);
+/* setSurfaceReference, lbm_solver.h:280 (clean VG-free flags; NULL = live flags) */
+int fc_set_surface_reference(fc_solver* s, const uint8_t* clean_flags);
+/* wallModelReadout, lbm_solver.h:307 */
+int fc_get_wall_model_readout(fc_solver* s, fc_wall_model_readout* out);
+/* Slip field (half bits, cells each) and per-list-entry u_tau; NULL skips. */
+int fc_read_wall_slip(fc_solver* s, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz, float* utau);
+/* Run one wall-model update on the current state now (fc_step runs one per call). */
+int fc_wall_update_now(fc_solver* s);
+int fc_wall_cell_count(fc_solver* s);
+/* Harvest training samples (y_s, |u_t|, nu, Newton u_tau; u_tau < 0 = none)
+ * from every update while enabled; fc_wall_harvest copies the last update's
+ * samples (4 floats each) and returns the count (out NULL: count only). */
+int fc_set_wall_harvest(fc_solver* s, int enabled);
+int fc_wall_harvest(fc_solver* s, float* out, int cap);
+/* Load (or with NULL, drop) the learned u_tau network used by FC_WALL_MLP. */
+int fc_set_wall_mlp(fc_solver* s, const fc_mlp_params* params);
+
+/* ---- tracer particles (src/render/particles.cuh:29-82, lbm_core.cuh:403-407) */
+/* Particles are 4 floats each: x, y, z in lattice-cell space, age in [0,1]. */
+/* launchParticleSeed: (re)allocate count particles, uniform box seed (lowbias32). */
+int fc_particles_seed(fc_solver* s, int count, unsigned seed);
+/* launchParticleAdvectRK2 on the solver-owned particles through the macro
+ * velocity field of the last fc_step call (dt_steps lattice steps; per-call seed). */
+int fc_particles_advect(fc_solver* s, float dt_steps, unsigned seed);
+/* launchAdvectParticles: same advection on a caller-owned device buffer. */
+int fc_advect_particles(fc_solver* s, void* device_pos4, int count, float dt_steps, unsigned seed);
+int fc_particles_read(fc_solver* s, float* pos4);
+int fc_particles_write(fc_solver* s, const float* pos4);
+int fc_particles_count(fc_solver* s);
+/* ParticleAdvectParams ageRate / inletSpawnFraction (defaults 0.002 / 0.3). */
+int fc_particles_set_params(fc_solver* s, float age_rate, float inlet_fraction);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Per-launch HIP-event timing of the stream-collide kernel (on the solver's
  * stream); fc_kernel_timing returns launches and summed milliseconds since

@@ -411,6 +411,182 @@ void momentsOf(const T* f, const Grid& g, double* rho, double* u, double* v, dou
     }
 }
 
+
+// ---- iMEM wall model (lbm_wallmodel.cu:31-196) -----------------------------
+// Reichardt law, Newton inversion, EMA, linear slip balance, fade, clamp, FP16
+// encoding. The reference uses the __logf / __expf intrinsics; this uses libm
+// (parity to the intrinsics' error, not to the last bit).
+constexpr float kKappa = 0.41f, kReichC = 7.8f, kFadeY0 = 3.0f, kFadeY1 = 6.0f;
+
+float reichardtUPlus(float yp) {
+    return std::log(1.0f + kKappa * yp) / kKappa
+         + kReichC * (1.0f - std::exp(-yp / 11.0f) - (yp / 11.0f) * std::exp(-yp / 3.0f));
+}
+
+float reichardtDuPlus(float yp) {
+    return 1.0f / (1.0f + kKappa * yp)
+         + kReichC * ((1.0f / 11.0f) * std::exp(-yp / 11.0f) - (1.0f / 11.0f) * std::exp(-yp / 3.0f)
+                      + (yp / 33.0f) * std::exp(-yp / 3.0f));
+}
+
+float solveUTau(float ut, float ys, float nu, float seed) {
+    float utau = (seed > 0.0f) ? seed : std::max(0.04f * ut, std::sqrt(nu * ut / ys));
+    for (int it = 0; it < 6; ++it) {
+        const float yp = std::max(ys * utau / nu, 1e-8f);
+        const float up = reichardtUPlus(yp);
+        const float g = utau * up - ut;
+        const float dg = up + yp * reichardtDuPlus(yp);
+        utau -= g / std::max(dg, 1e-6f);
+        utau = std::max(utau, 0.0f);
+    }
+    return utau;
+}
+
+// IEEE binary32 -> binary16, round to nearest even (= __float2half_rn).
+uint16_t floatToHalfBits(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t absx = x & 0x7FFFFFFFu;
+    if (absx >= 0x7F800000u)  // inf / nan
+        return static_cast<uint16_t>(sign | 0x7C00u | (absx > 0x7F800000u ? 0x200u : 0u));
+    if (absx >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);  // rounds to inf
+    if (absx < 0x38800000u) {  // subnormal half (or zero)
+        if (absx < 0x33000000u) return static_cast<uint16_t>(sign);  // < 2^-25 -> 0
+        const uint32_t e = absx >> 23;
+        const uint32_t m = (absx & 0x7FFFFFu) | 0x800000u;
+        const uint32_t shift = 126u - e;  // 14 + (113 - e)
+        const uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1);
+        uint32_t r = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+        return static_cast<uint16_t>(sign | r);
+    }
+    uint32_t r = absx - 0x38000000u;  // rebias exponent (127 -> 15)
+    const uint32_t q = r >> 13;
+    const uint32_t rem = r & 0x1FFFu;
+    uint32_t h = q + ((rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ? 1u : 0u);
+    return static_cast<uint16_t>(sign | h);
+}
+
+template <class T>
+void wallUpdate(const T* f, const Grid& g, int count, const long long* cellIdx,
+                const long long* sampleIdx, const float* nxA, const float* nyA, const float* nzA,
+                const float* ysA, const uint32_t* maskA, float* uTau, float nu, float cutoff,
+                float ema, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz, float* stats4) {
+    const long long n = g.ncells();
+    float sumY = 0, maxY = 0;
+    int active = 0, clamped = 0;
+    for (int i = 0; i < count; ++i) {
+        const long long cell = cellIdx[i];
+        const float nX = nxA[i], nY = nyA[i], nZ = nzA[i];
+        const long long sc = sampleIdx[i];
+        float rho = 0, jx = 0, jy = 0, jz = 0;
+        for (int q = 0; q < kQ; ++q) {
+            const float fq = (float)f[(long long)q * n + sc];
+            rho += fq; jx += kCx[q] * fq; jy += kCy[q] * fq; jz += kCz[q] * fq;
+        }
+        rho = std::max(rho, 0.05f);
+        const float inv = 1.0f / rho;
+        const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+        const float un = ux * nX + uy * nY + uz * nZ;
+        const float tx = ux - un * nX, ty = uy - un * nY, tz = uz - un * nZ;
+        const float ut = std::sqrt(tx * tx + ty * ty + tz * tz);
+        if (ut < cutoff) {
+            uTau[i] = 0.0f;
+            uwx[cell] = uwy[cell] = uwz[cell] = 0;
+            continue;
+        }
+        const float itx = tx / ut, ity = ty / ut, itz = tz / ut;
+        const float prev = uTau[i];
+        const float solved = solveUTau(ut, ysA[i], nu, prev);
+        const float utau = (prev > 0.0f) ? prev + ema * (solved - prev) : solved;
+        uTau[i] = utau;
+        float m0 = 0, dc = 0;
+        for (int q = 1; q < kQ; ++q) {
+            if (!(maskA[i] & (1u << q))) continue;
+            const int o = kOpp[q];
+            const float ct = -(kCx[q] * itx + kCy[q] * ity + kCz[q] * itz);
+            m0 += 2.0f * (float)f[(long long)o * n + cell] * ct;
+            dc += 6.0f * weight<float>(q) * ct * ct;
+        }
+        const float maxN = std::max(std::fabs(nX), std::max(std::fabs(nY), std::fabs(nZ)));
+        const float aw = 1.0f / std::max(maxN, 0.1f);
+        float uw = 0.0f;
+        if (dc > 1e-6f) {
+            const float target = rho * utau * utau * aw;
+            const float sgn = (m0 >= 0.0f) ? 1.0f : -1.0f;
+            uw = (m0 - sgn * target) / dc;
+        }
+        const float yps = ysA[i] * utau / nu;
+        const float fade = std::min(std::max((yps - kFadeY0) / (kFadeY1 - kFadeY0), 0.0f), 1.0f);
+        uw *= fade;
+        const float cap = std::min(0.5f * ut, 0.5f * 0.30f);
+        const float uwc = std::min(std::max(uw, -cap), cap);
+        uwx[cell] = floatToHalfBits(uwc * itx);
+        uwy[cell] = floatToHalfBits(uwc * ity);
+        uwz[cell] = floatToHalfBits(uwc * itz);
+        sumY += yps;
+        maxY = std::max(maxY, yps);
+        ++active;
+        if (uwc != uw) ++clamped;
+    }
+    if (stats4) {
+        stats4[0] = sumY; stats4[1] = maxY; stats4[2] = (float)active; stats4[3] = (float)clamped;
+    }
+}
+
+
+// ---- tracer particles (particles.cu:48-376) ---------------------------------
+// lowbias32 hash, 24-bit uniform draws, trilinear velocity at p - 0.5 (x / y
+// clamped, z periodic), RK2 midpoint, respawn rules. Function arguments of
+// the reference's make_float3/4 calls are taken as evaluated left to right.
+uint32_t hashU32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+float rand01(uint32_t& s) {
+    s = hashU32(s);
+    return (float)(s & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+struct P3 { float x, y, z; };
+P3 lerp3(P3 a, P3 b, float t) {
+    return P3{a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t};
+}
+struct VelField {
+    const float *u, *v, *w;
+    int nx, ny, nz;
+    P3 at(int x, int y, int z) const {
+        int zz = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        const long long c = (long long)clampi(x, 0, nx - 1) + (long long)nx * ((long long)clampi(y, 0, ny - 1) + (long long)ny * zz);
+        return P3{u[c], v[c], w[c]};
+    }
+    P3 sample(P3 p) const {
+        const float gx = std::min(std::max(p.x - 0.5f, 0.0f), (float)nx - 1.0001f);
+        const float gy = std::min(std::max(p.y - 0.5f, 0.0f), (float)ny - 1.0001f);
+        float gz = p.z - 0.5f;
+        if (gz < 0.0f) gz += (float)nz;
+        const int x0 = (int)gx, y0 = (int)gy, z0 = (int)gz;
+        const float fx = gx - (float)x0, fy = gy - (float)y0, fz = gz - (float)z0;
+        const P3 x00 = lerp3(at(x0, y0, z0), at(x0 + 1, y0, z0), fx);
+        const P3 x10 = lerp3(at(x0, y0 + 1, z0), at(x0 + 1, y0 + 1, z0), fx);
+        const P3 x01 = lerp3(at(x0, y0, z0 + 1), at(x0 + 1, y0, z0 + 1), fx);
+        const P3 x11 = lerp3(at(x0, y0 + 1, z0 + 1), at(x0 + 1, y0 + 1, z0 + 1), fx);
+        return lerp3(lerp3(x00, x10, fy), lerp3(x01, x11, fy), fz);
+    }
+};
+bool finite3(P3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+P3 inletSpawn(int ny, int nz, uint32_t& r) {
+    const float a = 0.5f + 1.5f * rand01(r);
+    const float b = (float)ny * rand01(r);
+    const float c = (float)nz * rand01(r);
+    return P3{a, b, c};
+}
+
 Grid makeGrid(int nx, int ny, int nz, int px, int py) { return Grid{nx, ny, nz, px, py}; }
 
 }  // namespace
@@ -556,6 +732,85 @@ void orc_moments_f32(const float* f, int nx, int ny, int nz, double* rho, double
 void orc_moments_f64(const double* f, int nx, int ny, int nz, double* rho, double* u, double* v,
                      double* w) {
     momentsOf<double>(f, makeGrid(nx, ny, nz, 0, 0), rho, u, v, w);
+}
+
+
+float orc_reichardt_uplus(float yp) { return reichardtUPlus(yp); }
+float orc_solve_utau(float ut, float ys, float nu, float seed) { return solveUTau(ut, ys, nu, seed); }
+uint16_t orc_float_to_half(float f) { return floatToHalfBits(f); }
+
+void orc_wall_update_f32(const float* f, int nx, int ny, int nz, int count, const long long* cellIdx,
+                         const long long* sampleIdx, const float* nxA, const float* nyA,
+                         const float* nzA, const float* ys, const uint32_t* mask, float* uTau,
+                         float nu, float cutoff, float ema, uint16_t* uwx, uint16_t* uwy,
+                         uint16_t* uwz, float* stats4) {
+    wallUpdate<float>(f, makeGrid(nx, ny, nz, 0, 0), count, cellIdx, sampleIdx, nxA, nyA, nzA, ys,
+                      mask, uTau, nu, cutoff, ema, uwx, uwy, uwz, stats4);
+}
+
+
+void orc_particles_seed(float* pos4, int count, int nx, int ny, int nz, uint32_t seed) {
+    for (int i = 0; i < count; ++i) {
+        uint32_t r = hashU32(seed ^ ((uint32_t)i * 0x9E3779B9u));
+        const float a = (float)nx * rand01(r);
+        const float b = (float)ny * rand01(r);
+        const float c = (float)nz * rand01(r);
+        const float e = rand01(r);
+        pos4[4 * i] = a; pos4[4 * i + 1] = b; pos4[4 * i + 2] = c; pos4[4 * i + 3] = e;
+    }
+}
+
+void orc_particles_advect(float* pos4, int count, const float* u, const float* v, const float* w,
+                          const uint8_t* flags, int nx, int ny, int nz, float dt, float ageRate,
+                          float inletFrac, uint32_t seed) {
+    const VelField vf{u, v, w, nx, ny, nz};
+    const float nzf = (float)nz;
+    auto cellOf = [&](int x, int y, int z) {
+        return (long long)x + (long long)nx * ((long long)y + (long long)ny * z);
+    };
+    for (int i = 0; i < count; ++i) {
+        uint32_t r = hashU32(seed ^ ((uint32_t)i * 0x9E3779B9u));
+        P3 p{pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]};
+        float age = pos4[4 * i + 3];
+        bool blew = !finite3(p);
+        if (!blew && dt > 0.0f) {
+            const P3 u1 = vf.sample(p);
+            P3 pm{p.x + 0.5f * dt * u1.x, p.y + 0.5f * dt * u1.y, p.z + 0.5f * dt * u1.z};
+            if (pm.z < 0.0f) pm.z += nzf; else if (pm.z >= nzf) pm.z -= nzf;
+            const P3 u2 = vf.sample(pm);
+            p = P3{p.x + dt * u2.x, p.y + dt * u2.y, p.z + dt * u2.z};
+            blew = !finite3(p) || !finite3(u2);
+            age -= ageRate * dt;
+        }
+        if (!blew) {
+            if (p.z < 0.0f) p.z += nzf; else if (p.z >= nzf) p.z -= nzf;
+        }
+        const bool outflow = blew || p.x >= (float)nx - 1.0f;
+        const bool oob = !outflow && (p.x < 0.0f || p.y < 0.0f || p.y >= (float)ny);
+        const int cx = clampi((int)p.x, 0, nx - 1), cy = clampi((int)p.y, 0, ny - 1),
+                  cz = clampi((int)p.z, 0, nz - 1);
+        const bool solid = !outflow && !oob && flags && flags[cellOf(cx, cy, cz)] == kSolid;
+        if (outflow || oob || solid || age <= 0.0f) {
+            const bool toInlet = outflow || oob || rand01(r) < inletFrac;
+            if (toInlet) {
+                p = inletSpawn(ny, nz, r);
+            } else {
+                bool done = false;
+                for (int attempt = 0; attempt < 4 && !done; ++attempt) {
+                    const float a = (float)nx * rand01(r);
+                    const float b = (float)ny * rand01(r);
+                    const float c = (float)nz * rand01(r);
+                    if (!flags) { p = P3{a, b, c}; done = true; break; }
+                    const long long cc = cellOf(clampi((int)a, 0, nx - 1), clampi((int)b, 0, ny - 1),
+                                                clampi((int)c, 0, nz - 1));
+                    if (flags[cc] != kSolid) { p = P3{a, b, c}; done = true; }
+                }
+                if (!done) p = inletSpawn(ny, nz, r);
+            }
+            age = 0.6f + 0.4f * rand01(r);
+        }
+        pos4[4 * i] = p.x; pos4[4 * i + 1] = p.y; pos4[4 * i + 2] = p.z; pos4[4 * i + 3] = age;
+    }
 }
 
 }  // extern "C"

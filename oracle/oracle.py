@@ -74,6 +74,19 @@ def lib():
         L.orc_force_f64.argtypes = [_dp] + common + [_u16p] * 3 + [_dp]
         L.orc_watchdog_f32.restype = C.c_int
         L.orc_watchdog_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_int]
+        L.orc_reichardt_uplus.restype = C.c_float
+        L.orc_reichardt_uplus.argtypes = [C.c_float]
+        L.orc_solve_utau.restype = C.c_float
+        L.orc_solve_utau.argtypes = [C.c_float] * 4
+        L.orc_float_to_half.restype = C.c_uint16
+        L.orc_float_to_half.argtypes = [C.c_float]
+        _llp = C.POINTER(C.c_longlong)
+        L.orc_wall_update_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int, C.c_int, _llp, _llp,
+                                          _fp, _fp, _fp, _fp, _u32p, _fp, C.c_float, C.c_float,
+                                          C.c_float, _u16p, _u16p, _u16p, _fp]
+        L.orc_particles_seed.argtypes = [_fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32]
+        L.orc_particles_advect.argtypes = [_fp, C.c_int, _fp, _fp, _fp, _u8p, C.c_int, C.c_int,
+                                           C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint32]
         L.orc_moments_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
         L.orc_moments_f64.argtypes = [_dp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
         _lib = L
@@ -228,3 +241,49 @@ def ramp_schedule(tau, nu_lat, u_lat, nx, steps, start=0, ramp=True):
             sm = t * t * (f32(3.0) - f32(2.0) * t)
             uins[i] = f32(u_lat) * sm
     return taus, uins
+
+
+# ---- wall model ---------------------------------------------------------------
+def reichardt_uplus(yp):
+    return lib().orc_reichardt_uplus(yp)
+
+
+def solve_utau(ut, ys, nu, seed=0.0):
+    return lib().orc_solve_utau(ut, ys, nu, seed)
+
+
+def float_to_half_bits(x):
+    return lib().orc_float_to_half(x)
+
+
+def wall_update(f, dims, wl, utau, nu, cutoff, ema=0.25):
+    """One wall-model update (lbm_wallmodel.cu:76-196) on canonical FP32 f.
+    wl: dict of SoA arrays (cell, sample, nx, ny, nz, ys, mask); utau updated
+    in place. Returns (uwx, uwy, uwz half bits, stats[4])."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    uw = [np.zeros(n, np.uint16) for _ in range(3)]
+    st = np.zeros(4, np.float32)
+    ll = C.POINTER(C.c_longlong)
+    f32 = np.ascontiguousarray(f, np.float32)
+    lib().orc_wall_update_f32(_p(f32, _fp), nx, ny, nz, len(wl["cell"]),
+                              wl["cell"].ctypes.data_as(ll), wl["sample"].ctypes.data_as(ll),
+                              _p(wl["nx"], _fp), _p(wl["ny"], _fp), _p(wl["nz"], _fp),
+                              _p(wl["ys"], _fp), _p(wl["mask"], _u32p), _p(utau, _fp), nu, cutoff,
+                              ema, *[_p(a, _u16p) for a in uw], _p(st, _fp))
+    return uw[0], uw[1], uw[2], st
+
+
+# ---- tracer particles -----------------------------------------------------------
+def particles_seed(count, dims, seed):
+    pos = np.zeros((count, 4), np.float32)
+    lib().orc_particles_seed(_p(pos, _fp), count, *dims, seed)
+    return pos
+
+
+def particles_advect(pos, u, v, w, flags, dims, dt, seed, age_rate=0.002, inlet_fraction=0.3):
+    """RK2 advection in place (pos: (n,4) float32)."""
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
+    lib().orc_particles_advect(_p(pos, _fp), len(pos), *[_p(a, _fp) for a in arrs],
+                               _p(fl, _u8p), *dims, dt, age_rate, inlet_fraction, seed)

@@ -291,3 +291,92 @@ def test_f32_oracle_tracks_f64():
     orc.run(f32, flags, d, taus.astype(np.float32), np.zeros(20, np.float32), cs=0.1733,
             periodic_xy=True)
     assert np.abs(f32 - f64).max() < 1e-6
+
+
+def test_reichardt_inversion_recovers_planted_utau():
+    """Plant u_tau, evaluate u_t = u_tau u+(y_s u_tau / nu), invert."""
+    for nu in (1.667e-4, 2e-3, 0.01):
+        for ys in (0.5, 1.5, 2.5, 3.5):
+            for yplus in (0.5, 2.0, 8.0, 30.0, 120.0, 600.0):
+                utau = yplus * nu / ys
+                ut = utau * orc.reichardt_uplus(yplus)
+                got = orc.solve_utau(ut, ys, nu, 0.0)
+                assert got == pytest.approx(utau, rel=2e-4), (nu, ys, yplus)
+                # warm start from a nearby value converges too
+                assert orc.solve_utau(ut, ys, nu, 1.3 * utau) == pytest.approx(utau, rel=2e-4)
+
+
+def test_reichardt_profile_limits():
+    # viscous sublayer u+ -> y+ ; log layer u+ -> ln(y+)/kappa + B (B ~ 5.6)
+    assert orc.reichardt_uplus(0.05) == pytest.approx(0.05, rel=2e-2)
+    yp = 2000.0
+    assert orc.reichardt_uplus(yp) == pytest.approx(np.log(0.41 * yp) / 0.41 + 7.8, rel=1e-3)
+
+
+def test_float_to_half_round_nearest_even():
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([
+        rng.standard_normal(4000).astype(np.float32) * np.float32(1e-2),
+        rng.standard_normal(2000).astype(np.float32) * np.float32(1e-6),   # subnormal halves
+        (rng.random(500) * 70000).astype(np.float32),                      # overflow edge
+        np.array([0.0, -0.0, 65504, 65520, 65519.996, 6.1035156e-05, 5.96e-08, 2.98e-08,
+                  np.inf, -np.inf], np.float32)])
+    # exact ties: values halfway between adjacent halves
+    h = np.arange(1, 30000, 7, dtype=np.uint16).view(np.float16).astype(np.float32)
+    h2 = (np.arange(2, 30001, 7, dtype=np.uint16)).view(np.float16).astype(np.float32)
+    xs = np.concatenate([xs, ((h.astype(np.float64) + h2) / 2).astype(np.float32)])
+    for x in xs:
+        got = orc.float_to_half_bits(float(x))
+        want = int(np.array([x], np.float32).astype(np.float16).view(np.uint16)[0])
+        assert got == want, (x, got, want)
+
+
+# ---- tracer particles --------------------------------------------------------
+def test_particle_seed_uniform_and_in_box():
+    d = (64, 32, 16)
+    pos = orc.particles_seed(200000, d, 0xF01DCAFE)
+    assert (pos[:, 0] >= 0).all() and (pos[:, 0] < 64).all()
+    assert (pos[:, 1] < 32).all() and (pos[:, 2] < 16).all()
+    assert (pos[:, 3] >= 0).all() and (pos[:, 3] < 1).all()
+    # 24-bit draws times the extent: every coordinate is k * extent / 2^24
+    k = pos[:, 0].astype(np.float64) * (2 ** 24) / 64
+    assert np.array_equal(k, np.round(k))
+    h, _ = np.histogram(pos[:, 0], bins=16, range=(0, 64))
+    chi2 = ((h - h.mean()) ** 2 / h.mean()).sum()
+    assert chi2 < 40  # 15 dof
+    other = orc.particles_seed(1000, d, 0xF01DCAFF)
+    assert not np.array_equal(other, pos[:1000])
+    assert np.array_equal(orc.particles_seed(1000, d, 0xF01DCAFE), pos[:1000])
+
+
+def test_particle_uniform_flow_translation():
+    d = (64, 32, 16)
+    n = np.prod(d)
+    u = np.full(n, 0.05, np.float32)
+    v = np.full(n, -0.02, np.float32)
+    w = np.full(n, 0.03, np.float32)
+    pos = np.array([[10.25, 12.5, 3.0, 0.9], [30.0, 20.0, 15.9, 0.5]], np.float32)
+    before = pos.copy()
+    orc.particles_advect(pos, u, v, w, None, d, 8.0, 123)
+    assert np.allclose(pos[0, :3], before[0, :3] + 8 * np.array([0.05, -0.02, 0.03]), atol=1e-5)
+    # z wraps periodically (15.9 + 0.24 -> 0.14)
+    assert pos[1, 2] == pytest.approx((15.9 + 0.24) - 16.0, abs=1e-5)
+    assert pos[0, 3] == pytest.approx(0.9 - 0.002 * 8, abs=1e-7)
+
+
+def test_particle_respawn_rules():
+    d = (64, 32, 16)
+    n = np.prod(d)
+    flags = np.zeros(n, np.uint8)
+    flags[(20 + 64 * (10 + 32 * 5))] = 1  # one solid cell at (20,10,5)
+    zero = np.zeros(n, np.float32)
+    pos = np.array([[63.5, 5.0, 5.0, 0.5],    # beyond the outflow line -> inlet
+                    [20.5, 10.5, 5.5, 0.5],   # inside the solid cell -> respawn
+                    [30.0, 5.0, 5.0, 0.001],  # lifetime exhausted
+                    [30.0, 5.0, 5.0, 0.5]],   # healthy: stays
+                   np.float32)
+    orc.particles_advect(pos, zero, zero, zero, flags, d, 1.0, 77)
+    assert 0.5 <= pos[0, 0] < 2.0 and 0.6 <= pos[0, 3] <= 1.0
+    assert not (20 <= pos[1, 0] < 21 and 10 <= pos[1, 1] < 11 and 5 <= pos[1, 2] < 6)
+    assert 0.6 <= pos[1, 3] <= 1.0 and 0.6 <= pos[2, 3] <= 1.0
+    assert pos[3, 0] == 30.0 and pos[3, 3] == pytest.approx(0.498)
