@@ -79,6 +79,19 @@ class FcMlpParams(C.Structure):
                 ("w3", C.c_uint16 * 32), ("b3", C.c_float)]
 
 
+class FcVGParams(C.Structure):
+    _fields_ = [("type", C.c_int), ("x_c", C.c_float), ("height_c", C.c_float),
+                ("length_h", C.c_float), ("beta_deg", C.c_float), ("pitch_c", C.c_float),
+                ("gap_h", C.c_float), ("count", C.c_int), ("common_flow_down", C.c_int),
+                ("enabled", C.c_int)]
+
+
+class FcVaneSlab(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("n_axis", C.c_float * 3), ("d_axis", C.c_float * 3),
+                ("b_axis", C.c_float * 3), ("half_height", C.c_float),
+                ("half_length", C.c_float), ("half_thick", C.c_float)]
+
+
 class FcInitOptions(C.Structure):
     _fields_ = [("storage", C.c_int), ("periodic_x", C.c_int), ("periodic_y", C.c_int)]
 
@@ -116,6 +129,13 @@ SIGNATURES = {
                                             C.POINTER(FcPatchBox), C.c_int, _u8p]),
     "fc_derive_patch_box": (C.c_int, [FcDims, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(FcPatchBox)]),
+    "fc_surface_frame_at": (C.c_int, [_fp, C.c_int, C.c_int, C.c_float, C.c_int, _fp]),
+    "fc_build_flags_with_vgs": (C.c_int, [_fp, C.c_int, C.c_int, C.c_float, C.POINTER(FcLayout),
+                                          C.POINTER(FcVGParams), C.c_int, _u8p, _u8p,
+                                          C.POINTER(FcVaneSlab), C.c_int, _ip]),
+    "fc_build_vane_qlinks": (C.c_int, [FcDims, _u8p, _u8p, C.POINTER(FcVaneSlab), C.c_int, _u8p,
+                                       C.POINTER(C.c_uint32)]),
+    "fc_ray_obb_entry": (C.c_float, [_fp, _fp, C.POINTER(FcVaneSlab)]),
     "fc_create": (_vp, []),
     "fc_destroy": (None, [_vp]),
     "fc_init": (C.c_int, [_vp, FcDims, C.POINTER(FcScaling), _u8p, C.POINTER(FcInitOptions),
@@ -161,6 +181,9 @@ SIGNATURES = {
     "fc_set_wall_harvest": (C.c_int, [_vp, C.c_int]),
     "fc_wall_harvest": (C.c_int, [_vp, _fp, C.c_int]),
     "fc_set_wall_mlp": (C.c_int, [_vp, C.POINTER(FcMlpParams)]),
+    "fc_set_qlibb_enabled": (C.c_int, [_vp, C.c_int]),
+    "fc_set_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
+    "fc_qlibb_readout": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "fc_particles_seed": (C.c_int, [_vp, C.c_int, C.c_uint]),
     "fc_particles_advect": (C.c_int, [_vp, C.c_float, C.c_uint]),
     "fc_advect_particles": (C.c_int, [_vp, _vp, C.c_int, C.c_float, C.c_uint]),

@@ -68,11 +68,18 @@ def tgv_kinetic_energy(u, v, w) -> float:
 
 def naca_tunnel(nx: int, ny: int, nz: int, chord: int, aoa: float = 8.0, u_lat: float = 0.08,
                 cs: float = 0.1, chord_m: float = 1.2, airspeed: float = 30.0,
-                nu: float = 1.48e-5, digits: str = "2412", ramp: bool = True) -> Case:
+                nu: float = 1.48e-5, digits: str = "2412", ramp: bool = True,
+                tau: float | None = None) -> Case:
+    """Wind tunnel around an extruded NACA section. With ``tau`` the physical
+    scaling is replaced by one whose lattice viscosity gives exactly that
+    relaxation time (a laminar, low-Reynolds variant of the same geometry)."""
     pts, _ = S.generate_naca4(digits, 200)
     layout = S.make_layout(nx, ny, nz, chord)
     flags = S.build_clean_foil_flags(pts, aoa, layout)
-    sc = S.compute_scaling(chord_m, airspeed, nu, chord, u_lat)
+    if tau is None:
+        sc = S.compute_scaling(chord_m, airspeed, nu, chord, u_lat)
+    else:
+        sc = S.scaling_for_tau(tau, u_lat, chord)
     return Case(f"naca{digits}_{nx}x{ny}x{nz}_c{chord}_a{aoa:g}", (nx, ny, nz), flags, sc,
                 cs=cs, ramp=ramp)
 

@@ -135,4 +135,9 @@ int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, voi
 int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
                          const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, void* stream);
 
+// ---- q-LIBB pre-pass ----------------------------------------------------------
+// Links: cell index + packed (q | byte << 8 | ff << 16); scratch: count doubles.
+int launchQLibbPrepass(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* packed,
+                       int count, double* scratch, void* stream);
+
 }  // namespace fc

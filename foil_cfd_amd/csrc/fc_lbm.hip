@@ -313,11 +313,23 @@ __global__ void __launch_bounds__(kBlock) speckKernel(KArgs a, float amp, float 
     const float wz = amp * (0.6f * __sinf(3.0f * tz + ph1) + 0.4f * __sinf(7.0f * tz + ph2)) *
                      (1.0f + 0.3f * __sinf(2.0f * tx + ph3));
     const C K = C(S::K);
+    // A cell that feeds the outlet also refreshes both feed-plane copies, so
+    // the first outlet copy sees the perturbed column as the reference's
+    // direct x-1 read does.
+    const bool feed = (a.kind[c] & kKindFeed) != 0;
+    T* const fd = static_cast<T*>(a.feed);
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+    const unsigned yz = static_cast<unsigned>(p.y + a.ny * p.z);
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         if (kCz[q] == 0) continue;
         T* slot = f + static_cast<long long>(kOpp[q]) * a.ncells + c;
-        S::store(slot, S::load(slot) + K * C(3) * wOf<C>(q) * C(kCz[q]) * C(wz));
+        const C h = S::load(slot) + K * C(3) * wOf<C>(q) * C(kCz[q]) * C(wz);
+        S::store(slot, h);
+        if (feed) {
+            S::store(fd + q * plane + yz, h);
+            S::store(fd + (kQ + q) * plane + yz, h);
+        }
     }
 }
 

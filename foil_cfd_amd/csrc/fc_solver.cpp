@@ -21,6 +21,7 @@
 #include "fc_geometry.hpp"
 #include "fc_kernels.hpp"
 #include "fc_units.hpp"
+#include "fc_vg.hpp"
 #include "fc_wall.hpp"
 
 namespace fc {
@@ -236,6 +237,7 @@ public:
         wm_.free();
         hipRelease(dMlp_);
         hipRelease(dParticles_);
+        freeQLinks();
         particleCount_ = 0;
         if (hForce_) (void)hipHostFree(hForce_);
         if (hNan_) (void)hipHostFree(hNan_);
@@ -333,6 +335,9 @@ public:
                 ev = nextKernelEvents();
                 if (ev) (void)hipEventRecord(ev->first, stream_);
             }
+            if (qCount_ > 0 && qlibbEnabled_ &&
+                launchQLibbPrepass(lat, layout_, dQCells_, dQPacked_, qCount_, dQScratch_, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "q-LIBB pre-pass failed");
             const int e = launchStep(lat, layout_, a, stream_);
             if (ev) (void)hipEventRecord(ev->second, stream_);
             if (e != 0) return fail(FC_ERR_DEVICE, std::string("step launch failed: ") +
@@ -369,6 +374,84 @@ public:
             if (int e = wallUpdate(); e != FC_OK) return e;
         }
         return FC_OK;
+    }
+
+    // ---- q-LIBB (reference setQLIBBEnabled / setCoarseQLinks, lbm_solver.cpp:1027-1080)
+    void freeQLinks() {
+        hipRelease(dQCells_);
+        hipRelease(dQPacked_);
+        hipRelease(dQScratch_);
+        qCount_ = 0;
+    }
+
+    int setQLibbEnabled(bool on) {
+        qlibbEnabled_ = on;
+        if (!on) {
+            freeQLinks();
+            qLinks_ = qFallback_ = 0;
+        }
+        return FC_OK;
+    }
+
+    // Dense per-cell-per-direction cut bytes (cells*19) + per-cell two-node
+    // mask -> the sparse link list the pre-pass walks. Links whose result is
+    // the plain half-way value (qf < 1/2 without a second fluid node) are
+    // dropped from the list.
+    int setQLinks(const std::uint8_t* qFrac, const std::uint32_t* ffMask, int links, int fallback) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        freeQLinks();
+        qLinks_ = qFallback_ = 0;
+        if (!qlibbEnabled_ || !qFrac || !ffMask) return FC_OK;
+        std::vector<int> cells;
+        std::vector<std::uint32_t> packed;
+        for (int z = 0; z < dims_.nz; ++z)
+            for (int y = 0; y < dims_.ny; ++y)
+                for (int x = 0; x < dims_.nx; ++x) {
+                    const std::size_t c = at(x, y, z);
+                    if (hostFlags_[c] != kFluid) continue;
+                    for (int q = 1; q < kQ; ++q) {
+                        const std::uint8_t b = qFrac[c * kQ + q];
+                        if (!b) continue;
+                        const int xn = stepAxis(x, -kCx[q], dims_.nx, periodicX_);
+                        const int yn = stepAxis(y, -kCy[q], dims_.ny, periodicY_);
+                        const int zn = stepAxis(z, -kCz[q], dims_.nz, true);
+                        if (hostFlags_[at(xn, yn, zn)] != kSolid) continue;
+                        const bool ff = (ffMask[c] >> q) & 1u;
+                        const float qf = static_cast<float>(b) * (1.0f / 255.0f);
+                        if (qf < 0.5f && !ff) continue;
+                        cells.push_back(static_cast<int>(c));
+                        packed.push_back(static_cast<std::uint32_t>(q) | (static_cast<std::uint32_t>(b) << 8) |
+                                         (ff ? (1u << 16) : 0u));
+                    }
+                }
+        const std::size_t n = cells.size();
+        if (n > 0) {
+            int rc = hipAlloc(&dQCells_, n * sizeof(int));
+            rc |= hipAlloc(&dQPacked_, n * sizeof(std::uint32_t));
+            rc |= hipAlloc(&dQScratch_, n * sizeof(double));
+            if (rc != FC_OK) {
+                freeQLinks();
+                return fail(FC_ERR_ALLOC, "q-link allocation failed");
+            }
+            if (hipMemcpyAsync(dQCells_, cells.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                hipMemcpyAsync(dQPacked_, packed.data(), n * sizeof(std::uint32_t), hipMemcpyHostToDevice, stream_) !=
+                    hipSuccess ||
+                hipStreamSynchronize(stream_) != hipSuccess) {
+                freeQLinks();
+                return fail(FC_ERR_DEVICE, "q-link upload failed");
+            }
+            qCount_ = static_cast<int>(n);
+        }
+        qLinks_ = links;
+        qFallback_ = fallback;
+        return FC_OK;
+    }
+
+    void qlibbReadout(int* enabled, int* links, int* fallback, int* active) const {
+        if (enabled) *enabled = qlibbEnabled_ ? 1 : 0;
+        if (links) *links = qLinks_;
+        if (fallback) *fallback = qFallback_;
+        if (active) *active = qCount_;
     }
 
     // ---- tracer particles (reference particles.cu / viz.cpp) ---------------
@@ -1125,6 +1208,11 @@ private:
     bool harvestOn_ = false;
     MlpParams* dMlp_ = nullptr;
     float* dParticles_ = nullptr;
+    bool qlibbEnabled_ = false;
+    int* dQCells_ = nullptr;
+    std::uint32_t* dQPacked_ = nullptr;
+    double* dQScratch_ = nullptr;
+    int qCount_ = 0, qLinks_ = 0, qFallback_ = 0;
     int particleCount_ = 0;
     ParticleParams particleParams_;
     long long ncells_ = 0;
@@ -1540,6 +1628,123 @@ int fc_particles_set_params(fc_solver* s, float age_rate, float inlet_fraction) 
     if (!s) return FC_ERR_INVALID;
     s->impl.setParticleParams(age_rate, inlet_fraction);
     return FC_OK;
+}
+
+}  // extern "C"
+
+// ---- q-LIBB C ABI -------------------------------------------------------------
+extern "C" {
+
+int fc_set_qlibb_enabled(fc_solver* s, int enabled) {
+    return s ? s->impl.setQLibbEnabled(enabled != 0) : FC_ERR_INVALID;
+}
+int fc_set_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback) {
+    return s ? s->impl.setQLinks(qfrac, ffmask, links, fallback) : FC_ERR_INVALID;
+}
+int fc_qlibb_readout(fc_solver* s, int* enabled, int* links, int* fallback, int* active_links) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.qlibbReadout(enabled, links, fallback, active_links);
+    return FC_OK;
+}
+
+}  // extern "C"
+
+// ---- VG geometry C ABI ------------------------------------------------------------
+namespace {
+fc::VaneSlab slabFromC(const fc_vane_slab& c) {
+    fc::VaneSlab s;
+    s.center = fc::Vec3{c.center[0], c.center[1], c.center[2]};
+    s.nAxis = fc::Vec3{c.n_axis[0], c.n_axis[1], c.n_axis[2]};
+    s.dAxis = fc::Vec3{c.d_axis[0], c.d_axis[1], c.d_axis[2]};
+    s.bAxis = fc::Vec3{c.b_axis[0], c.b_axis[1], c.b_axis[2]};
+    s.halfHeight = c.half_height;
+    s.halfLength = c.half_length;
+    s.halfThick = c.half_thick;
+    return s;
+}
+fc_vane_slab slabToC(const fc::VaneSlab& s) {
+    fc_vane_slab c{};
+    const fc::Vec3* v[4] = {&s.center, &s.nAxis, &s.dAxis, &s.bAxis};
+    float* o[4] = {c.center, c.n_axis, c.d_axis, c.b_axis};
+    for (int i = 0; i < 4; ++i) {
+        o[i][0] = v[i]->x;
+        o[i][1] = v[i]->y;
+        o[i][2] = v[i]->z;
+    }
+    c.half_height = s.halfHeight;
+    c.half_length = s.halfLength;
+    c.half_thick = s.halfThick;
+    return c;
+}
+fc::Airfoil foilWithLe(const float* xy, int npts, int le) {
+    fc::Airfoil a = fc::airfoilFrom(xy, npts);
+    a.leIndex = le;
+    return a;
+}
+}  // namespace
+
+extern "C" {
+
+int fc_surface_frame_at(const float* xy, int npts, int le_index, float x_c, int upper, float* out6) {
+    if (!xy || !out6 || npts <= 0) return 0;
+    const fc::SurfaceFrame f = fc::surfaceFrameAt(foilWithLe(xy, npts, le_index), x_c, upper != 0);
+    out6[0] = f.point.x;
+    out6[1] = f.point.y;
+    out6[2] = f.tangent.x;
+    out6[3] = f.tangent.y;
+    out6[4] = f.normal.x;
+    out6[5] = f.normal.y;
+    return f.valid ? 1 : 0;
+}
+
+int fc_build_flags_with_vgs(const float* xy, int npts, int le_index, float aoa_deg, const fc_layout* layout,
+                            const fc_vg_params* vgs, int nvg, const uint8_t* clean_flags, uint8_t* flags_out,
+                            fc_vane_slab* slabs, int slab_cap, int* nslabs) {
+    if (!xy || !layout || !clean_flags || !flags_out || (nvg > 0 && !vgs)) return FC_ERR_INVALID;
+    const fc::DomainLayout l = fc::fromC(*layout);
+    std::vector<fc::VGParams> v(static_cast<std::size_t>(std::max(nvg, 0)));
+    for (int i = 0; i < nvg; ++i) {
+        v[i].type = vgs[i].type;
+        v[i].x_c = vgs[i].x_c;
+        v[i].height_c = vgs[i].height_c;
+        v[i].length_h = vgs[i].length_h;
+        v[i].beta_deg = vgs[i].beta_deg;
+        v[i].pitch_c = vgs[i].pitch_c;
+        v[i].gap_h = vgs[i].gap_h;
+        v[i].count = vgs[i].count;
+        v[i].commonFlowDown = vgs[i].common_flow_down != 0;
+        v[i].enabled = vgs[i].enabled != 0;
+    }
+    const std::vector<std::uint8_t> clean(clean_flags, clean_flags + l.dims.cells());
+    std::vector<fc::VaneSlab> sl;
+    const auto out = fc::buildFlagsWithVGs(v, foilWithLe(xy, npts, le_index), aoa_deg, l, clean, &sl);
+    std::memcpy(flags_out, out.data(), out.size());
+    if (nslabs) *nslabs = static_cast<int>(sl.size());
+    if (slabs)
+        for (int i = 0; i < std::min(slab_cap, static_cast<int>(sl.size())); ++i) slabs[i] = slabToC(sl[i]);
+    return FC_OK;
+}
+
+int fc_build_vane_qlinks(fc_dims dims, const uint8_t* active, const uint8_t* clean, const fc_vane_slab* slabs,
+                         int nslabs, uint8_t* qfrac, uint32_t* ffmask) {
+    if (!active || !clean || !qfrac || !ffmask || (nslabs > 0 && !slabs)) return -1;
+    const fc::Dims d{dims.nx, dims.ny, dims.nz};
+    const std::vector<std::uint8_t> a(active, active + d.cells()), c(clean, clean + d.cells());
+    std::vector<fc::VaneSlab> sl;
+    for (int i = 0; i < nslabs; ++i) sl.push_back(slabFromC(slabs[i]));
+    const fc::QLinkList ql = fc::buildVaneQLinks(d, a, c, sl);
+    std::vector<std::uint8_t> qf;
+    std::vector<std::uint32_t> ff;
+    fc::densifyQLinks(ql, d.cells(), qf, ff);
+    std::memcpy(qfrac, qf.data(), qf.size());
+    std::memcpy(ffmask, ff.data(), ff.size() * sizeof(std::uint32_t));
+    return static_cast<int>(ql.size());
+}
+
+float fc_ray_obb_entry(const float* origin3, const float* dir3, const fc_vane_slab* slab) {
+    if (!origin3 || !dir3 || !slab) return -1.0f;
+    return fc::rayOBBEntry(fc::Vec3{origin3[0], origin3[1], origin3[2]}, fc::Vec3{dir3[0], dir3[1], dir3[2]},
+                           slabFromC(*slab));
 }
 
 }  // extern "C"

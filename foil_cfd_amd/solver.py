@@ -178,6 +178,57 @@ def build_wall_cell_list(dims, active, clean=None, sample_cells=2):
 
 WALL_OFF, WALL_NEWTON, WALL_MLP = 0, 1, 2
 
+VG_SINGLE, VG_PAIR, VG_COROTATING, VG_RAMP = 0, 1, 2, 3
+
+
+def vg_params(type=VG_PAIR, x_c=0.07, height_c=0.01, length_h=3.0, beta_deg=16.0, pitch_c=0.06,
+              gap_h=2.5, count=4, common_flow_down=True, enabled=True):
+    """VGParams (vg.h:24-36; defaults = defaultVGParams, vg.h:38-50)."""
+    from ._abi import FcVGParams
+    return FcVGParams(type, x_c, height_c, length_h, beta_deg, pitch_c, gap_h, count,
+                      1 if common_flow_down else 0, 1 if enabled else 0)
+
+
+def surface_frame_at(points, le_index, x_c, upper=True):
+    pts = np.ascontiguousarray(points, np.float32)
+    out = np.zeros(6, np.float32)
+    ok = load().fc_surface_frame_at(_f32(pts), len(pts), le_index, x_c, 1 if upper else 0,
+                                    _f32(out))
+    return (out[0:2], out[2:4], out[4:6]) if ok else None
+
+
+def build_flags_with_vgs(points, le_index, aoa_deg, layout, vgs, clean):
+    """buildFlagsWithVGs (vg.cpp:246-261) -> (flags, list of FcVaneSlab)."""
+    from ._abi import FcVaneSlab, FcVGParams
+    pts = np.ascontiguousarray(points, np.float32)
+    cl = np.ascontiguousarray(clean, np.uint8).reshape(-1)
+    out = np.zeros_like(cl)
+    arr = (FcVGParams * max(len(vgs), 1))(*vgs)
+    cap = 4096
+    slabs = (FcVaneSlab * cap)()
+    ns = C.c_int(0)
+    check(load().fc_build_flags_with_vgs(_f32(pts), len(pts), le_index, aoa_deg, C.byref(layout),
+                                         arr, len(vgs), _u8(cl), _u8(out), slabs, cap,
+                                         C.byref(ns)), "fc_build_flags_with_vgs")
+    return out, [slabs[i] for i in range(min(ns.value, cap))]
+
+
+def build_vane_qlinks(dims, active, clean, slabs):
+    """buildVaneQLinks + densifyQLinks (vg.cpp:300-397) -> (qfrac, ffmask, links)."""
+    from ._abi import FcVaneSlab
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    a = np.ascontiguousarray(active, np.uint8).reshape(-1)
+    c = np.ascontiguousarray(clean, np.uint8).reshape(-1)
+    qf = np.zeros(n * Q, np.uint8)
+    ff = np.zeros(n, np.uint32)
+    arr = (FcVaneSlab * max(len(slabs), 1))(*slabs)
+    links = load().fc_build_vane_qlinks(FcDims(nx, ny, nz), _u8(a), _u8(c), arr, len(slabs),
+                                        _u8(qf), ff.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if links < 0:
+        raise RuntimeError("fc_build_vane_qlinks failed")
+    return qf, ff, links
+
 
 # --------------------------------------------------------------- solver ----
 @dataclass
@@ -363,6 +414,24 @@ class LBMSolver:
     def set_wall_mlp(self, params):
         check(self._lib.fc_set_wall_mlp(self._h, None if params is None else C.byref(params)),
               "fc_set_wall_mlp")
+
+    # q-LIBB ------------------------------------------------------------------
+    def set_qlibb_enabled(self, on: bool):
+        check(self._lib.fc_set_qlibb_enabled(self._h, 1 if on else 0), "fc_set_qlibb_enabled")
+
+    def set_qlinks(self, qfrac, ffmask, links=0, fallback=0):
+        qf = np.ascontiguousarray(qfrac, np.uint8).reshape(-1)
+        ff = np.ascontiguousarray(ffmask, np.uint32).reshape(-1)
+        if qf.size != Q * self.ncells or ff.size != self.ncells:
+            raise ValueError("q-link field size mismatch")
+        check(self._lib.fc_set_qlinks(self._h, _u8(qf), ff.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      links, fallback), "fc_set_qlinks")
+
+    def qlibb_readout(self):
+        v = [C.c_int(0) for _ in range(4)]
+        check(self._lib.fc_qlibb_readout(self._h, *[C.byref(x) for x in v]), "fc_qlibb_readout")
+        return {"enabled": v[0].value, "links": v[1].value, "fallback": v[2].value,
+                "active": v[3].value}
 
     # tracer particles ---------------------------------------------------------
     def particles_seed(self, count: int, seed: int = 0xF01DCAFE):

@@ -136,6 +136,40 @@ int fc_build_fine_patch_flags(const float* xy, int npts, float aoa_deg, const fc
 int fc_derive_patch_box(fc_dims dims, const uint8_t* flags, int margin_x0, int margin_x1,
                         int margin_y0, int margin_y1, int clearance, fc_patch_box* out);
 
+/* ---- vortex generators and q-links (src/geom/vg.h, airfoil.h:105) -------- */
+/* VGParams, vg.h:24-36. type: 0 single vane, 1 counter-rotating pair,
+ * 2 co-rotating array, 3 ramp. */
+typedef struct fc_vg_params {
+    int type;
+    float x_c, height_c, length_h, beta_deg, pitch_c, gap_h;
+    int count;
+    int common_flow_down;
+    int enabled;
+} fc_vg_params;
+
+/* VaneSlab, voxelizer.h:47-55 (oriented box of one vane, lattice cells). */
+typedef struct fc_vane_slab {
+    float center[3], n_axis[3], d_axis[3], b_axis[3];
+    float half_height, half_length, half_thick;
+} fc_vane_slab;
+
+/* surfaceFrameAt, airfoil.h:105: out6 = point(2), tangent(2), normal(2);
+ * returns 1 when valid. */
+int fc_surface_frame_at(const float* xy, int npts, int le_index, float x_c, int upper, float* out6);
+/* buildFlagsWithVGs, vg.h (clean flags + every VG, then TE closure). slabs
+ * (may be NULL) receive up to slab_cap vane boxes; *nslabs the total. */
+int fc_build_flags_with_vgs(const float* xy, int npts, int le_index, float aoa_deg,
+                            const fc_layout* layout, const fc_vg_params* vgs, int nvg,
+                            const uint8_t* clean_flags, uint8_t* flags_out,
+                            fc_vane_slab* slabs, int slab_cap, int* nslabs);
+/* buildVaneQLinks + densifyQLinks, vg.h: dense cut bytes qfrac[cell*19+q]
+ * and two-node mask ffmask[cell] for every fluid->vane link; returns the
+ * link count. */
+int fc_build_vane_qlinks(fc_dims dims, const uint8_t* active, const uint8_t* clean,
+                         const fc_vane_slab* slabs, int nslabs, uint8_t* qfrac, uint32_t* ffmask);
+/* rayOBBEntry, vg.cpp:272-296 (entry parameter in [0,1] or -1). */
+float fc_ray_obb_entry(const float* origin3, const float* dir3, const fc_vane_slab* slab);
+
 /* ---- solver lifecycle (LBMSolver, lbm_solver.h:133-514) ------------------ */
 typedef struct fc_solver fc_solver;
 
@@ -258,6 +292,16 @@ int fc_set_wall_harvest(fc_solver* s, int enabled);
 int fc_wall_harvest(fc_solver* s, float* out, int cap);
 /* Load (or with NULL, drop) the learned u_tau network used by FC_WALL_MLP. */
 int fc_set_wall_mlp(fc_solver* s, const fc_mlp_params* params);
+
+/* ---- interpolated bounce-back (q-LIBB), lbm_solver.h:311-346 -------------- */
+/* setQLIBBEnabled, :315 (off frees the link field; every link reverts to half-way). */
+int fc_set_qlibb_enabled(fc_solver* s, int enabled);
+/* setCoarseQLinks, :329-331: dense cut bytes qfrac[cell*19 + q] (0 = no link,
+ * b -> q = b/255) and the per-cell two-node mask ffmask[cell] (bit q); links /
+ * fallback are the readout counts (QLIBBReadout, lbm_solver.h:97-101). */
+int fc_set_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback);
+/* qlibbReadout, :346 (+ active_links: links the kernel interpolates). */
+int fc_qlibb_readout(fc_solver* s, int* enabled, int* links, int* fallback, int* active_links);
 
 /* ---- tracer particles (src/render/particles.cuh:29-82, lbm_core.cuh:403-407) */
 /* Particles are 4 floats each: x, y, z in lattice-cell space, age in [0,1]. */

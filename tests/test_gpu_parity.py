@@ -70,8 +70,13 @@ def test_tgv_step_parity_small(gpu, steps):
 
 
 def test_tgv64_config1_parity_and_decay(gpu):
-    """Config 1: TGV 64^3, TRT (3/16), no LES, u0 0.05, nu 0.01, 2000 steps."""
-    say("case: tgv64 f32 2000 steps vs oracle f32")
+    """Config 1: TGV 64^3, TRT (3/16), no LES, u0 0.05, nu 0.01, 2000 steps.
+
+    The GPU stores FP32 deviations f - w, the FP32 oracle stores f itself (as
+    the reference does), so after 2000 steps the two FP32 runs drift apart by
+    their own rounding. Both are measured against the FP64 oracle: the GPU
+    must be at least as close to it as the FP32 restatement (with margin)."""
+    say("case: tgv64 f32 2000 steps vs oracle f32 / f64")
     case = cases.taylor_green(64, 0.05, 0.01)
     s = cases.make_solver(case, "f32")
     ke = []
@@ -80,13 +85,19 @@ def test_tgv64_config1_parity_and_decay(gpu):
         s.synchronize()
         _, u, v, w = s.read_macro()
         ke.append(cases.tgv_kinetic_energy(u, v, w))
-    f = case.f0.astype(np.float32)
     taus, uins = schedule(case, 2000)
-    rho, uo, vo, wo = orc.run(f, case.flags, case.dims, taus, uins, cs=0.0, periodic_xy=True)
+    f32 = case.f0.astype(np.float32)
+    _, uo, vo, _ = orc.run(f32, case.flags, case.dims, taus, uins, cs=0.0, periodic_xy=True)
+    f64 = case.f0.copy()
+    _, ud, vd, _ = orc.run(f64, case.flags, case.dims, taus.astype(np.float64),
+                           uins.astype(np.float64), cs=0.0, periodic_xy=True)
     _, u, v, w = s.read_macro()
-    err = max(rel_l2(u, uo), rel_l2(v, vo))
-    say(f"tgv64 rel L2 vs oracle after 2000 steps: {err:.3e}")
-    assert err < 1e-5
+    e_gpu = rel_l2(np.stack([u, v]), np.stack([ud, vd]))
+    e_o32 = rel_l2(np.stack([uo, vo]), np.stack([ud, vd]))
+    e_pair = rel_l2(np.stack([u, v]), np.stack([uo, vo]))
+    say(f"tgv64 after 2000 steps: gpu-f64 {e_gpu:.3e}, oracle32-f64 {e_o32:.3e}, gpu-oracle32 {e_pair:.3e}")
+    assert e_gpu <= max(1.5 * e_o32, 2e-6)
+    assert e_pair < 2e-4
     assert all(b < a for a, b in zip(ke, ke[1:]))
     s.close()
 
@@ -115,9 +126,29 @@ def tunnel(nx=128, ny=48, nz=16, chord=40, aoa=8.0, cs=0.1):
     return cases.naca_tunnel(nx, ny, nz, chord, aoa=aoa, cs=cs)
 
 
+def vel_err(a, b, u_ref):
+    """L2 error of a velocity field relative to a uniform field of speed u_ref
+    (well posed while the flow is still near rest)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / (u_ref * np.sqrt(a.size)))
+
+
+def oracle_pair(f, case, steps):
+    """FP32 and FP64 oracle runs of ``steps`` steps from the same state."""
+    taus, uins = schedule(case, steps)
+    f32 = np.array(f, dtype=np.float32)
+    m32 = orc.run(f32, case.flags, case.dims, taus, uins, cs=case.cs)
+    f64 = np.array(f, dtype=np.float64)
+    m64 = orc.run(f64, case.flags, case.dims, taus.astype(np.float64), uins.astype(np.float64),
+                  cs=case.cs)
+    return (f32, m32), (f64, m64)
+
+
 @pytest.mark.parametrize("steps", [1, 2, 3, 10, 101])
 def test_tunnel_step_parity(gpu, steps):
-    """Inlet / outlet / slip / bounce-back / LES, ramps on, against the FP32 oracle."""
+    """Inlet / outlet / slip / bounce-back / LES, ramps on: the GPU FP32 path
+    against the FP64 oracle, no further from it than the FP32 oracle is."""
     say(f"case: tunnel 128x48x16 f32 steps={steps}")
     case = tunnel()
     s = cases.make_solver(case, "f32")
@@ -130,13 +161,19 @@ def test_tunnel_step_parity(gpu, steps):
     s.step_n(steps)
     s.synchronize()
     got = s.download_f(f64=True).reshape(Q, -1)
-    taus, uins = schedule(case, steps)
-    rho, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs)
+    (f32, (rho, u, v, w)), (f64, (rd, ud, vd, wd)) = oracle_pair(f, case, steps)
     fluid = case.flags == fc.FLUID
-    assert rel_l2(got[:, fluid], f.reshape(Q, -1)[:, fluid]) < 1e-6
+    ref = f64.reshape(Q, -1)[:, fluid]
+    e_gpu = rel_l2(got[:, fluid], ref)
+    e_o32 = rel_l2(f32.reshape(Q, -1)[:, fluid], ref)
+    say(f"populations rel L2 vs f64: gpu {e_gpu:.3e}, oracle32 {e_o32:.3e}")
+    assert e_gpu <= max(2.0 * e_o32, 1e-7)
     r2, u2, v2, w2 = s.read_macro()
-    for a, b in ((r2, rho), (u2, u), (v2, v)):
-        assert rel_l2(a, b) < 1e-5
+    ul = case.scaling.u_lat
+    assert rel_l2(r2, rd) < 1e-6
+    for a, b, c in ((u2, ud, u), (v2, vd, v), (w2, wd, w)):
+        eg, eo = vel_err(a, b, ul), vel_err(c, b, ul)
+        assert eg <= max(2.0 * eo, 1e-6), (eg, eo)
     F = s.force_now()
     Fo = orc.force(f, case.flags, case.dims)
     assert np.abs(F - Fo).max() < 1e-4 * max(np.abs(Fo).max(), 1e-3)
@@ -145,20 +182,24 @@ def test_tunnel_step_parity(gpu, steps):
 
 def test_config2_tunnel_parity(gpu):
     """Config 2 geometry (384 x 128 x 64, chord 128, AoA 8, Cs 0.1, u_lat 0.08)
-    for 300 steps against the FP32 oracle."""
+    for 300 steps: GPU FP32 against the FP64 oracle, no further from it than
+    the FP32 oracle; force within 1e-4 of the FP64 oracle's."""
     say("case: config2 tunnel 384x128x64 f32 300 steps")
     case = cases.config2_tunnel()
     s = cases.make_solver(case, "f32")
     f = s.download_f(f64=False)
     s.step_n(300)
     s.synchronize()
-    taus, uins = schedule(case, 300)
-    rho, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs)
+    (f32, (rho, u, v, w)), (f64, (rd, ud, vd, wd)) = oracle_pair(f, case, 300)
     r2, u2, v2, w2 = s.read_macro()
-    err = max(rel_l2(u2, u), rel_l2(v2, v), rel_l2(r2, rho))
-    say(f"config2 rel L2 after 300 steps: {err:.3e}")
-    assert err < 1e-5
-    F, Fo = s.force_now(), orc.force(f, case.flags, case.dims)
+    ul = case.scaling.u_lat
+    eg = max(vel_err(u2, ud, ul), vel_err(v2, vd, ul))
+    eo = max(vel_err(u, ud, ul), vel_err(v, vd, ul))
+    say(f"config2 velocity error vs f64 after 300 steps: gpu {eg:.3e}, oracle32 {eo:.3e}")
+    assert eg <= max(2.0 * eo, 1e-6)
+    assert rel_l2(r2, rd) < 1e-6
+    F, Fo = s.force_now(), orc.force(f64, case.flags, case.dims)
+    say(f"config2 force gpu {F} vs f64 oracle {Fo}")
     assert np.abs(F - Fo).max() < 1e-4 * np.abs(Fo).max()
     s.close()
 
@@ -202,17 +243,33 @@ def test_reduced_storage_vs_fp64(gpu, storage, tol):
     ref.close()
 
 
+def stored_mass(f0, storage):
+    """Mass of a canonical state after rounding to the storage format's scaled
+    deviations K (f - w) (K = 2^15 for FP16, 1 otherwise)."""
+    w = np.asarray(fc.W, np.float64)[:, None]
+    if storage == "f64":
+        h = f0.reshape(Q, -1) - w
+    else:  # the upload kernel computes K (f - w) in FP32, then rounds once more
+        d = f0.reshape(Q, -1).astype(np.float32) - w.astype(np.float32)
+        if storage == "f16":
+            h = (d * np.float32(32768.0)).astype(np.float16).astype(np.float64) / 32768.0
+        else:
+            h = d.astype(np.float64)
+    return float(np.sum(w * h.shape[1]) + np.sum(h))
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
 def test_mass_conservation_periodic(gpu, storage):
     say(f"case: tgv32 mass {storage}")
     case = cases.taylor_green(32)
     s = cases.make_solver(case, storage)
     m0 = s.mass()
+    assert abs(m0 - stored_mass(case.f0, storage)) < 1e-9 * 32 ** 3
     s.step_n(500)
     s.synchronize()
     m1 = s.mass()
-    assert abs(m0 - 32 ** 3) < 1e-3
     tol = {"f32": 1e-6, "f16": 1e-4, "f64": 1e-12}[storage]
+    say(f"{storage} mass drift over 500 steps: {abs(m1 - m0) / m0:.3e}")
     assert abs(m1 - m0) / m0 < tol
     s.close()
 
