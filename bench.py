@@ -27,9 +27,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_storage(case, storage, steps, warmup, per_call, kernel_steps):
+def run_storage(case, storage, steps, warmup, per_call, kernel_steps, variant="auto"):
     from foil_cfd_amd import cases
     s = cases.make_solver(case, storage)
+    s.set_step_variant(variant)
     n = case.dims[0] * case.dims[1] * case.dims[2]
     # warmup (not timed)
     done = 0
@@ -60,7 +61,7 @@ def run_storage(case, storage, steps, warmup, per_call, kernel_steps):
     nan = s.nan_detected()
     fr = s.forces()
     s.close()
-    avg_kernel_ms = kms / max(launches, 1)
+    avg_kernel_ms = kms / launches if launches else float("nan")
     res = {
         "glups": n * steps / wall / 1e9,
         "ms_per_step": wall * 1e3 / steps,
@@ -71,7 +72,7 @@ def run_storage(case, storage, steps, warmup, per_call, kernel_steps):
         "cl_ema": fr.cl,
         "cd_ema": fr.cd,
     }
-    log(f"[bench] {storage}: {res['glups']:.2f} GLUPS wall, kernel {avg_kernel_ms:.4f} ms "
+    log(f"[bench] {storage} ({variant}): {res['glups']:.2f} GLUPS wall, kernel {avg_kernel_ms:.4f} ms "
         f"({res['kernel_gbs']:.0f} GB/s algorithmic)")
     return res
 
@@ -105,6 +106,21 @@ def cpu_baseline(case, seconds=12.0):
                       f"({dt:.1f} s wall)", "seconds": dt}
 
 
+def copy_calibration(mib=1024, reps=5):
+    """A plain device copy of known size (elementwise negate, N bytes read and
+    N bytes written per launch) for scaling the HBM traffic counters."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.neg float32", "bytes_read": n * 4,
+                      "bytes_written": n * 4, "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +131,10 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16,f32")
+    ap.add_argument("--variant", default="auto", choices=["auto", "single"],
+                    help="step kernel: auto (two cells per lane where possible) or single")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -122,6 +142,9 @@ def main():
         print("bench.py: this build measures one GPU only (the solver core is single-GPU; "
               "--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        copy_calibration()
+        return
 
     import foil_cfd_amd as fc
     from foil_cfd_amd import cases
@@ -134,7 +157,7 @@ def main():
     results = {}
     for st in storages:
         results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call,
-                                  args.kernel_steps)
+                                  args.kernel_steps, args.variant)
     head = "f16" if "f16" in results else storages[0]
     r = results[head]
     cpu = None if args.no_cpu else cpu_baseline(case)
@@ -160,6 +183,7 @@ def main():
                         "f64": "fp64"}[head],
             "collision": "TRT(3/16) + Smagorinsky(0.1733)",
             "streaming": "in-place AA",
+            "step_variant": args.variant,
             "steps_per_call": args.per_call,
             "not_included": "refinement patch, wall model, particles (coarse grid only)",
             "parallelism": "single_gpu",

@@ -146,6 +146,7 @@ SIGNATURES = {
     "fc_set_startup_ramp": (C.c_int, [_vp, C.c_int]),
     "fc_set_force_ema_window": (C.c_int, [_vp, C.c_float]),
     "fc_set_smagorinsky_cs": (C.c_int, [_vp, C.c_float]),
+    "fc_set_step_variant": (C.c_int, [_vp, C.c_int]),
     "fc_step": (C.c_int, [_vp, C.c_int]),
     "fc_synchronize": (C.c_int, [_vp]),
     "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),

@@ -25,25 +25,34 @@ namespace fc {
 struct StoreF32 {
     using T = float;
     using C = float;
+    using Bits = unsigned;
     static constexpr float K = 1.0f;
     __device__ static inline C load(const T* p) { return *p; }
     __device__ static inline void store(T* p, C v) { *p = v; }
+    __device__ static inline C fromBits(Bits b) { return __builtin_bit_cast(float, b); }
+    __device__ static inline Bits toBits(C v) { return __builtin_bit_cast(Bits, v); }
 };
 
 struct StoreF16 {
     using T = _Float16;
     using C = float;
+    using Bits = unsigned short;
     static constexpr float K = 32768.0f;
     __device__ static inline C load(const T* p) { return static_cast<float>(*p); }
     __device__ static inline void store(T* p, C v) { *p = static_cast<_Float16>(v); }
+    __device__ static inline C fromBits(Bits b) { return static_cast<float>(__builtin_bit_cast(_Float16, b)); }
+    __device__ static inline Bits toBits(C v) { return __builtin_bit_cast(Bits, static_cast<_Float16>(v)); }
 };
 
 struct StoreBF16 {
     using T = __bf16;
     using C = float;
+    using Bits = unsigned short;
     static constexpr float K = 1.0f;
     __device__ static inline C load(const T* p) { return static_cast<float>(*p); }
     __device__ static inline void store(T* p, C v) { *p = static_cast<__bf16>(v); }
+    __device__ static inline C fromBits(Bits b) { return static_cast<float>(__builtin_bit_cast(__bf16, b)); }
+    __device__ static inline Bits toBits(C v) { return __builtin_bit_cast(Bits, static_cast<__bf16>(v)); }
 };
 
 struct StoreF64 {

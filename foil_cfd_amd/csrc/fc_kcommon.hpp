@@ -6,6 +6,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "fc_device.hpp"
 #include "fc_kernels.hpp"
 
@@ -110,6 +113,61 @@ __device__ inline bool rowPos(const KArgs& a, RowPos& p) {
     return true;
 }
 
+// Two-cells-per-lane decomposition: a wave covers 128 cells of a row, lane l
+// the cells x0 = 128 seg + 2 l and x0 + 1 (nx even, x / y not periodic). The
+// kernel arguments carry the pair geometry (segments of 128 cells).
+inline KArgs pairArgs(const KArgs& a) {
+    KArgs b = a;
+    b.nseg = (a.nx + 2 * kWave - 1) / (2 * kWave);
+    b.nwaves = b.nseg * a.ny * a.nz;
+    b.nblocks = (b.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+    b.chunk = (b.nblocks + kXcds - 1) / kXcds;
+    return b;
+}
+
+__device__ inline bool rowPosPair(const KArgs& a, RowPos& p) {
+    const int hb = blockIdx.x;
+    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
+    if (lb >= a.nblocks) return false;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int gw = lb * kWavesPerBlock + wv;
+    if (gw >= a.nwaves) return false;
+    const int row = gw / a.nseg;
+    const int seg = gw - row * a.nseg;
+    const int z = row / a.ny;
+    const int y = row - z * a.ny;
+    const int x = seg * (2 * kWave) + 2 * static_cast<int>(threadIdx.x & (kWave - 1));
+    if (x >= a.nx) return false;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.xm = x - 1;
+    p.xp = x + 1;
+    int ys[3] = {y - 1, y, y + 1};
+    p.yOk[0] = ys[0] >= 0;
+    p.yOk[1] = true;
+    p.yOk[2] = ys[2] < a.ny;
+    if (!p.yOk[0]) ys[0] = 0;
+    if (!p.yOk[2]) ys[2] = a.ny - 1;
+    const int zs[3] = {z == 0 ? a.nz - 1 : z - 1, z, z == a.nz - 1 ? 0 : z + 1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) p.R[i][j] = (ys[i] + a.ny * zs[j]) * a.nx;
+    return true;
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N-1, so
+// direction tables index as constants inside the body.
+template <class F, int... Is>
+__device__ inline void staticForImpl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ inline void staticFor(F&& f) {
+    staticForImpl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Cell index of (x + ex, y + ey, z + ez) for compile-time offsets.
 __device__ inline unsigned nb(const RowPos& p, int ex, int ey, int ez) {
     const int xx = ex < 0 ? p.xm : (ex > 0 ? p.xp : p.x);
@@ -126,6 +184,106 @@ __device__ inline bool inDomain(const KArgs& a, const RowPos& p, int ex, int ey)
     }
     return ok;
 }
+
+// ---------------------------------------------------------------------------
+// Population accessors of the step kernel: slot (q, cell (x+ex, y+ey, z+ez))
+// for compile-time offsets.
+//
+// PopPtr: flat 64-bit addressing, any array size.
+// PopBuf: raw buffer instructions over one wave-uniform descriptor. The
+//   direction slab and the row start are a scalar byte offset (SGPR soffset),
+//   only the x byte offset lives in a vector register, so an access costs one
+//   scalar add and no per-lane address arithmetic. Every offset must stay
+//   below 2^31 bytes, so the host only selects it for arrays under 2 GiB.
+// ---------------------------------------------------------------------------
+template <class S>
+struct PopPtr {
+    using T = typename S::T;
+    using C = typename S::C;
+    T* f;
+    long long n;
+    const RowPos* p;
+    __device__ PopPtr(const KArgs& a, const RowPos& rp) : f(static_cast<T*>(a.f)), n(a.ncells), p(&rp) {}
+    __device__ C ld(int q, int ex, int ey, int ez) const {
+        return S::load(f + static_cast<long long>(q) * n + nb(*p, ex, ey, ez));
+    }
+    __device__ void st(int q, int ex, int ey, int ez, C v) const {
+        S::store(f + static_cast<long long>(q) * n + nb(*p, ex, ey, ez), v);
+    }
+};
+
+template <class S>
+struct PopBuf {
+    using T = typename S::T;
+    using C = typename S::C;
+    using Bits = typename S::Bits;
+    static constexpr unsigned kSz = sizeof(T);
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned slab;       // bytes per direction slab (ncells * sizeof(T))
+    unsigned vx[3];      // per-lane byte offsets of x-1, x, x+1
+    unsigned row[3][3];  // wave-uniform byte offsets of the 9 row starts
+    __device__ PopBuf(const KArgs& a, const RowPos& p) {
+        // 0x00020000: raw (untyped) buffer descriptor word 3 of the gfx9 family
+        rs = __builtin_amdgcn_make_buffer_rsrc(a.f, 0, 0x7FFFFFFF, 0x00020000);
+        slab = static_cast<unsigned>(a.ncells) * kSz;
+        vx[0] = static_cast<unsigned>(p.xm) * kSz;
+        vx[1] = static_cast<unsigned>(p.x) * kSz;
+        vx[2] = static_cast<unsigned>(p.xp) * kSz;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) row[i][j] = static_cast<unsigned>(p.R[i][j]) * kSz;
+    }
+    __device__ C ld(int q, int ex, int ey, int ez) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0)));
+        else
+            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0)));
+    }
+    __device__ void st(int q, int ex, int ey, int ez, C v) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            __builtin_amdgcn_raw_buffer_store_b16(S::toBits(v), rs, vo, so, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b32(S::toBits(v), rs, vo, so, 0);
+    }
+
+    // Two consecutive slots (x+ex, x+ex+1) of one direction and row in one
+    // access: 4 bytes for 2-byte formats, 8 bytes for FP32.
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    using Pair = std::conditional_t<sizeof(Bits) == 2, unsigned, U2>;
+    __device__ static Bits lo(Pair v) {
+        if constexpr (sizeof(Bits) == 2) return static_cast<Bits>(v & 0xFFFFu);
+        else return v.x;
+    }
+    __device__ static Bits hi(Pair v) {
+        if constexpr (sizeof(Bits) == 2) return static_cast<Bits>(v >> 16);
+        else return v.y;
+    }
+    __device__ static Pair pack(Bits a, Bits b) {
+        if constexpr (sizeof(Bits) == 2) return static_cast<unsigned>(a) | (static_cast<unsigned>(b) << 16);
+        else return U2{a, b};
+    }
+    __device__ Pair ld2(int q, int ex, int ey, int ez) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0);
+        else
+            return __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0);
+    }
+    __device__ void st2(int q, int ex, int ey, int ez, Pair v) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            __builtin_amdgcn_raw_buffer_store_b32(v, rs, vo, so, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, so, 0);
+    }
+};
 
 // Post-collision population f*_q of cell (x,y,z) in the given layout, as a
 // deviation H (unscaled by K). Used by the readout kernels.

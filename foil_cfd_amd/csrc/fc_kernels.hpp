@@ -14,12 +14,18 @@ namespace fc {
 // StepParams, src/sim/lbm_core.cuh:233-241, with Cs turned into the
 // precomputed Smagorinsky prefactor 18*sqrt(2)*Cs^2 exactly as the reference
 // launch wrapper forms it, src/sim/lbm_core.cu:785-787).
+// Step kernel variants: kStepAuto picks two cells per lane where the grid
+// allows it (even nx, non-periodic x / y, array under 2 GiB), kStepSingle
+// forces one cell per lane.
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1 };
+
 struct StepArgs {
     float tau0 = 0.6f;
     float magicLambda = 3.0f / 16.0f;
     float smagPrefactor = 0.0f;
     float uInlet = 0.08f;
     int writeMacro = 0;
+    int variant = kStepAuto;
 };
 
 // Device view of one lattice level (non-owning).

@@ -129,15 +129,13 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
 // The fused step. FROM_L: the array is in layout L and this step writes
 // layout S (scatter writes); otherwise it reads layout S and writes L (fully
 // local). WM compiles in the moving-wall (Ladd) slip term on solid links.
+// M is the population accessor (PopBuf for arrays under 2 GiB, else PopPtr).
 // ---------------------------------------------------------------------------
-template <class S, bool FROM_L, bool WM, bool PX, bool PY>
-__global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
+template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
+__device__ inline void stepCell(const KArgs& a, const RowPos& p) {
     using T = typename S::T;
     using C = typename S::C;
-    RowPos p;
-    if (!rowPos<PX, PY>(a, p)) return;
-    T* const f = static_cast<T*>(a.f);
-    const long long n = a.ncells;
+    const M<S> pop(a, p);
     const unsigned c = nb(p, 0, 0, 0);
     const std::uint8_t k = a.kind[c];
     const std::uint8_t flag = k & kKindFlagMask;
@@ -149,12 +147,13 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
     // Store f*_q of this cell into the output layout (scatter writes are
     // bounds checked only for boundary cells; fluid cells never leave the
     // domain because every non-periodic face plane is boundary-flagged).
-    auto storeOut = [&](int q, C val, bool check) {
+    auto storeOut = [&](auto qc, C val, bool check) {
+        constexpr int q = decltype(qc)::value;
         if (FROM_L) {
             if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
-            S::store(f + static_cast<long long>(q) * n + nb(p, kCx[q], kCy[q], kCz[q]), val);
+            pop.st(q, kCx[q], kCy[q], kCz[q], val);
         } else {
-            S::store(f + static_cast<long long>(kOpp[q]) * n + c, val);
+            pop.st(kOpp[q], 0, 0, 0, val);
         }
     };
 
@@ -162,9 +161,10 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
         if (flag == kInlet) {  // equilibrium Dirichlet, reference lbm_core.cu:242-253
             const C uIn = C(a.uIn);
             const C usq = C(1.5) * uIn * uIn;
-#pragma unroll
-            for (int q = 0; q < kQ; ++q)
-                storeOut(q, heqOf<C>(q, K, C(0), C(1), uIn, C(0), C(0), usq), true);
+            staticFor<kQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                storeOut(qc, heqOf<C>(q, K, C(0), C(1), uIn, C(0), C(0), usq), true);
+            });
             if (a.writeMacro) {
                 a.rho[c] = 1.0f;
                 a.u[c] = a.uIn;
@@ -174,15 +174,15 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
         } else if (flag == kOutlet) {  // zero-gradient copy, lbm_core.cu:254-272
             const T* src = static_cast<const T*>(a.feed) + (FROM_L ? 0 : 1) * kQ * plane;
             C r = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
+            staticFor<kQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
                 const C h = S::load(src + q * plane + yz);
-                storeOut(q, h, true);
+                storeOut(qc, h, true);
                 r += h;
                 jx += C(kCx[q]) * h;
                 jy += C(kCy[q]) * h;
                 jz += C(kCz[q]) * h;
-            }
+            });
             if (a.writeMacro) {
                 const C rr = C(1) + r / K;
                 const C inv = C(1) / rr;
@@ -202,13 +202,13 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
 
     // ---- fluid: pull the 19 incoming populations ---------------------------
     C H[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) {
+    staticFor<kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         if (FROM_L)
-            H[q] = S::load(f + static_cast<long long>(kOpp[q]) * n + nb(p, -kCx[q], -kCy[q], -kCz[q]));
+            H[q] = pop.ld(kOpp[q], -kCx[q], -kCy[q], -kCz[q]);
         else
-            H[q] = S::load(f + static_cast<long long>(q) * n + c);
-    }
+            H[q] = pop.ld(q, 0, 0, 0);
+    });
     if (k & kKindSpecial) {
         // Per-link source select by the upstream neighbour's flag, as the
         // reference (lbm_core.cu:306-331), mapped onto the AA slots.
@@ -218,40 +218,39 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
             uwy = C(halfBits(a.uwy[c]));
             uwz = C(halfBits(a.uwz[c]));
         }
-#pragma unroll
-        for (int q = 1; q < kQ; ++q) {
-            const int cx = kCx[q], cy = kCy[q], cz = kCz[q];
+        staticFor<kQ - 1>([&](auto qm) {
+            constexpr int q = decltype(qm)::value + 1;
+            constexpr int cx = kCx[q], cy = kCy[q], cz = kCz[q];
             const std::uint8_t nf = a.kind[nb(p, -cx, -cy, -cz)] & kKindFlagMask;
             if (nf == kSolid) {
                 // half-way bounce-back: own f*_opp(q)
                 if (FROM_L)
-                    H[q] = S::load(f + static_cast<long long>(q) * n + c);
+                    H[q] = pop.ld(q, 0, 0, 0);
                 else
-                    H[q] = S::load(f + static_cast<long long>(kOpp[q]) * n + nb(p, -cx, -cy, -cz));
+                    H[q] = pop.ld(kOpp[q], -cx, -cy, -cz);
                 if (WM) {
                     const C cuw = C(cx) * uwx + C(cy) * uwy + C(cz) * uwz;
                     H[q] += K * C(6) * wOf<C>(q) * cuw;
                 }
             } else if (nf == kSlipTop || nf == kSlipBottom) {
-                const int m = kMirY[q];
+                constexpr int m = kMirY[q];
                 if (FROM_L)
-                    H[q] = S::load(f + static_cast<long long>(kOpp[m]) * n + nb(p, -cx, 0, -cz));
+                    H[q] = pop.ld(kOpp[m], -cx, 0, -cz);
                 else
-                    H[q] = S::load(f + static_cast<long long>(m) * n + nb(p, 0, -cy, 0));
+                    H[q] = pop.ld(m, 0, -cy, 0);
             } else if (nf == kSlipFront || nf == kSlipBack) {
-                const int m = kMirZ[q];
+                constexpr int m = kMirZ[q];
                 if (FROM_L)
-                    H[q] = S::load(f + static_cast<long long>(kOpp[m]) * n + nb(p, -cx, -cy, 0));
+                    H[q] = pop.ld(kOpp[m], -cx, -cy, 0);
                 else
-                    H[q] = S::load(f + static_cast<long long>(m) * n + nb(p, 0, 0, -cz));
+                    H[q] = pop.ld(m, 0, 0, -cz);
             }
-        }
+        });
     }
 
     const Macro<C> mac = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
 
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) storeOut(q, H[q], false);
+    staticFor<kQ>([&](auto qc) { storeOut(qc, H[decltype(qc)::value], false); });
     if (k & kKindFeed) {
         T* dstFeed = static_cast<T*>(a.feed) + newLayout * kQ * plane;
 #pragma unroll
@@ -262,6 +261,79 @@ __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
         a.u[c] = static_cast<float>(mac.ux);
         a.v[c] = static_cast<float>(mac.uy);
         a.w[c] = static_cast<float>(mac.uz);
+    }
+}
+
+template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
+__global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
+    RowPos p;
+    if (!rowPos<PX, PY>(a, p)) return;
+    stepCell<S, M, FROM_L, WM, PX, PY>(a, p);
+}
+
+// ---------------------------------------------------------------------------
+// The same step with two x-adjacent cells per lane (a wave covers 128 cells of
+// a row; nx even, x and y not periodic). A wave whose 128 cells are all plain
+// bulk fluid (kind byte 0) moves both cells' populations of a direction with
+// one 4-byte (FP16 / BF16) or 8-byte (FP32) buffer access: twice the bytes
+// per memory instruction, so twice the bytes in flight per wave. The slots of
+// x-shifted directions start at odd element offsets; the accesses are
+// unaligned and rely on the unaligned memory mode ROCm programs on gfx9
+// parts. Any other wave (boundary, special links, outlet feed, wall model)
+// runs the per-cell path for its two cells in turn.
+// ---------------------------------------------------------------------------
+template <class S, bool FROM_L, bool WM>
+__global__ void __launch_bounds__(kBlock) stepPairKernel(KArgs a) {
+    using C = typename S::C;
+    using Bits = typename S::Bits;
+    RowPos p;
+    if (!rowPosPair(a, p)) return;
+    const unsigned c0 = nb(p, 0, 0, 0);
+    const unsigned short kk = *reinterpret_cast<const unsigned short*>(a.kind + c0);
+    const bool fast = __builtin_amdgcn_read_exec() == ~0ull && __builtin_amdgcn_ballot_w64(kk != 0) == 0ull;
+    if (!fast) {
+        stepCell<S, PopBuf, FROM_L, WM, false, false>(a, p);
+        RowPos p1 = p;
+        p1.x = p.x + 1;
+        p1.xm = p.x;
+        p1.xp = p.x + 2;
+        stepCell<S, PopBuf, FROM_L, WM, false, false>(a, p1);
+        return;
+    }
+    const PopBuf<S> pop(a, p);
+    using P2 = typename PopBuf<S>::Pair;
+    P2 P[kQ];
+    staticFor<kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (FROM_L)
+            P[q] = pop.ld2(kOpp[q], -kCx[q], -kCy[q], -kCz[q]);
+        else
+            P[q] = pop.ld2(q, 0, 0, 0);
+    });
+    const C K = C(S::K);
+    C H[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(PopBuf<S>::lo(P[q]));
+    const Macro<C> m0 = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
+    Bits O[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) O[q] = S::toBits(H[q]);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(PopBuf<S>::hi(P[q]));
+    const Macro<C> m1 = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
+    staticFor<kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const P2 v = PopBuf<S>::pack(O[q], S::toBits(H[q]));
+        if (FROM_L)
+            pop.st2(q, kCx[q], kCy[q], kCz[q], v);
+        else
+            pop.st2(kOpp[q], 0, 0, 0, v);
+    });
+    if (a.writeMacro) {
+        *reinterpret_cast<float2*>(a.rho + c0) = make_float2(static_cast<float>(m0.rho), static_cast<float>(m1.rho));
+        *reinterpret_cast<float2*>(a.u + c0) = make_float2(static_cast<float>(m0.ux), static_cast<float>(m1.ux));
+        *reinterpret_cast<float2*>(a.v + c0) = make_float2(static_cast<float>(m0.uy), static_cast<float>(m1.uy));
+        *reinterpret_cast<float2*>(a.w + c0) = make_float2(static_cast<float>(m0.uz), static_cast<float>(m1.uz));
     }
 }
 
@@ -501,8 +573,8 @@ unsigned rowGrid(const KArgs& a) { return static_cast<unsigned>(a.chunk * kXcds)
 
 int lastError() { return static_cast<int>(hipGetLastError()); }
 
-template <class S>
-int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, hipStream_t st) {
+template <class S, template <class> class M>
+int dispatchStepM(const KArgs& a, const LatticeView& lat, int layout, hipStream_t st) {
     const bool wm = lat.uwx && lat.uwy && lat.uwz;
     const bool fromL = layout == kLayoutL;
     const bool pxy = lat.periodicX || lat.periodicY;
@@ -510,21 +582,51 @@ int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, hipStream_t
     // Periodic x / y are only used together (fully periodic box).
     if (pxy) {
         if (fromL)
-            stepKernel<S, true, false, true, true><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, true, false, true, true><<<g, b, 0, st>>>(a);
         else
-            stepKernel<S, false, false, true, true><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, false, false, true, true><<<g, b, 0, st>>>(a);
     } else if (wm) {
         if (fromL)
-            stepKernel<S, true, true, false, false><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, true, true, false, false><<<g, b, 0, st>>>(a);
         else
-            stepKernel<S, false, true, false, false><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, false, true, false, false><<<g, b, 0, st>>>(a);
     } else {
         if (fromL)
-            stepKernel<S, true, false, false, false><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, true, false, false, false><<<g, b, 0, st>>>(a);
         else
-            stepKernel<S, false, false, false, false><<<g, b, 0, st>>>(a);
+            stepKernel<S, M, false, false, false, false><<<g, b, 0, st>>>(a);
     }
     return lastError();
+}
+
+// Buffer addressing whenever every population byte offset fits in 31 bits
+// (FP16 / BF16 up to 56 M cells, FP32 up to 28 M); FP64 and larger arrays
+// take the flat-pointer accessor. The two-cells-per-lane kernel needs an even
+// nx and non-periodic x / y.
+template <class S>
+int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, int variant, hipStream_t st) {
+    const unsigned long long bytes =
+        static_cast<unsigned long long>(kQ) * static_cast<unsigned long long>(a.ncells) * sizeof(typename S::T);
+    if constexpr (!std::is_same_v<S, StoreF64>) {
+        if (bytes < (1ull << 31)) {
+            const bool pairOk = (a.nx % 2 == 0) && !lat.periodicX && !lat.periodicY;
+            if (variant != kStepSingle && pairOk) {
+                const KArgs a2 = pairArgs(a);
+                const bool wm = lat.uwx && lat.uwy && lat.uwz;
+                const dim3 g(rowGrid(a2)), b(kBlock);
+                if (layout == kLayoutL) {
+                    if (wm) stepPairKernel<S, true, true><<<g, b, 0, st>>>(a2);
+                    else stepPairKernel<S, true, false><<<g, b, 0, st>>>(a2);
+                } else {
+                    if (wm) stepPairKernel<S, false, true><<<g, b, 0, st>>>(a2);
+                    else stepPairKernel<S, false, false><<<g, b, 0, st>>>(a2);
+                }
+                return lastError();
+            }
+            return dispatchStepM<S, PopBuf>(a, lat, layout, st);
+        }
+    }
+    return dispatchStepM<S, PopPtr>(a, lat, layout, st);
 }
 
 template <template <class> class F, class... Args>
@@ -547,7 +649,7 @@ struct StepOp {
         a.pre = sa.smagPrefactor;
         a.uIn = sa.uInlet;
         a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
-        return dispatchStep<S>(a, lat, layout, st);
+        return dispatchStep<S>(a, lat, layout, sa.variant, st);
     }
 };
 

@@ -314,6 +314,11 @@ public:
     void setRampEnabled(bool on) { rampEnabled_ = on; }
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
+    int setStepVariant(int v) {
+        if (v != kStepAuto && v != kStepSingle) return fail(FC_ERR_INVALID, "unknown step variant");
+        stepVariant_ = v;
+        return FC_OK;
+    }
 
     int stepN(int n) {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
@@ -330,6 +335,7 @@ public:
             a.smagPrefactor = smagPrefactor(cs_);
             a.uInlet = effectiveU();
             a.writeMacro = (i == n - 1) ? 1 : 0;
+            a.variant = stepVariant_;
             std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
             if (kernelTiming_) {
                 ev = nextKernelEvents();
@@ -1246,6 +1252,7 @@ private:
     bool rampActive_ = true, rampEnabled_ = true;
     long long rampStartStep_ = 0;
     float cs_ = kSmagorinskyCs;
+    int stepVariant_ = kStepAuto;
     long long gateOpenAtSteps_ = 0;
     float emaWindowFlowThroughs_ = 1.0f;
     float emaFx_ = 0, emaFy_ = 0;
@@ -1469,6 +1476,10 @@ int fc_set_smagorinsky_cs(fc_solver* s, float cs) {
     if (!s || cs < 0.0f) return FC_ERR_INVALID;
     s->impl.setCs(cs);
     return FC_OK;
+}
+
+int fc_set_step_variant(fc_solver* s, int variant) {
+    return s ? s->impl.setStepVariant(variant) : FC_ERR_INVALID;
 }
 
 int fc_step(fc_solver* s, int n) { return s ? s->impl.stepN(n) : FC_ERR_INVALID; }

@@ -292,6 +292,11 @@ class LBMSolver:
     def set_smagorinsky_cs(self, cs: float):
         check(self._lib.fc_set_smagorinsky_cs(self._h, cs), "fc_set_smagorinsky_cs")
 
+    def set_step_variant(self, variant: str):
+        """"auto" (two cells per lane where possible) or "single"."""
+        v = {"auto": 0, "single": 1}[variant]
+        check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
+
     def set_force_ema_window(self, ft: float):
         check(self._lib.fc_set_force_ema_window(self._h, ft), "fc_set_force_ema_window")
 

@@ -189,6 +189,14 @@ int fc_set_force_ema_window(fc_solver* s, float flow_throughs);  /* setForceEmaW
  * 0 disables the eddy-viscosity term. Default 0.1733. */
 int fc_set_smagorinsky_cs(fc_solver* s, float cs);
 
+/* Step kernel selection (no reference counterpart: the reference has one
+ * kernel). FC_STEP_AUTO (default) runs two x-adjacent cells per lane with
+ * packed population accesses where the grid allows it (even nx, x / y not
+ * periodic, population array under 2 GiB); FC_STEP_SINGLE forces one cell per
+ * lane. Both give the same cell update. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1 };
+int fc_set_step_variant(fc_solver* s, int variant);
+
 /* ---- stepping ------------------------------------------------------------ */
 /* LBMSolver::stepN (lbm_solver.h:354): n fused steps with the startup ramps,
  * watchdog every 200 steps, one force sample + wall-model update per call.
