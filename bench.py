@@ -1,12 +1,13 @@
 """FoilCFD MI355X benchmark: GLUPS and HBM roofline fraction of the fused
 D3Q19 TRT + Smagorinsky stream-collide on the 23.6 M-cell NACA 2412 wing
-(512 x 192 x 240, chord 256, AoA 8 deg), FP16 shifted storage vs FP32.
+(512 x 192 x 240, chord 256, AoA 8 deg), 16-bit shifted storage (FP16C,
+s1e4m11) vs FP32.
 
 One "step" is one lattice time step of the whole grid (the reference's
 LBMSolver::stepN unit). Steps run in calls of 100 (one macro store and one
 momentum-exchange sample per call, like the reference's selftest frame).
 
-Prints ONE JSON line. value = FP16-storage GLUPS (whole job, one GPU).
+Prints ONE JSON line. value = FP16C-storage GLUPS (whole job, one GPU).
 """
 import argparse
 import json
@@ -18,7 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {"f16": 76.0, "bf16": 76.0, "f32": 152.0, "f64": 304.0}
+BYTES_PER_CELL = {"f16": 76.0, "f16c": 76.0, "bf16": 76.0, "f32": 152.0, "f64": 304.0}
 METRIC = "GLUPS & % HBM peak, 23.6M-cell wing, FP16-store vs FP32 at ≤1% Cl/Cd err"
 REFERENCE_PUBLISHED_GLUPS_FP32 = 8.151  # RTX 5090, 23.6M cells, FP32 (notes/PERF.md:10-20)
 
@@ -130,7 +131,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=200)
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--storages", default="f16,f32")
+    ap.add_argument("--storages", default="f16c,f32")
     ap.add_argument("--variant", default="auto", choices=["auto", "single"],
                     help="step kernel: auto (two cells per lane where possible) or single")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -158,7 +159,7 @@ def main():
     for st in storages:
         results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call,
                                   args.kernel_steps, args.variant)
-    head = "f16" if "f16" in results else storages[0]
+    head = next((h for h in ("f16c", "f16") if h in results), storages[0])
     r = results[head]
     cpu = None if args.no_cpu else cpu_baseline(case)
 
@@ -179,8 +180,8 @@ def main():
         "config": {
             "workload": f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_{head}store",
             "cells": ncells,
-            "storage": {"f16": "fp16_shifted", "bf16": "bf16_shifted", "f32": "fp32",
-                        "f64": "fp64"}[head],
+            "storage": {"f16c": "fp16c_s1e4m11_shifted", "f16": "fp16_shifted",
+                        "bf16": "bf16_shifted", "f32": "fp32", "f64": "fp64"}[head],
             "collision": "TRT(3/16) + Smagorinsky(0.1733)",
             "streaming": "in-place AA",
             "step_variant": args.variant,
@@ -204,8 +205,8 @@ def main():
                             for kk, vv in v.items()} for k, v in results.items()},
         "reference_published_glups_fp32": REFERENCE_PUBLISHED_GLUPS_FP32,
     }
-    if "f16" in results and "f32" in results:
-        out["fp16_vs_fp32_speedup"] = round(results["f16"]["glups"] / results["f32"]["glups"], 3)
+    if head in ("f16c", "f16") and "f32" in results:
+        out["fp16_vs_fp32_speedup"] = round(results[head]["glups"] / results["f32"]["glups"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -22,10 +22,12 @@ FC_STORAGE_F32 = 0
 FC_STORAGE_F16 = 1
 FC_STORAGE_BF16 = 2
 FC_STORAGE_F64 = 3
+FC_STORAGE_F16C = 4
 
 STORAGE_NAMES = {"f32": FC_STORAGE_F32, "f16": FC_STORAGE_F16, "bf16": FC_STORAGE_BF16,
-                 "f64": FC_STORAGE_F64}
-STORAGE_BYTES = {FC_STORAGE_F32: 4, FC_STORAGE_F16: 2, FC_STORAGE_BF16: 2, FC_STORAGE_F64: 8}
+                 "f64": FC_STORAGE_F64, "f16c": FC_STORAGE_F16C}
+STORAGE_BYTES = {FC_STORAGE_F32: 4, FC_STORAGE_F16: 2, FC_STORAGE_BF16: 2, FC_STORAGE_F64: 8,
+                 FC_STORAGE_F16C: 2}
 
 
 class FcDims(C.Structure):
@@ -147,6 +149,7 @@ SIGNATURES = {
     "fc_set_force_ema_window": (C.c_int, [_vp, C.c_float]),
     "fc_set_smagorinsky_cs": (C.c_int, [_vp, C.c_float]),
     "fc_set_step_variant": (C.c_int, [_vp, C.c_int]),
+    "fc_last_error": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "fc_step": (C.c_int, [_vp, C.c_int]),
     "fc_synchronize": (C.c_int, [_vp]),
     "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),

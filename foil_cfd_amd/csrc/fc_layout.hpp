@@ -57,6 +57,7 @@ enum Storage : int {
     kStoreF16 = 1,
     kStoreBF16 = 2,
     kStoreF64 = 3,
+    kStoreF16C = 4,
 };
 
 inline constexpr int storageBytes(int s) {

@@ -635,6 +635,7 @@ int byStorage(int storage, Args&&... args) {
         case kStoreF32: return F<StoreF32>::run(args...);
         case kStoreF16: return F<StoreF16>::run(args...);
         case kStoreBF16: return F<StoreBF16>::run(args...);
+        case kStoreF16C: return F<StoreF16C>::run(args...);
         case kStoreF64: return F<StoreF64>::run(args...);
         default: return static_cast<int>(hipErrorInvalidValue);
     }

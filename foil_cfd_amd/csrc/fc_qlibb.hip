@@ -108,6 +108,7 @@ int launchQLibbPrepass(const LatticeView& lat, int layout, const int* cells, con
         case kStoreF32: return runQ<StoreF32>(lat, layout, cells, packed, count, scratch, s);
         case kStoreF16: return runQ<StoreF16>(lat, layout, cells, packed, count, scratch, s);
         case kStoreBF16: return runQ<StoreBF16>(lat, layout, cells, packed, count, scratch, s);
+        case kStoreF16C: return runQ<StoreF16C>(lat, layout, cells, packed, count, scratch, s);
         case kStoreF64: return runQ<StoreF64>(lat, layout, cells, packed, count, scratch, s);
         default: return static_cast<int>(hipErrorInvalidValue);
     }

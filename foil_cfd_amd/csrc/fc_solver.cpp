@@ -147,7 +147,7 @@ public:
         if (dims_.cells() >= (1LL << 31))
             return fail(FC_ERR_INVALID, "grid exceeds 2^31 cells");
         storage_ = opt ? opt->storage : kStoreF32;
-        if (storage_ < kStoreF32 || storage_ > kStoreF64)
+        if (storage_ < kStoreF32 || storage_ > kStoreF16C)
             return fail(FC_ERR_INVALID, "unknown storage format");
         periodicX_ = opt ? (opt->periodic_x != 0) : false;
         periodicY_ = opt ? (opt->periodic_y != 0) : false;
@@ -1475,6 +1475,12 @@ int fc_set_force_ema_window(fc_solver* s, float flow_throughs) {
 int fc_set_smagorinsky_cs(fc_solver* s, float cs) {
     if (!s || cs < 0.0f) return FC_ERR_INVALID;
     s->impl.setCs(cs);
+    return FC_OK;
+}
+
+int fc_last_error(fc_solver* s, char* buf, int cap) {
+    if (!s) return FC_ERR_INVALID;
+    fc::copyErr(s->impl.error, buf, cap);
     return FC_OK;
 }
 

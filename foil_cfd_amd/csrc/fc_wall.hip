@@ -191,6 +191,7 @@ int launchWallUpdate(const LatticeView& lat, int layout, const WallListView& wl,
         case kStoreF32: return WallOp<StoreF32>::run(lat, layout, wl, uwx, uwy, uwz, p, stats, harvest, s);
         case kStoreF16: return WallOp<StoreF16>::run(lat, layout, wl, uwx, uwy, uwz, p, stats, harvest, s);
         case kStoreBF16: return WallOp<StoreBF16>::run(lat, layout, wl, uwx, uwy, uwz, p, stats, harvest, s);
+        case kStoreF16C: return WallOp<StoreF16C>::run(lat, layout, wl, uwx, uwy, uwz, p, stats, harvest, s);
         case kStoreF64: return WallOp<StoreF64>::run(lat, layout, wl, uwx, uwy, uwz, p, stats, harvest, s);
         default: return static_cast<int>(hipErrorInvalidValue);
     }

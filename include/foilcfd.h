@@ -35,6 +35,7 @@ extern "C" {
 #define FC_STORAGE_F16 1   /* IEEE FP16 of 2^15 (f - w); FP32 arithmetic */
 #define FC_STORAGE_BF16 2  /* BF16 of (f - w); FP32 arithmetic */
 #define FC_STORAGE_F64 3   /* FP64 storage and arithmetic (reference-quality runs) */
+#define FC_STORAGE_F16C 4  /* 16-bit s1e4m11 float of (f - w), normals 2^-14..2; FP32 arithmetic */
 
 /* ---- plain data ---------------------------------------------------------- */
 typedef struct fc_dims { int nx, ny, nz; } fc_dims;  /* GridDims, lbm_core.cuh:139-158 */
@@ -174,6 +175,9 @@ float fc_ray_obb_entry(const float* origin3, const float* dir3, const fc_vane_sl
 typedef struct fc_solver fc_solver;
 
 fc_solver* fc_create(void);                 /* LBMSolver::LBMSolver */
+/* Message of the last failing call on this solver (empty if none); no
+ * reference counterpart (the reference logs to stderr). */
+int fc_last_error(fc_solver* s, char* buf, int cap);
 void fc_destroy(fc_solver* s);              /* LBMSolver::~LBMSolver */
 /* LBMSolver::init (lbm_solver.h:152-154): allocate, upload flags, reset.
  * flags: dims cells bytes (CellFlag values). options may be NULL (FP32,

@@ -709,6 +709,83 @@ void orc_run_f64(double* fA, double* fB, const uint8_t* flags, int nx, int ny, i
     if (src != fA) std::memcpy(fA, src, sizeof(double) * (size_t)kQ * (size_t)g.ncells());
 }
 
+// Storage-format emulation (test infrastructure, no reference counterpart):
+// the FP64 step sequence of orc_run_f64 with every population rounded after
+// each step (and once on entry) as a device storage format keeps it: the
+// deviation d = f - w_q scaled by `scale`, rounded to nearest-even in a
+// binary format with `mant` stored mantissa bits and normals from 2^emin
+// (subnormal quantum below), then unscaled. Formats: FP32 (23, -126, 1),
+// IEEE FP16 with K = 2^15 (10, -14, 32768), BF16 (7, -126, 1).
+static double roundFormat(double x, int mant, int emin) {
+    if (x == 0.0 || !std::isfinite(x)) return x;
+    int e = 0;
+    std::frexp(x, &e);
+    int ex = e - 1;
+    if (ex < emin) ex = emin;
+    const double q = std::ldexp(1.0, ex - mant);
+    return std::nearbyint(x / q) * q;
+}
+
+// FP16C (s1e4m11) exactly as the device encodes it: the FP32 deviation is
+// rescaled by 2^-112 and its bits rounded to nearest even at bit 12 (15-bit
+// magnitude, saturating), then decoded back (restatement of the device's
+// StoreF16C, which has no reference counterpart).
+static uint16_t f16cEncode(float v) {
+    const float y = v * 0x1p-112f;
+    uint32_t bits;
+    std::memcpy(&bits, &y, 4);
+    const uint32_t mag = bits & 0x7FFFFFFFu;
+    uint32_t r = (mag + 0x7FFu + ((mag >> 12) & 1u)) >> 12;
+    if (r > 0x7FFFu) r = 0x7FFFu;
+    return static_cast<uint16_t>(r | ((bits >> 16) & 0x8000u));
+}
+static float f16cDecode(uint16_t b) {
+    const uint32_t x = b;
+    const uint32_t u = ((x & 0x7FFFu) << 12) | ((x & 0x8000u) << 16);
+    float y;
+    std::memcpy(&y, &u, 4);
+    return y * 0x1p112f;
+}
+
+// mant < 0 selects FP16C.
+static void roundStorage(double* f, long long n, int mant, int emin, double scale) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
+        for (int q = 0; q < kQ; ++q) {
+            const double w = weight<double>(q);
+            double& v = f[(long long)q * n + i];
+            if (mant < 0)
+                v = w + (double)f16cDecode(f16cEncode((float)(v - w)));
+            else
+                v = w + roundFormat((v - w) * scale, mant, emin) / scale;
+        }
+    }
+}
+
+void orc_run_stored_f64(double* fA, double* fB, const uint8_t* flags, int nx, int ny, int nz,
+                        int periodicX, int periodicY, int nsteps, const double* tauPerStep,
+                        const double* uInPerStep, double magicLambda, double smagPrefactor,
+                        double* rho, double* u, double* v, double* w, int mant, int emin,
+                        double scale) {
+    const Grid g = makeGrid(nx, ny, nz, periodicX, periodicY);
+    const long long n = g.ncells();
+    double* src = fA;
+    double* dst = fB;
+    roundStorage(src, n, mant, emin, scale);
+    std::memcpy(dst, src, sizeof(double) * (size_t)kQ * (size_t)n);
+    for (int i = 0; i < nsteps; ++i) {
+        StepArgs a{tauPerStep[i], magicLambda, smagPrefactor, uInPerStep[i], i == nsteps - 1};
+        streamCollide<double, double>(src, dst, flags, g, a, rho, u, v, w, nullptr, nullptr,
+                                      nullptr, nullptr, nullptr);
+        roundStorage(dst, n, mant, emin, scale);
+        std::swap(src, dst);
+    }
+    if (src != fA) std::memcpy(fA, src, sizeof(double) * (size_t)kQ * (size_t)n);
+}
+
+uint16_t orc_f16c_encode(float v) { return f16cEncode(v); }
+float orc_f16c_decode(uint16_t b) { return f16cDecode(b); }
+
 void orc_force_f32(const float* f, const uint8_t* flags, int nx, int ny, int nz,
                    int periodicX, int periodicY, const uint16_t* uwx, const uint16_t* uwy,
                    const uint16_t* uwz, double* out3) {

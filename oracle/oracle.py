@@ -70,6 +70,13 @@ def lib():
         L.orc_run_f64.argtypes = [_dp, _dp] + common + [C.c_int, _dp, _dp, C.c_double,
                                                         C.c_double] + \
             [_dp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
+        L.orc_run_stored_f64.argtypes = [_dp, _dp] + common + [C.c_int, _dp, _dp, C.c_double,
+                                                               C.c_double] + \
+            [_dp] * 4 + [C.c_int, C.c_int, C.c_double]
+        L.orc_f16c_encode.restype = C.c_uint16
+        L.orc_f16c_encode.argtypes = [C.c_float]
+        L.orc_f16c_decode.restype = C.c_float
+        L.orc_f16c_decode.argtypes = [C.c_uint16]
         L.orc_force_f32.argtypes = [_fp] + common + [_u16p] * 3 + [_dp]
         L.orc_force_f64.argtypes = [_dp] + common + [_u16p] * 3 + [_dp]
         L.orc_watchdog_f32.restype = C.c_int
@@ -183,6 +190,34 @@ def run(f, flags, dims, taus, uins, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False
     return tuple(m)
 
 
+STORAGE_FORMATS = {  # (stored mantissa bits, min normal exponent, deviation scale)
+    "f32": (23, -126, 1.0),
+    "f16": (10, -14, 32768.0),
+    "bf16": (7, -126, 1.0),
+    "f16c": (-1, 0, 1.0),  # the device's s1e4m11 encode/decode, bit for bit
+}
+
+
+def run_stored(f, flags, dims, taus, uins, fmt, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False):
+    """FP64 steps with every population rounded after each step as storage
+    format ``fmt`` (a STORAGE_FORMATS key or a (mant, emin, scale) tuple)
+    keeps it. ``f`` (FP64 canonical SoA) is advanced in place."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    mant, emin, scale = STORAGE_FORMATS[fmt] if isinstance(fmt, str) else fmt
+    assert f.dtype == np.float64
+    tmp = np.empty_like(f)
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    taus = np.ascontiguousarray(taus, dtype=np.float64)
+    uins = np.ascontiguousarray(uins, dtype=np.float64)
+    m = [np.zeros(n, dtype=np.float64) for _ in range(4)]
+    lib().orc_run_stored_f64(_p(f, _dp), _p(tmp, _dp), _p(fl, _u8p), nx, ny, nz, int(periodic_xy),
+                             int(periodic_xy), len(taus), _p(taus, _dp), _p(uins, _dp), lam,
+                             float(smag_prefactor(cs)), *[_p(a, _dp) for a in m], int(mant),
+                             int(emin), float(scale))
+    return tuple(m)
+
+
 def force(f, flags, dims, periodic_xy=False, slip=None):
     nx, ny, nz = dims
     out = np.zeros(3, dtype=np.float64)
@@ -254,6 +289,15 @@ def solve_utau(ut, ys, nu, seed=0.0):
 
 def float_to_half_bits(x):
     return lib().orc_float_to_half(x)
+
+
+def f16c_encode(x):
+    """The device's FP16C (s1e4m11) code of an FP32 deviation."""
+    return int(lib().orc_f16c_encode(float(x)))
+
+
+def f16c_decode(b):
+    return float(lib().orc_f16c_decode(int(b)))
 
 
 def wall_update(f, dims, wl, utau, nu, cutoff, ema=0.25):

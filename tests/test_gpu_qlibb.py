@@ -48,8 +48,11 @@ def synthetic_qlinks(flags, dims, seed=1234):
     return qfrac.reshape(-1), ffm, links
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 1e-6), ("f64", 1e-12)])
-def test_qlibb_parity(gpu, storage, tol):
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_qlibb_parity(gpu, storage):
+    """FP64 storage: within 1e-12 of the FP64 oracle. FP32 storage (which
+    keeps f - w): against the FP64 oracle, no further from it than the FP32
+    oracle (which keeps f, as the reference does) and within 1e-6."""
     print(f"case: q-LIBB tunnel {storage}", flush=True)
     case = cases.naca_tunnel(128, 48, 8, 40, aoa=8.0, cs=0.1, ramp=False)
     qfrac, ffm, links = synthetic_qlinks(case.flags, case.dims)
@@ -63,15 +66,27 @@ def test_qlibb_parity(gpu, storage, tol):
     s.step_n(steps)
     s.synchronize()
     got = s.download_f(f64=True).reshape(Q, -1)
-    dt = np.float64 if storage == "f64" else np.float32
-    taus = np.full(steps, case.scaling.tau, dt)
-    uins = np.full(steps, case.scaling.u_lat, dt)
-    orc.run(f, case.flags, case.dims, taus, uins, cs=case.cs, qfrac=qfrac, ffmask=ffm)
     fluid = case.flags == fc.FLUID
-    a, b = got[:, fluid], f.reshape(Q, -1)[:, fluid]
+
+    def oracle(dt):
+        g = np.array(f, dtype=dt)
+        taus = np.full(steps, case.scaling.tau, dt)
+        uins = np.full(steps, case.scaling.u_lat, dt)
+        orc.run(g, case.flags, case.dims, taus, uins, cs=case.cs, qfrac=qfrac, ffmask=ffm)
+        return g.reshape(Q, -1)[:, fluid].astype(np.float64)
+
+    a = got[:, fluid]
+    b = oracle(np.float64)
     err = np.linalg.norm(a - b) / np.linalg.norm(b)
-    print(f"q-LIBB {storage} rel L2 after {steps} steps: {err:.3e}", flush=True)
-    assert err < tol
+    print(f"q-LIBB {storage} rel L2 vs FP64 oracle after {steps} steps: {err:.3e}", flush=True)
+    if storage == "f64":
+        tol = 1e-12
+        assert err < tol
+    else:
+        tol = 1e-6
+        e32 = np.linalg.norm(oracle(np.float32) - b) / np.linalg.norm(b)
+        print(f"FP32 oracle rel L2 vs FP64 oracle: {e32:.3e}", flush=True)
+        assert err < tol and err <= max(2.0 * e32, 1e-7)
     # and the q-links actually changed the flow vs plain bounce-back
     s2 = cases.make_solver(case, storage)
     s2.step_n(steps)
