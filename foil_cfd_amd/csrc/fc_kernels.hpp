@@ -14,10 +14,11 @@ namespace fc {
 // StepParams, src/sim/lbm_core.cuh:233-241, with Cs turned into the
 // precomputed Smagorinsky prefactor 18*sqrt(2)*Cs^2 exactly as the reference
 // launch wrapper forms it, src/sim/lbm_core.cu:785-787).
-// Step kernel variants: kStepAuto picks two cells per lane where the grid
-// allows it (even nx, non-periodic x / y, array under 2 GiB), kStepSingle
-// forces one cell per lane.
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1 };
+// Step kernel variants: kStepAuto is the library's default (one cell per
+// lane, the fastest measured on MI355X), kStepSingle forces one cell per
+// lane, kStepPair runs two x-adjacent cells per lane with packed accesses
+// where the grid allows it (even nx, non-periodic x / y, array under 2 GiB).
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2 };
 
 struct StepArgs {
     float tau0 = 0.6f;

@@ -293,8 +293,9 @@ class LBMSolver:
         check(self._lib.fc_set_smagorinsky_cs(self._h, cs), "fc_set_smagorinsky_cs")
 
     def set_step_variant(self, variant: str):
-        """"auto" (two cells per lane where possible) or "single"."""
-        v = {"auto": 0, "single": 1}[variant]
+        """"auto" (library default), "single" (one cell per lane) or "pair"
+        (two x-adjacent cells per lane where the grid allows it)."""
+        v = {"auto": 0, "single": 1, "pair": 2}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):

@@ -194,11 +194,12 @@ int fc_set_force_ema_window(fc_solver* s, float flow_throughs);  /* setForceEmaW
 int fc_set_smagorinsky_cs(fc_solver* s, float cs);
 
 /* Step kernel selection (no reference counterpart: the reference has one
- * kernel). FC_STEP_AUTO (default) runs two x-adjacent cells per lane with
- * packed population accesses where the grid allows it (even nx, x / y not
- * periodic, population array under 2 GiB); FC_STEP_SINGLE forces one cell per
- * lane. Both give the same cell update. */
-enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1 };
+ * kernel). FC_STEP_AUTO (default) is the library's choice, currently one
+ * cell per lane (the fastest measured on MI355X); FC_STEP_SINGLE forces it;
+ * FC_STEP_PAIR runs two x-adjacent cells per lane with packed population
+ * accesses where the grid allows it (even nx, x / y not periodic, population
+ * array under 2 GiB). All give the same cell update. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2 };
 int fc_set_step_variant(fc_solver* s, int variant);
 
 /* ---- stepping ------------------------------------------------------------ */

@@ -589,6 +589,260 @@ P3 inletSpawn(int ny, int nz, uint32_t& r) {
 
 Grid makeGrid(int nx, int ny, int nz, int px, int py) { return Grid{nx, ny, nz, px, py}; }
 
+// ---------------------------------------------------------------------------
+// Two-level grid refinement (lbm_refine.cu / lbm_refine.cuh, stepN sequencing
+// lbm_solver.cpp:1172-1290). The fine level covers the coarse patch box
+// [x0,x1) x [y0,y1) x [0,nz) at factor m (fine dims m*W, m*H, m*nz), x / y not
+// periodic, z periodic; its x / y faces carry a 2-cell Interface shell.
+// ---------------------------------------------------------------------------
+constexpr uint8_t kIface = 8;
+constexpr int kRestrictBand = 2;   // lbm_refine.cuh:77
+constexpr int kRestrictBlend = 3;  // lbm_refine.cuh:85
+
+struct PBox {
+    int x0, y0, x1, y1;
+};
+
+// Latt-Chopard projection of fneq onto the second-order Hermite space,
+// lbm_refine.cu:64-95. Pinned by: idempotence, zero mass / momentum of the
+// result, and exact reproduction of a pure second-order fneq (tests).
+template <class T>
+void regularizeFneq(T fneq[kQ]) {
+    T pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    for (int q = 0; q < kQ; ++q) {
+        const T f = fneq[q];
+        const T cx = T(kCx[q]), cy = T(kCy[q]), cz = T(kCz[q]);
+        pxx += cx * cx * f; pyy += cy * cy * f; pzz += cz * cz * f;
+        pxy += cx * cy * f; pxz += cx * cz * f; pyz += cy * cz * f;
+    }
+    const T trace = T(1.0 / 3.0) * (pxx + pyy + pzz);
+    for (int q = 0; q < kQ; ++q) {
+        const T cx = T(kCx[q]), cy = T(kCy[q]), cz = T(kCz[q]);
+        const T qPi = cx * cx * pxx + cy * cy * pyy + cz * cz * pzz +
+                      T(2) * (cx * cy * pxy + cx * cz * pxz + cy * cz * pyz) - trace;
+        fneq[q] = T(4.5) * weight<T>(q) * qPi;
+    }
+}
+
+// Moments with the collision's density floor and speed cap
+// (lbm_refine.cu:211-226 / :300-311).
+template <class T>
+void cappedMoments(const T fi[kQ], T& rho, T& ux, T& uy, T& uz) {
+    T r = 0, jx = 0, jy = 0, jz = 0;
+    for (int q = 0; q < kQ; ++q) {
+        r += fi[q];
+        jx += T(kCx[q]) * fi[q];
+        jy += T(kCy[q]) * fi[q];
+        jz += T(kCz[q]) * fi[q];
+    }
+    r = std::max(r, T(0.05f));
+    const T inv = T(1) / r;
+    ux = jx * inv; uy = jy * inv; uz = jz * inv;
+    const T u2 = ux * ux + uy * uy + uz * uz;
+    const T cap = T(0.30f);
+    if (u2 > cap * cap) {
+        const T sc = cap / std::sqrt(u2);
+        ux *= sc; uy *= sc; uz *= sc;
+    }
+    rho = r;
+}
+
+// coarseToFineFillKernel, lbm_refine.cu:123-258: fine Interface cells (or,
+// fullVolume, every non-solid fine cell) get the coarse post-collision
+// populations trilinearly interpolated at their centre (solid corners dropped
+// and weights renormalised), blended between coarse t (cT0) and t+1 (cT1),
+// re-split into feq + regularised fneq, fneq rescaled by neqScale.
+template <class T>
+void coarseToFineFill(const T* cT0, const T* cT1, const uint8_t* cFlags, const Grid& cg,
+                      const uint8_t* fFlags, T* ff, const Grid& fg, const PBox& box, int m,
+                      T timeWeight, T neqScale, bool fullVolume) {
+    const long long fn = fg.ncells(), cn = cg.ncells();
+    const T invM = T(1) / T(m);
+#pragma omp parallel for schedule(static)
+    for (long long cell = 0; cell < fn; ++cell) {
+        const uint8_t flag = fFlags[cell];
+        if (fullVolume ? flag == kSolid : flag != kIface) continue;
+        const int xf = (int)(cell % fg.nx);
+        const int yf = (int)((cell / fg.nx) % fg.ny);
+        const int zf = (int)(cell / ((long long)fg.nx * fg.ny));
+        const T gx = T(box.x0) + (T(xf) + T(0.5)) * invM - T(0.5);
+        const T gy = T(box.y0) + (T(yf) + T(0.5)) * invM - T(0.5);
+        const T gz = (T(zf) + T(0.5)) * invM - T(0.5);
+        const int ix = (int)std::floor(gx), iy = (int)std::floor(gy), iz = (int)std::floor(gz);
+        const T tx = gx - T(ix), ty = gy - T(iy), tz = gz - T(iz);
+        T wgt[8] = {(1 - tx) * (1 - ty) * (1 - tz), tx * (1 - ty) * (1 - tz),
+                    (1 - tx) * ty * (1 - tz),       tx * ty * (1 - tz),
+                    (1 - tx) * (1 - ty) * tz,       tx * (1 - ty) * tz,
+                    (1 - tx) * ty * tz,             tx * ty * tz};
+        long long corner[8];
+        for (int c = 0; c < 8; ++c) {
+            const int cx = std::min(std::max(ix + (c & 1), 0), cg.nx - 1);
+            const int cy = std::min(std::max(iy + ((c >> 1) & 1), 0), cg.ny - 1);
+            int cz = iz + ((c >> 2) & 1);
+            if (cz < 0) cz += cg.nz;
+            if (cz >= cg.nz) cz -= cg.nz;
+            corner[c] = cg.idx(cx, cy, cz);
+        }
+        T wsum = 0;
+        for (int c = 0; c < 8; ++c) {
+            if (cFlags[corner[c]] == kSolid) wgt[c] = 0;
+            wsum += wgt[c];
+        }
+        if (wsum < T(1e-6f)) continue;
+        const T invW = T(1) / wsum;
+        for (int c = 0; c < 8; ++c) wgt[c] *= invW;
+        T fi[kQ];
+        for (int q = 0; q < kQ; ++q) {
+            T s0 = 0, s1 = 0;
+            for (int c = 0; c < 8; ++c) {
+                s0 += wgt[c] * cT0[(long long)q * cn + corner[c]];
+                s1 += wgt[c] * cT1[(long long)q * cn + corner[c]];
+            }
+            fi[q] = s0 + timeWeight * (s1 - s0);
+        }
+        T rho, ux, uy, uz;
+        cappedMoments(fi, rho, ux, uy, uz);
+        T feq[kQ], fneq[kQ];
+        equilibrium19<T>(rho, ux, uy, uz, feq);
+        for (int q = 0; q < kQ; ++q) fneq[q] = fi[q] - feq[q];
+        regularizeFneq(fneq);
+        for (int q = 0; q < kQ; ++q) ff[(long long)q * fn + cell] = feq[q] + neqScale * fneq[q];
+    }
+}
+
+// childAvgFneq, lbm_refine.cu:274-317: mean of the m^3 fine children
+// (solid children skipped), capped moments, fneq = mean - feq.
+template <class T>
+bool childAvgFneq(const T* ff, const uint8_t* fFlags, const Grid& fg, int fx, int fy, int fz, int m,
+                  T fneqOut[kQ], T& rho, T& ux, T& uy, T& uz) {
+    const long long fn = fg.ncells();
+    T fi[kQ];
+    for (int q = 0; q < kQ; ++q) fi[q] = 0;
+    int nf = 0;
+    for (int k = 0; k < m; ++k)
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) {
+                const long long c = fg.idx(fx + i, fy + j, fz + k);
+                if (fFlags[c] == kSolid) continue;
+                ++nf;
+                for (int q = 0; q < kQ; ++q) fi[q] += ff[(long long)q * fn + c];
+            }
+    if (nf == 0) return false;
+    const T invN = T(1) / T(nf);
+    for (int q = 0; q < kQ; ++q) fi[q] *= invN;
+    cappedMoments(fi, rho, ux, uy, uz);
+    T feq[kQ];
+    equilibrium19<T>(rho, ux, uy, uz, feq);
+    for (int q = 0; q < kQ; ++q) fneqOut[q] = fi[q] - feq[q];
+    return true;
+}
+
+// fineToCoarseRestrictKernel, lbm_refine.cu:319-429: coarse Fluid cells of the
+// box shrunk by kRestrictBand get the child-averaged state with the 19-point
+// fneq filter (neighbours inside the restricted x / y range with fluid
+// children; z periodic), regularised, rescaled by neqScaleInv, blended into
+// the existing coarse post-collision value over a kRestrictBlend-cell ramp.
+template <class T, class M>
+void fineToCoarseRestrict(const T* ff, const uint8_t* fFlags, const Grid& fg, T* fc,
+                          const uint8_t* cFlags, const Grid& cg, const PBox& box, int m,
+                          T neqScaleInv, M* mRho, M* mU, M* mV, M* mW) {
+    const int rx0 = box.x0 + kRestrictBand, ry0 = box.y0 + kRestrictBand;
+    const int rx1 = box.x1 - kRestrictBand, ry1 = box.y1 - kRestrictBand;
+    const long long cn = cg.ncells();
+    const bool macro = mRho && mU && mV && mW;
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int zc = 0; zc < cg.nz; ++zc)
+        for (int yc = ry0; yc < ry1; ++yc)
+            for (int xc = rx0; xc < rx1; ++xc) {
+                const long long cc = cg.idx(xc, yc, zc);
+                if (cFlags[cc] != kFluid) continue;
+                const int dEdge = std::min(std::min(xc - rx0, rx1 - 1 - xc),
+                                           std::min(yc - ry0, ry1 - 1 - yc));
+                const T blend = std::min(T(1), T((float)(dEdge + 1) / (float)(kRestrictBlend + 1)));
+                T fneqC[kQ], rho, ux, uy, uz;
+                if (!childAvgFneq(ff, fFlags, fg, m * (xc - box.x0), m * (yc - box.y0), m * zc, m,
+                                  fneqC, rho, ux, uy, uz))
+                    continue;
+                T fneqF[kQ];
+                for (int q = 0; q < kQ; ++q) fneqF[q] = fneqC[q];
+                int nS = 1;
+                for (int nq = 1; nq < kQ; ++nq) {
+                    const int nxc = xc + kCx[nq], nyc = yc + kCy[nq];
+                    if (nxc < rx0 || nxc >= rx1 || nyc < ry0 || nyc >= ry1) continue;
+                    const int nzw = (zc + kCz[nq] + cg.nz) % cg.nz;
+                    T fneqN[kQ], rn, un, vn, wn;
+                    if (!childAvgFneq(ff, fFlags, fg, m * (nxc - box.x0), m * (nyc - box.y0), m * nzw,
+                                      m, fneqN, rn, un, vn, wn))
+                        continue;
+                    for (int q = 0; q < kQ; ++q) fneqF[q] += fneqN[q];
+                    ++nS;
+                }
+                const T invS = T(1) / T(nS);
+                for (int q = 0; q < kQ; ++q) fneqF[q] *= invS;
+                regularizeFneq(fneqF);
+                T feq[kQ];
+                equilibrium19<T>(rho, ux, uy, uz, feq);
+                T br = 0, bjx = 0, bjy = 0, bjz = 0;
+                for (int q = 0; q < kQ; ++q) {
+                    T& slot = fc[(long long)q * cn + cc];
+                    const T restricted = feq[q] + neqScaleInv * fneqF[q];
+                    const T blended = slot + blend * (restricted - slot);
+                    slot = blended;
+                    br += blended;
+                    bjx += T(kCx[q]) * blended;
+                    bjy += T(kCy[q]) * blended;
+                    bjz += T(kCz[q]) * blended;
+                }
+                if (macro) {
+                    br = std::max(br, T(0.05f));
+                    const T inv = T(1) / br;
+                    mRho[cc] = M(br); mU[cc] = M(bjx * inv); mV[cc] = M(bjy * inv); mW[cc] = M(bjz * inv);
+                }
+            }
+}
+
+// fineTauFor, lbm_refine.cuh:96-98 (FP32 as the reference computes it).
+float fineTauFor(float tauC, int m) { return (float)m * (tauC - 0.5f) + 0.5f; }
+
+// The refined step sequence of stepN (lbm_solver.cpp:1144-1290, coarse +
+// fine levels): coarse step t -> t+1; for each of the m fine sub-steps the
+// Interface fill from coarse (t, t+1) at weight k/m, then the fine step;
+// then the restriction of the fine state into the coarse t+1 buffer. Taus and
+// rescale factors are formed in FP32 as the reference does.
+template <class T>
+void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* fB,
+                const uint8_t* fFlags, const Grid& fg, const PBox& box, int m, int nsteps,
+                const T* taus, const T* uins, T lam, T pre, T* rho, T* u, T* v, T* w) {
+    T* cs = cA;
+    T* cd = cB;
+    T* fs = fA;
+    T* fd = fB;
+    for (int i = 0; i < nsteps; ++i) {
+        const float tauC = (float)taus[i];
+        const float tauF = fineTauFor(tauC, m);
+        const bool last = i == nsteps - 1;
+        StepArgs a{taus[i], lam, pre, uins[i], last};
+        streamCollide<T, T>(cs, cd, cFlags, cg, a, rho, u, v, w, nullptr, nullptr, nullptr,
+                            nullptr, nullptr);
+        StepArgs af{(T)tauF, lam, pre, uins[i], 0};
+        const float neqScale = tauF / ((float)m * tauC);
+        for (int k = 0; k < m; ++k) {
+            coarseToFineFill<T>(cs, cd, cFlags, cg, fFlags, fs, fg, box, m,
+                                (T)((float)k / (float)m), (T)neqScale, false);
+            streamCollide<T, T>(fs, fd, fFlags, fg, af, (T*)nullptr, (T*)nullptr, (T*)nullptr,
+                                (T*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            std::swap(fs, fd);
+        }
+        const float neqScaleInv = ((float)m * tauC) / tauF;
+        fineToCoarseRestrict<T, T>(fs, fFlags, fg, cd, cFlags, cg, box, m, (T)neqScaleInv,
+                                   last ? rho : nullptr, last ? u : nullptr, last ? v : nullptr,
+                                   last ? w : nullptr);
+        std::swap(cs, cd);
+    }
+    if (cs != cA) std::memcpy(cA, cs, sizeof(T) * (size_t)kQ * (size_t)cg.ncells());
+    if (fs != fA) std::memcpy(fA, fs, sizeof(T) * (size_t)kQ * (size_t)fg.ncells());
+}
+
 }  // namespace
 
 extern "C" {
@@ -781,6 +1035,55 @@ void orc_run_stored_f64(double* fA, double* fB, const uint8_t* flags, int nx, in
         std::swap(src, dst);
     }
     if (src != fA) std::memcpy(fA, src, sizeof(double) * (size_t)kQ * (size_t)n);
+}
+
+// ---- refinement wrappers ----
+void orc_regularize_fneq_f64(double* fneq) { regularizeFneq<double>(fneq); }
+
+void orc_fine_fill_f64(const double* cT0, const double* cT1, const uint8_t* cFlags, int cnx,
+                       int cny, int cnz, const uint8_t* fFlags, double* ff, int fnx, int fny,
+                       int fnz, int x0, int y0, int x1, int y1, int m, double timeWeight,
+                       double neqScale, int fullVolume) {
+    coarseToFineFill<double>(cT0, cT1, cFlags, makeGrid(cnx, cny, cnz, 0, 0), fFlags, ff,
+                             makeGrid(fnx, fny, fnz, 0, 0), PBox{x0, y0, x1, y1}, m, timeWeight,
+                             neqScale, fullVolume != 0);
+}
+void orc_fine_fill_f32(const float* cT0, const float* cT1, const uint8_t* cFlags, int cnx,
+                       int cny, int cnz, const uint8_t* fFlags, float* ff, int fnx, int fny,
+                       int fnz, int x0, int y0, int x1, int y1, int m, float timeWeight,
+                       float neqScale, int fullVolume) {
+    coarseToFineFill<float>(cT0, cT1, cFlags, makeGrid(cnx, cny, cnz, 0, 0), fFlags, ff,
+                            makeGrid(fnx, fny, fnz, 0, 0), PBox{x0, y0, x1, y1}, m, timeWeight,
+                            neqScale, fullVolume != 0);
+}
+void orc_fine_restrict_f64(const double* ff, const uint8_t* fFlags, int fnx, int fny, int fnz,
+                           double* fc, const uint8_t* cFlags, int cnx, int cny, int cnz, int x0,
+                           int y0, int x1, int y1, int m, double neqScaleInv, double* rho,
+                           double* u, double* v, double* w) {
+    fineToCoarseRestrict<double, double>(ff, fFlags, makeGrid(fnx, fny, fnz, 0, 0), fc, cFlags,
+                                         makeGrid(cnx, cny, cnz, 0, 0), PBox{x0, y0, x1, y1}, m,
+                                         neqScaleInv, rho, u, v, w);
+}
+float orc_fine_tau_for(float tauC, int m) { return fineTauFor(tauC, m); }
+
+void orc_run_refined_f64(double* cA, double* cB, const uint8_t* cFlags, int cnx, int cny,
+                         int cnz, double* fA, double* fB, const uint8_t* fFlags, int x0, int y0,
+                         int x1, int y1, int m, int nsteps, const double* taus,
+                         const double* uins, double lam, double pre, double* rho, double* u,
+                         double* v, double* w) {
+    const Grid cg = makeGrid(cnx, cny, cnz, 0, 0);
+    const Grid fg = makeGrid(m * (x1 - x0), m * (y1 - y0), m * cnz, 0, 0);
+    runRefined<double>(cA, cB, cFlags, cg, fA, fB, fFlags, fg, PBox{x0, y0, x1, y1}, m, nsteps,
+                       taus, uins, lam, pre, rho, u, v, w);
+}
+void orc_run_refined_f32(float* cA, float* cB, const uint8_t* cFlags, int cnx, int cny, int cnz,
+                         float* fA, float* fB, const uint8_t* fFlags, int x0, int y0, int x1,
+                         int y1, int m, int nsteps, const float* taus, const float* uins,
+                         float lam, float pre, float* rho, float* u, float* v, float* w) {
+    const Grid cg = makeGrid(cnx, cny, cnz, 0, 0);
+    const Grid fg = makeGrid(m * (x1 - x0), m * (y1 - y0), m * cnz, 0, 0);
+    runRefined<float>(cA, cB, cFlags, cg, fA, fB, fFlags, fg, PBox{x0, y0, x1, y1}, m, nsteps,
+                      taus, uins, lam, pre, rho, u, v, w);
 }
 
 uint16_t orc_f16c_encode(float v) { return f16cEncode(v); }

@@ -77,6 +77,25 @@ def lib():
         L.orc_f16c_encode.argtypes = [C.c_float]
         L.orc_f16c_decode.restype = C.c_float
         L.orc_f16c_decode.argtypes = [C.c_uint16]
+        L.orc_regularize_fneq_f64.argtypes = [_dp]
+        i6 = [C.c_int] * 6
+        L.orc_fine_fill_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _u8p, _dp,
+                                        C.c_int, C.c_int, C.c_int] + [C.c_int] * 5 + \
+            [C.c_double, C.c_double, C.c_int]
+        L.orc_fine_fill_f32.argtypes = [_fp, _fp, _u8p, C.c_int, C.c_int, C.c_int, _u8p, _fp,
+                                        C.c_int, C.c_int, C.c_int] + [C.c_int] * 5 + \
+            [C.c_float, C.c_float, C.c_int]
+        L.orc_fine_restrict_f64.argtypes = [_dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _u8p,
+                                            C.c_int, C.c_int, C.c_int] + [C.c_int] * 5 + \
+            [C.c_double] + [_dp] * 4
+        L.orc_fine_tau_for.restype = C.c_float
+        L.orc_fine_tau_for.argtypes = [C.c_float, C.c_int]
+        L.orc_run_refined_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                          _u8p] + [C.c_int] * 6 + [_dp, _dp, C.c_double,
+                                                                   C.c_double] + [_dp] * 4
+        L.orc_run_refined_f32.argtypes = [_fp, _fp, _u8p, C.c_int, C.c_int, C.c_int, _fp, _fp,
+                                          _u8p] + [C.c_int] * 6 + [_fp, _fp, C.c_float,
+                                                                   C.c_float] + [_fp] * 4
         L.orc_force_f32.argtypes = [_fp] + common + [_u16p] * 3 + [_dp]
         L.orc_force_f64.argtypes = [_dp] + common + [_u16p] * 3 + [_dp]
         L.orc_watchdog_f32.restype = C.c_int
@@ -289,6 +308,80 @@ def solve_utau(ut, ys, nu, seed=0.0):
 
 def float_to_half_bits(x):
     return lib().orc_float_to_half(x)
+
+
+def regularize_fneq(fneq):
+    a = np.ascontiguousarray(fneq, dtype=np.float64).copy()
+    lib().orc_regularize_fneq_f64(_p(a, _dp))
+    return a
+
+
+def fine_tau_for(tau_c, m):
+    return float(lib().orc_fine_tau_for(tau_c, m))
+
+
+def fine_dims(box, cdims, m):
+    x0, y0, x1, y1 = box
+    return (m * (x1 - x0), m * (y1 - y0), m * cdims[2])
+
+
+def fine_fill(c_t0, c_t1, cflags, cdims, fflags, ff, box, m, time_weight, neq_scale,
+              full_volume=False):
+    """coarseToFineFill (lbm_refine.cu:123-258) into the fine canonical array ``ff``."""
+    f64 = ff.dtype == np.float64
+    dt, pt = (np.float64, _dp) if f64 else (np.float32, _fp)
+    fn = lib().orc_fine_fill_f64 if f64 else lib().orc_fine_fill_f32
+    fd = fine_dims(box, cdims, m)
+    a0 = np.ascontiguousarray(c_t0, dtype=dt)
+    a1 = np.ascontiguousarray(c_t1, dtype=dt)
+    fn(_p(a0, pt), _p(a1, pt), _p(np.ascontiguousarray(cflags, np.uint8), _u8p), *cdims,
+       _p(np.ascontiguousarray(fflags, np.uint8), _u8p), _p(ff, pt), *fd, *box, m, time_weight,
+       neq_scale, int(full_volume))
+
+
+def fine_restrict(ff, fflags, cf, cflags, cdims, box, m, neq_scale_inv, macro=False):
+    """fineToCoarseRestrict (lbm_refine.cu:319-429) into the coarse FP64 array ``cf``."""
+    n = cdims[0] * cdims[1] * cdims[2]
+    m4 = [np.zeros(n) for _ in range(4)] if macro else [None] * 4
+    fd = fine_dims(box, cdims, m)
+    lib().orc_fine_restrict_f64(_p(np.ascontiguousarray(ff, np.float64), _dp),
+                                _p(np.ascontiguousarray(fflags, np.uint8), _u8p), *fd, _p(cf, _dp),
+                                _p(np.ascontiguousarray(cflags, np.uint8), _u8p), *cdims, *box, m,
+                                neq_scale_inv, *[_p(a, _dp) for a in m4])
+    return tuple(m4) if macro else None
+
+
+def seed_fine(cf, cflags, cdims, fflags, box, m, tau_c, f64=True):
+    """The fine level's cold-start state (seedFineFromCoarse,
+    lbm_solver.cpp:502-521): rest equilibrium everywhere, then the
+    full-volume fill from the current coarse state at weight 0."""
+    fd = fine_dims(box, cdims, m)
+    ff = init_equilibrium(np.zeros(fd[0] * fd[1] * fd[2], np.uint8), fd, 0.0, f64=f64)
+    tau_f = fine_tau_for(tau_c, m)
+    neq = np.float32(tau_f) / (np.float32(m) * np.float32(tau_c))
+    fine_fill(cf, cf, cflags, cdims, fflags, ff, box, m, 0.0, float(neq), full_volume=True)
+    return ff
+
+
+def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.0, cs=0.1733):
+    """len(taus) refined steps (coarse + m fine sub-steps + restriction,
+    lbm_solver.cpp:1144-1290); cf / ff are advanced in place. Returns the
+    coarse (rho, u, v, w) of the last step."""
+    nx, ny, nz = cdims
+    n = nx * ny * nz
+    f64 = cf.dtype == np.float64
+    dt, pt = (np.float64, _dp) if f64 else (np.float32, _fp)
+    assert ff.dtype == cf.dtype
+    ctmp, ftmp = np.empty_like(cf), np.empty_like(ff)
+    taus = np.ascontiguousarray(taus, dtype=dt)
+    uins = np.ascontiguousarray(uins, dtype=dt)
+    m4 = [np.zeros(n, dtype=dt) for _ in range(4)]
+    fn = lib().orc_run_refined_f64 if f64 else lib().orc_run_refined_f32
+    fn(_p(cf, pt), _p(ctmp, pt), _p(np.ascontiguousarray(cflags, np.uint8), _u8p), nx, ny, nz,
+       _p(ff, pt), _p(ftmp, pt), _p(np.ascontiguousarray(fflags, np.uint8), _u8p), *box, m,
+       len(taus), _p(taus, pt), _p(uins, pt), lam, float(smag_prefactor(cs)),
+       *[_p(a, pt) for a in m4])
+    return tuple(m4)
 
 
 def f16c_encode(x):
