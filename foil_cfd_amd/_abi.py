@@ -52,6 +52,12 @@ class FcPatchBox(C.Structure):
     _fields_ = [("x0", C.c_int), ("y0", C.c_int), ("x1", C.c_int), ("y1", C.c_int)]
 
 
+class FcRefinementInfo(C.Structure):
+    _fields_ = [("active", C.c_int), ("factor", C.c_int), ("box", FcPatchBox),
+                ("fine_dims", FcDims), ("fine_scaling", FcScaling), ("device_bytes", C.c_double),
+                ("forces_from_fine", C.c_int)]
+
+
 class FcForceReadout(C.Structure):
     _fields_ = [("cl", C.c_float), ("cd", C.c_float), ("lift_to_drag", C.c_float),
                 ("valid", C.c_int), ("flow_throughs", C.c_float),
@@ -150,6 +156,13 @@ SIGNATURES = {
     "fc_set_smagorinsky_cs": (C.c_int, [_vp, C.c_float]),
     "fc_set_step_variant": (C.c_int, [_vp, C.c_int]),
     "fc_last_error": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "fc_init_refinement": (C.c_int, [_vp, C.POINTER(FcPatchBox), C.c_int, _u8p, C.c_char_p,
+                                     C.c_int]),
+    "fc_shutdown_refinement": (C.c_int, [_vp]),
+    "fc_set_refined_flags": (C.c_int, [_vp, _u8p]),
+    "fc_get_refinement_info": (C.c_int, [_vp, C.POINTER(FcRefinementInfo)]),
+    "fc_download_fine_f": (C.c_int, [_vp, _vp, C.c_int]),
+    "fc_upload_fine_f": (C.c_int, [_vp, _vp, C.c_int]),
     "fc_step": (C.c_int, [_vp, C.c_int]),
     "fc_synchronize": (C.c_int, [_vp]),
     "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),

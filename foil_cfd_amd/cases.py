@@ -94,6 +94,23 @@ def config3_wing(aoa: float = 8.0) -> Case:
     return naca_tunnel(512, 192, 240, 256, aoa=aoa, u_lat=0.08, cs=0.1733)
 
 
+def refinement_patch(case: Case, factor: int = 2, upstream_c: float = 0.35, wake_c: float = 1.0,
+                     below_c: float = 0.45, above_c: float = 0.45, digits: str = "2412",
+                     aoa: float = 8.0):
+    """Patch box and fine flags for a tunnel case the way the reference app
+    derives them (derivePatchBox with the chord margins of ui.h:136-140 and
+    clearance 3, main.cpp:590-598; buildFinePatchFlags for the fine level)."""
+    nx, ny, nz = case.dims
+    chord = case.scaling.chord_cells
+    cells = lambda c: max(2, int(round(c * chord)))  # noqa: E731
+    box = S.derive_patch_box(nx, ny, nz, case.flags, cells(upstream_c), cells(wake_c),
+                             cells(below_c), cells(above_c))
+    pts, _ = S.generate_naca4(digits, 200)
+    layout = S.make_layout(nx, ny, nz, chord)
+    fine = S.build_fine_patch_flags(pts, aoa, layout, box, factor)
+    return (box.x0, box.y0, box.x1, box.y1), fine
+
+
 def make_solver(case: Case, storage: str = "f32") -> S.LBMSolver:
     s = S.LBMSolver()
     s.set_startup_ramp_enabled(case.ramp)

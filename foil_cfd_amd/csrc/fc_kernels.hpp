@@ -144,6 +144,32 @@ int launchParticleAdvect(float* pos4, int count, const float* u, const float* v,
 
 // ---- q-LIBB pre-pass ----------------------------------------------------------
 // Links: cell index + packed (q | byte << 8 | ff << 16); scratch: count doubles.
+// ---- two-level refinement (fc_refine.hip) ---------------------------------
+// Patch box in coarse cells: fine grid covers [x0,x1) x [y0,y1) x [0,nz) at
+// `factor` (fine dims factor*W, factor*H, factor*nz).
+struct PatchBoxI {
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+};
+struct RefineGeom {
+    PatchBoxI box;
+    int factor = 2;
+    Dims coarse, fine;
+};
+// Coarse post-collision deviations of the ring cells (list bandXY, pairs of
+// x, y) at the current time into band[(q * nz + z) * nband + i], FP32.
+int launchBandSnapshot(const LatticeView& coarse, int layout, const RefineGeom& g, const int* bandXY, int nband,
+                       float* band, void* stream);
+// Coarse-to-fine fill (shell from band + live coarse, or full volume from the
+// live coarse array alone), written in the fine level's layout fLayout.
+int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, const int* bandIndex, int nband,
+                   const LatticeView& fine, int fLayout, const RefineGeom& g, float timeWeight, float neqScale,
+                   int fullVolume, void* stream);
+// Fine-to-coarse restriction into the coarse array in layout cLayout;
+// optional coarse macro fields (all four or none).
+int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
+                       const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
+                       void* stream);
+
 int launchQLibbPrepass(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* packed,
                        int count, double* scratch, void* stream);
 

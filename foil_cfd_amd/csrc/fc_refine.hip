@@ -1,0 +1,524 @@
+// Two-level grid refinement on the in-place (AA) population store, gfx950:
+// the coarse time-t band snapshot, the coarse-to-fine Interface fill and the
+// LDS-tiled fine-to-coarse restriction.
+//
+// Same coupling as the reference (src/sim/lbm_refine.cu:123-429, sequenced by
+// lbm_solver.cpp:1172-1290):
+//  * fill: fine Interface cells (or every non-solid fine cell when seeding)
+//    get the coarse post-collision populations trilinearly interpolated at
+//    their centre (solid corners dropped, weights renormalised), blended
+//    between coarse t and t+1 by k/m, split into feq + fneq by the capped
+//    moments, fneq projected on the stress space (Latt-Chopard) and rescaled
+//    by tau_f / (m tau_c);
+//  * restriction: coarse Fluid cells of the box shrunk by 2 get the mean of
+//    their m^3 fine children (solid children skipped), the 19-point fneq
+//    filter over the neighbours' child averages, the projection, the rescale
+//    m tau_c / tau_f, and a blend into the existing coarse value ramping over
+//    3 cells from the edge of the restricted box.
+// Organised for the in-place store and CDNA4:
+//  * the coarse step overwrites time t in place, so the ring of coarse cells
+//    the shell fill reads (3 cells wide around the box edge) is copied to a
+//    compact FP32 band before the coarse step;
+//  * all populations are handled as deviations f - w (interpolation weights
+//    sum to one, so interpolating deviations is interpolating f);
+//  * the restriction stages each workgroup's child averages (tile plus a
+//    one-cell halo) in LDS once, then filters from LDS: 2.5x fine-array reads
+//    per coarse cell instead of 19x.
+#include <hip/hip_runtime.h>
+
+#include "fc_kcommon.hpp"
+
+namespace fc {
+namespace {
+
+constexpr int kRestrictBand = 2;   // coarse cells skipped at the box edge (lbm_refine.cuh:77)
+constexpr int kRestrictBlend = 3;  // blend ramp width (lbm_refine.cuh:85)
+constexpr float kMaxSpeed = 0.30f;
+
+struct RArgs {
+    int x0, y0, x1, y1, m;
+    int cnx, cny, cnz;
+    long long cn;
+    int fnx, fny, fnz;
+    long long fn;
+    int ew;  // expanded box width (W + 2): columns x0-1 .. x1
+    int eh;  // expanded box height (H + 2)
+};
+
+RArgs makeRArgs(const RefineGeom& g) {
+    RArgs r{};
+    r.x0 = g.box.x0;
+    r.y0 = g.box.y0;
+    r.x1 = g.box.x1;
+    r.y1 = g.box.y1;
+    r.m = g.factor;
+    r.cnx = g.coarse.nx;
+    r.cny = g.coarse.ny;
+    r.cnz = g.coarse.nz;
+    r.cn = g.coarse.cells();
+    r.fnx = g.fine.nx;
+    r.fny = g.fine.ny;
+    r.fnz = g.fine.nz;
+    r.fn = g.fine.cells();
+    r.ew = g.box.x1 - g.box.x0 + 2;
+    r.eh = g.box.y1 - g.box.y0 + 2;
+    return r;
+}
+
+__device__ inline long long cellOf(int x, int y, int z, int nx, int ny) {
+    return static_cast<long long>(x) + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
+}
+
+// Post-collision deviation f*_q - w_q of cell (x,y,z) of a level stored in
+// `layout` (x / y in range; z periodic).
+template <class S>
+__device__ inline float readH(const void* fv, long long n, int layout, int q, int x, int y, int z, int nx,
+                              int ny, int nz) {
+    const typename S::T* f = static_cast<const typename S::T*>(fv);
+    if (layout == kLayoutL)
+        return static_cast<float>(S::load(f + static_cast<long long>(kOpp[q]) * n + cellOf(x, y, z, nx, ny))) /
+               S::K;
+    int xx = x + kCx[q], yy = y + kCy[q], zz = z + kCz[q];
+    if (zz < 0) zz += nz;
+    if (zz >= nz) zz -= nz;
+    xx = xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx);  // slots outside x / y are never read back
+    yy = yy < 0 ? 0 : (yy >= ny ? ny - 1 : yy);
+    return static_cast<float>(S::load(f + static_cast<long long>(q) * n + cellOf(xx, yy, zz, nx, ny))) / S::K;
+}
+
+// Store f*_q - w_q of cell (x,y,z) into its slot for `layout`; in layout S a
+// slot outside the non-periodic x / y range belongs to no cell and is skipped.
+template <class S>
+__device__ inline void writeH(void* fv, long long n, int layout, int q, int x, int y, int z, int nx, int ny,
+                              int nz, float h) {
+    typename S::T* f = static_cast<typename S::T*>(fv);
+    using C = typename S::C;
+    if (layout == kLayoutL) {
+        S::store(f + static_cast<long long>(kOpp[q]) * n + cellOf(x, y, z, nx, ny), C(h) * C(S::K));
+        return;
+    }
+    const int xx = x + kCx[q], yy = y + kCy[q];
+    int zz = z + kCz[q];
+    if (xx < 0 || xx >= nx || yy < 0 || yy >= ny) return;
+    if (zz < 0) zz += nz;
+    if (zz >= nz) zz -= nz;
+    S::store(f + static_cast<long long>(q) * n + cellOf(xx, yy, zz, nx, ny), C(h) * C(S::K));
+}
+
+// Latt-Chopard projection of fneq onto the second-order Hermite space
+// (lbm_refine.cu:64-95).
+__device__ inline void regularize(float fneq[kQ]) {
+    float pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float f = fneq[q];
+        const float cx = kCx[q], cy = kCy[q], cz = kCz[q];
+        pxx += cx * cx * f;
+        pyy += cy * cy * f;
+        pzz += cz * cz * f;
+        pxy += cx * cy * f;
+        pxz += cx * cz * f;
+        pyz += cy * cz * f;
+    }
+    const float trace = (1.0f / 3.0f) * (pxx + pyy + pzz);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float cx = kCx[q], cy = kCy[q], cz = kCz[q];
+        const float qPi = cx * cx * pxx + cy * cy * pyy + cz * cz * pzz +
+                          2.0f * (cx * cy * pxy + cx * cz * pxz + cy * cz * pyz) - trace;
+        fneq[q] = 4.5f * wOf<float>(q) * qPi;
+    }
+}
+
+struct Mom {
+    float rho, ux, uy, uz;
+};
+
+// Capped moments of a deviation set (density floor 0.05, |u| cap 0.30, as the
+// collision; lbm_refine.cu:211-226).
+__device__ inline Mom momentsOf(const float h[kQ]) {
+    float r = 1.0f, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        r += h[q];
+        jx += kCx[q] * h[q];
+        jy += kCy[q] * h[q];
+        jz += kCz[q] * h[q];
+    }
+    r = fmaxf(r, 0.05f);
+    const float inv = 1.0f / r;
+    Mom mo{r, jx * inv, jy * inv, jz * inv};
+    const float u2 = mo.ux * mo.ux + mo.uy * mo.uy + mo.uz * mo.uz;
+    if (u2 > kMaxSpeed * kMaxSpeed) {
+        const float sc = kMaxSpeed * frsqrt(u2);
+        mo.ux *= sc;
+        mo.uy *= sc;
+        mo.uz *= sc;
+    }
+    return mo;
+}
+
+// feq_q(rho, u) - w_q.
+__device__ inline float heq(int q, const Mom& mo) {
+    const float usq = 1.5f * (mo.ux * mo.ux + mo.uy * mo.uy + mo.uz * mo.uz);
+    const float cu = 3.0f * (kCx[q] * mo.ux + kCy[q] * mo.uy + kCz[q] * mo.uz);
+    return wOf<float>(q) * (mo.rho - 1.0f + mo.rho * (cu + 0.5f * cu * cu - usq));
+}
+
+// ---------------------------------------------------------------------------
+// Band snapshot: post-collision deviations of the coarse ring cells at time t.
+// band[(q * nz + z) * nband + i] for ring cell i = (bandXY[2i], bandXY[2i+1]).
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(kBlock) bandSnapshotKernel(const void* f, int layout, RArgs r, const int* bandXY,
+                                                            int nband, float* band) {
+    const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    if (t >= static_cast<long long>(nband) * r.cnz) return;
+    const int z = static_cast<int>(t / nband);
+    const int i = static_cast<int>(t - static_cast<long long>(z) * nband);
+    const int x = bandXY[2 * i], y = bandXY[2 * i + 1];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+        band[(static_cast<long long>(q) * r.cnz + z) * nband + i] =
+            readH<S>(f, r.cn, layout, q, x, y, z, r.cnx, r.cny, r.cnz);
+}
+
+// Fine cell of shell index t (2-cell Interface ring on the x / y faces of every
+// fine z plane), or false past the end.
+__device__ inline bool shellCell(long long t, const RArgs& r, int& x, int& y, int& z) {
+    const long long ring = 4LL * r.fnx + 4LL * (r.fny - 4);
+    z = static_cast<int>(t / ring);
+    if (z >= r.fnz) return false;
+    long long k = t - static_cast<long long>(z) * ring;
+    if (k < 2LL * r.fnx) {
+        y = static_cast<int>(k / r.fnx);
+        x = static_cast<int>(k - static_cast<long long>(y) * r.fnx);
+        return true;
+    }
+    k -= 2LL * r.fnx;
+    if (k < 2LL * r.fnx) {
+        const int yy = static_cast<int>(k / r.fnx);
+        x = static_cast<int>(k - static_cast<long long>(yy) * r.fnx);
+        y = r.fny - 2 + yy;
+        return true;
+    }
+    k -= 2LL * r.fnx;
+    y = 2 + static_cast<int>(k / 4);
+    const int c = static_cast<int>(k % 4);
+    x = c < 2 ? c : r.fnx - 4 + c;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-to-fine fill. FULL: every non-solid fine cell, time levels both the
+// live coarse array (seeding). Otherwise: the Interface shell, t from the
+// band snapshot and t+1 from the live coarse array.
+// ---------------------------------------------------------------------------
+template <class S, bool FULL>
+__global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::uint8_t* ckind, int cLayout,
+                                                    const float* band, const int* bandIndex, int nband,
+                                                    void* ff, const std::uint8_t* fkind, int fLayout, RArgs r,
+                                                    float timeWeight, float neqScale) {
+    const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    int xf, yf, zf;
+    if (FULL) {
+        if (t >= r.fn) return;
+        zf = static_cast<int>(t / (static_cast<long long>(r.fnx) * r.fny));
+        const long long rem = t - static_cast<long long>(zf) * r.fnx * r.fny;
+        yf = static_cast<int>(rem / r.fnx);
+        xf = static_cast<int>(rem - static_cast<long long>(yf) * r.fnx);
+        if ((fkind[t] & kKindFlagMask) == kSolid) return;
+    } else {
+        if (!shellCell(t, r, xf, yf, zf)) return;
+        if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) return;
+    }
+    const float invM = 1.0f / static_cast<float>(r.m);
+    const float gx = static_cast<float>(r.x0) + (static_cast<float>(xf) + 0.5f) * invM - 0.5f;
+    const float gy = static_cast<float>(r.y0) + (static_cast<float>(yf) + 0.5f) * invM - 0.5f;
+    const float gz = (static_cast<float>(zf) + 0.5f) * invM - 0.5f;
+    const int ix = static_cast<int>(floorf(gx)), iy = static_cast<int>(floorf(gy)),
+              iz = static_cast<int>(floorf(gz));
+    const float tx = gx - ix, ty = gy - iy, tz = gz - iz;
+    float wgt[8] = {(1 - tx) * (1 - ty) * (1 - tz), tx * (1 - ty) * (1 - tz), (1 - tx) * ty * (1 - tz),
+                    tx * ty * (1 - tz),             (1 - tx) * (1 - ty) * tz, tx * (1 - ty) * tz,
+                    (1 - tx) * ty * tz,             tx * ty * tz};
+    int cx[8], cy[8], cz[8];
+    float wsum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        cx[c] = min(max(ix + (c & 1), 0), r.cnx - 1);
+        cy[c] = min(max(iy + ((c >> 1) & 1), 0), r.cny - 1);
+        int zz = iz + ((c >> 2) & 1);
+        if (zz < 0) zz += r.cnz;
+        if (zz >= r.cnz) zz -= r.cnz;
+        cz[c] = zz;
+        if ((ckind[cellOf(cx[c], cy[c], cz[c], r.cnx, r.cny)] & kKindFlagMask) == kSolid) wgt[c] = 0.0f;
+        wsum += wgt[c];
+    }
+    if (wsum < 1e-6f) return;
+    const float invW = 1.0f / wsum;
+    // Ring slot of each corner in the band snapshot (every shell corner lies in
+    // the ring; a corner outside it would fall back to the live array).
+    int slot[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        wgt[c] *= invW;
+        slot[c] = -1;
+        if (!FULL) {
+            const int ex = cx[c] - (r.x0 - 1), ey = cy[c] - (r.y0 - 1);
+            if (ex >= 0 && ex < r.ew && ey >= 0 && ey < r.eh) slot[c] = bandIndex[ey * r.ew + ex];
+        }
+    }
+    float h[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (wgt[c] == 0.0f) continue;  // solid corner: skip its (unused) slots
+            const float v1 = readH<S>(cf, r.cn, cLayout, q, cx[c], cy[c], cz[c], r.cnx, r.cny, r.cnz);
+            const float v0 =
+                slot[c] < 0 ? v1 : band[(static_cast<long long>(q) * r.cnz + cz[c]) * nband + slot[c]];
+            s0 += wgt[c] * v0;
+            s1 += wgt[c] * v1;
+        }
+        h[q] = s0 + timeWeight * (s1 - s0);
+    }
+    const Mom mo = momentsOf(h);
+    float fneq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) fneq[q] = h[q] - heq(q, mo);
+    regularize(fneq);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+        writeH<S>(ff, r.fn, fLayout, q, xf, yf, zf, r.fnx, r.fny, r.fnz, heq(q, mo) + neqScale * fneq[q]);
+}
+
+// ---------------------------------------------------------------------------
+// Fine-to-coarse restriction, LDS-tiled: a workgroup covers TX x TY x TZ
+// coarse cells of the restricted box; phase 1 stores the child-averaged fneq
+// (and, for the tile's own cells, the capped moments) of the tile plus a
+// one-cell halo in LDS; phase 2 filters, projects, rescales and blends.
+// ---------------------------------------------------------------------------
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;  // 648 cells, 19 floats each = 49 KB
+
+template <class S>
+__global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
+                                                        void* cf, const std::uint8_t* ckind, int cLayout, RArgs r,
+                                                        float neqScaleInv, float* mRho, float* mU, float* mV,
+                                                        float* mW) {
+    __shared__ float sFneq[kQ][kHalo];
+    __shared__ unsigned char sValid[kHalo];
+    __shared__ float sMom[4][kTX * kTY * kTZ];
+    const int rx0 = r.x0 + kRestrictBand, ry0 = r.y0 + kRestrictBand;
+    const int rx1 = r.x1 - kRestrictBand, ry1 = r.y1 - kRestrictBand;
+    const int ntx = (rx1 - rx0 + kTX - 1) / kTX, nty = (ry1 - ry0 + kTY - 1) / kTY;
+    const int b = blockIdx.x;
+    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    const int ox = rx0 + bx * kTX, oy = ry0 + by * kTY, oz = bz * kTZ;
+
+    // Phase 1: child averages of the halo tile.
+    for (int i = threadIdx.x; i < kHalo; i += kBlock) {
+        const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
+        const int xc = ox + hx - 1, yc = oy + hy - 1;
+        int zc = oz + hz - 1;
+        zc = (zc % r.cnz + r.cnz) % r.cnz;
+        unsigned char ok = 0;
+        if (xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
+            float h[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) h[q] = 0.0f;
+            int nf = 0;
+            const int fx = r.m * (xc - r.x0), fy = r.m * (yc - r.y0), fz = r.m * zc;
+            for (int k = 0; k < r.m; ++k)
+                for (int j = 0; j < r.m; ++j)
+                    for (int l = 0; l < r.m; ++l) {
+                        const int x = fx + l, y = fy + j, z = fz + k;
+                        if ((fkind[cellOf(x, y, z, r.fnx, r.fny)] & kKindFlagMask) == kSolid) continue;
+                        ++nf;
+#pragma unroll
+                        for (int q = 0; q < kQ; ++q) h[q] += readH<S>(ff, r.fn, fLayout, q, x, y, z, r.fnx, r.fny, r.fnz);
+                    }
+            if (nf > 0) {
+                ok = 1;
+                const float invN = 1.0f / static_cast<float>(nf);
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) h[q] *= invN;
+                const Mom mo = momentsOf(h);
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) sFneq[q][i] = h[q] - heq(q, mo);
+                if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY && hz >= 1 && hz <= kTZ) {
+                    const int o = (hx - 1) + kTX * ((hy - 1) + kTY * (hz - 1));
+                    sMom[0][o] = mo.rho;
+                    sMom[1][o] = mo.ux;
+                    sMom[2][o] = mo.uy;
+                    sMom[3][o] = mo.uz;
+                }
+            }
+        }
+        sValid[i] = ok;
+    }
+    __syncthreads();
+
+    // Phase 2: one coarse cell per thread.
+    const int lx = threadIdx.x % kTX, ly = (threadIdx.x / kTX) % kTY, lz = threadIdx.x / (kTX * kTY);
+    const int xc = ox + lx, yc = oy + ly, zc = oz + lz;
+    if (xc >= rx1 || yc >= ry1 || zc >= r.cnz) return;
+    const long long cc = cellOf(xc, yc, zc, r.cnx, r.cny);
+    if ((ckind[cc] & kKindFlagMask) != kFluid) return;
+    const int hc = (lx + 1) + kHX * ((ly + 1) + kHY * (lz + 1));
+    if (!sValid[hc]) return;
+    const int dEdge = min(min(xc - rx0, rx1 - 1 - xc), min(yc - ry0, ry1 - 1 - yc));
+    const float blend = fminf(1.0f, static_cast<float>(dEdge + 1) / static_cast<float>(kRestrictBlend + 1));
+    float fneq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) fneq[q] = sFneq[q][hc];
+    int nS = 1;
+    for (int n = 1; n < kQ; ++n) {
+        const int hn = hc + kCx[n] + kHX * (kCy[n] + kHY * kCz[n]);
+        if (!sValid[hn]) continue;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) fneq[q] += sFneq[q][hn];
+        ++nS;
+    }
+    const float invS = 1.0f / static_cast<float>(nS);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) fneq[q] *= invS;
+    regularize(fneq);
+    const int o = lx + kTX * (ly + kTY * lz);
+    const Mom mo{sMom[0][o], sMom[1][o], sMom[2][o], sMom[3][o]};
+    float br = 0, bjx = 0, bjy = 0, bjz = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float existing = readH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz);
+        const float restricted = heq(q, mo) + neqScaleInv * fneq[q];
+        const float blended = existing + blend * (restricted - existing);
+        writeH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz, blended);
+        br += blended;
+        bjx += kCx[q] * blended;
+        bjy += kCy[q] * blended;
+        bjz += kCz[q] * blended;
+    }
+    if (mRho) {
+        const float rr = fmaxf(1.0f + br, 0.05f);
+        const float inv = 1.0f / rr;
+        mRho[cc] = rr;
+        mU[cc] = bjx * inv;
+        mV[cc] = bjy * inv;
+        mW[cc] = bjz * inv;
+    }
+}
+
+template <class S>
+struct RefineOps {
+    static int snapshot(const LatticeView& c, int layout, const RArgs& r, const int* bandXY, int nband,
+                        float* band, hipStream_t st) {
+        const long long n = static_cast<long long>(nband) * r.cnz;
+        bandSnapshotKernel<S><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+            c.f, layout, r, bandXY, nband, band);
+        return static_cast<int>(hipGetLastError());
+    }
+    static int fill(const LatticeView& c, int cLayout, const float* band, const int* bandIndex, int nband,
+                    const LatticeView& f, int fLayout, const RArgs& r, float tw, float neqScale, bool full,
+                    hipStream_t st) {
+        if (full) {
+            fillKernel<S, true><<<static_cast<unsigned>((r.fn + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale);
+        } else {
+            const long long n = (4LL * r.fnx + 4LL * (r.fny - 4)) * r.fnz;
+            fillKernel<S, false><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale);
+        }
+        return static_cast<int>(hipGetLastError());
+    }
+    static int restrictTo(const LatticeView& f, int fLayout, const LatticeView& c, int cLayout, const RArgs& r,
+                          float neqScaleInv, float* rho, float* u, float* v, float* w, hipStream_t st) {
+        const int rw = r.x1 - r.x0 - 2 * kRestrictBand, rh = r.y1 - r.y0 - 2 * kRestrictBand;
+        if (rw <= 0 || rh <= 0) return static_cast<int>(hipErrorInvalidValue);
+        const unsigned blocks = static_cast<unsigned>(((rw + kTX - 1) / kTX) * ((rh + kTY - 1) / kTY) *
+                                                      ((r.cnz + kTZ - 1) / kTZ));
+        const bool macro = rho && u && v && w;
+        restrictKernel<S><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r, neqScaleInv,
+                                                     macro ? rho : nullptr, u, v, w);
+        return static_cast<int>(hipGetLastError());
+    }
+};
+
+bool geomOk(const RefineGeom& g) {
+    const int w = g.box.x1 - g.box.x0, h = g.box.y1 - g.box.y0;
+    return g.factor >= 2 && g.factor <= 4 && w > 8 && h > 8 && g.box.x0 >= 2 && g.box.y0 >= 2 &&
+           g.box.x1 <= g.coarse.nx - 2 && g.box.y1 <= g.coarse.ny - 2 && g.fine.nx == g.factor * w &&
+           g.fine.ny == g.factor * h && g.fine.nz == g.factor * g.coarse.nz;
+}
+
+}  // namespace
+
+int launchBandSnapshot(const LatticeView& coarse, int layout, const RefineGeom& g, const int* bandXY, int nband,
+                       float* band, void* stream) {
+    if (!geomOk(g) || !bandXY || !band || nband <= 0) return static_cast<int>(hipErrorInvalidValue);
+    const RArgs r = makeRArgs(g);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (coarse.storage) {
+        case kStoreF32: return RefineOps<StoreF32>::snapshot(coarse, layout, r, bandXY, nband, band, st);
+        case kStoreF16: return RefineOps<StoreF16>::snapshot(coarse, layout, r, bandXY, nband, band, st);
+        case kStoreBF16: return RefineOps<StoreBF16>::snapshot(coarse, layout, r, bandXY, nband, band, st);
+        case kStoreF64: return RefineOps<StoreF64>::snapshot(coarse, layout, r, bandXY, nband, band, st);
+        case kStoreF16C: return RefineOps<StoreF16C>::snapshot(coarse, layout, r, bandXY, nband, band, st);
+        default: return static_cast<int>(hipErrorInvalidValue);
+    }
+}
+
+int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, const int* bandIndex, int nband,
+                   const LatticeView& fine, int fLayout, const RefineGeom& g, float timeWeight, float neqScale,
+                   int fullVolume, void* stream) {
+    if (!geomOk(g) || coarse.storage != fine.storage) return static_cast<int>(hipErrorInvalidValue);
+    if (!fullVolume && (!band || !bandIndex || nband <= 0)) return static_cast<int>(hipErrorInvalidValue);
+    const RArgs r = makeRArgs(g);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool full = fullVolume != 0;
+    switch (coarse.storage) {
+        case kStoreF32:
+            return RefineOps<StoreF32>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
+                                             neqScale, full, st);
+        case kStoreF16:
+            return RefineOps<StoreF16>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
+                                             neqScale, full, st);
+        case kStoreBF16:
+            return RefineOps<StoreBF16>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r,
+                                              timeWeight, neqScale, full, st);
+        case kStoreF64:
+            return RefineOps<StoreF64>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
+                                             neqScale, full, st);
+        case kStoreF16C:
+            return RefineOps<StoreF16C>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r,
+                                              timeWeight, neqScale, full, st);
+        default: return static_cast<int>(hipErrorInvalidValue);
+    }
+}
+
+int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
+                       const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
+                       void* stream) {
+    if (!geomOk(g) || coarse.storage != fine.storage) return static_cast<int>(hipErrorInvalidValue);
+    const RArgs r = makeRArgs(g);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (coarse.storage) {
+        case kStoreF32:
+            return RefineOps<StoreF32>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+        case kStoreF16:
+            return RefineOps<StoreF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+        case kStoreBF16:
+            return RefineOps<StoreBF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
+                                                    st);
+        case kStoreF64:
+            return RefineOps<StoreF64>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+        case kStoreF16C:
+            return RefineOps<StoreF16C>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
+                                                    st);
+        default: return static_cast<int>(hipErrorInvalidValue);
+    }
+}
+
+}  // namespace fc

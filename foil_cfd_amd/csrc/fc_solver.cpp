@@ -234,6 +234,7 @@ public:
         hipRelease(dForceCells_);
         hipRelease(dForceMasks_);
         hipRelease(dForcePartials_);
+        freeFine();
         wm_.free();
         hipRelease(dMlp_);
         hipRelease(dParticles_);
@@ -289,6 +290,9 @@ public:
                                       speckPhase(kSpeckSeed, 1u), speckPhase(kSpeckSeed, 2u),
                                       speckPhase(kSpeckSeed, 3u), stream_);
         if (e != 0) return fail(FC_ERR_DEVICE, "init kernels failed");
+        if (fine_.active) {
+            if (int fe = seedFine(); fe != FC_OK) return fe;
+        }
         // Wall-model EMA state and slip field restart with the flow.
         if (wm_.active) {
             const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
@@ -345,16 +349,26 @@ public:
             if (qCount_ > 0 && qlibbEnabled_ &&
                 launchQLibbPrepass(lat, layout_, dQCells_, dQPacked_, qCount_, dQScratch_, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "q-LIBB pre-pass failed");
+            // The coarse step overwrites time t in place: keep the ring the
+            // fine shell fill interpolates from (time t) first.
+            if (fine_.active &&
+                launchBandSnapshot(lat, layout_, fineGeom(), fine_.dBandXY, fine_.nband, fine_.dBand, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "refinement band snapshot failed");
             const int e = launchStep(lat, layout_, a, stream_);
             if (ev) (void)hipEventRecord(ev->second, stream_);
             if (e != 0) return fail(FC_ERR_DEVICE, std::string("step launch failed: ") +
                                                        hipGetErrorString(static_cast<hipError_t>(e)));
+            if (fine_.active) {
+                if (int fe = fineSubSteps(a, 1 - layout_); fe != FC_OK) return fe;
+            }
             layout_ = 1 - layout_;
             ++steps_;
             if (++stepsSinceWatchdog_ >= kWatchdogPeriodSteps && !nanPending_) {
                 stepsSinceWatchdog_ = 0;
                 (void)hipMemsetAsync(dNan_, 0, sizeof(int), stream_);
                 launchWatchdog(lat, layout_, steps_ * 977, 0, dNan_, stream_);
+                // The fine level is sampled too (verdict 3 / 4 names it).
+                if (fine_.active) launchWatchdog(fineView(), fine_.layout, steps_ * 977, 2, dNan_, stream_);
                 (void)hipMemcpyAsync(hNan_, dNan_, sizeof(int), hipMemcpyDeviceToHost, stream_);
                 (void)hipEventRecord(evNan_, stream_);
                 nanPending_ = true;
@@ -634,9 +648,13 @@ public:
         if (!initialized_) return FC_OK;
         pollAsync();
         r->flow_throughs = flowThroughsCompleted();
-        const int span = zSlipWalls_ ? std::max(1, dims_.nz - 2) : dims_.nz;
-        r->cd = scaling_.forceToCoefficient(emaFx_, span);
-        r->cl = scaling_.forceToCoefficient(emaFy_, span);
+        // lbm_solver.cpp:1429-1435: fine scaling and fine span when the
+        // momentum exchange runs on the fine grid.
+        const bool fineForces = fine_.active && fine_.forcesFromFine;
+        const Scaling& fs = fineForces ? fine_.scaling : scaling_;
+        const int span = fineForces ? fine_.dims.nz : (zSlipWalls_ ? std::max(1, dims_.nz - 2) : dims_.nz);
+        r->cd = fs.forceToCoefficient(emaFx_, span);
+        r->cl = fs.forceToCoefficient(emaFy_, span);
         r->lift_to_drag = (std::fabs(r->cd) > 1e-6f) ? r->cl / r->cd : 0.0f;
         r->valid = (emaSeeded_ && steps_ >= gateOpenAtSteps_) ? 1 : 0;
         if (r->valid && ringCount_ > 0) {
@@ -646,8 +664,8 @@ public:
                 const int idx = (ringHead_ - 1 - i + kAvgRingSize) % kAvgRingSize;
                 const Sample& smp = ring_[idx];
                 if (smp.step < oneFT) continue;
-                const float c1 = scaling_.forceToCoefficient(smp.fy, span);
-                const float c2 = scaling_.forceToCoefficient(smp.fx, span);
+                const float c1 = fs.forceToCoefficient(smp.fy, span);
+                const float c2 = fs.forceToCoefficient(smp.fx, span);
                 cl.push_back(c1);
                 cd.push_back(c2);
                 ld.push_back((std::fabs(c2) > 1e-6f) ? c1 / c2 : 0.0f);
@@ -788,8 +806,12 @@ public:
         *p = fc_perf_stats{};
         p->last_steps_per_frame = timingBatchN_;
         p->last_step_ms = lastStepMs_;
-        if (lastStepMs_ > 0.0) p->mlups = static_cast<double>(ncells_) / (lastStepMs_ * 1000.0);
-        p->population_bytes_per_step = 2.0 * kQ * storageBytes(storage_) * static_cast<double>(ncells_);
+        // Cell updates per coarse step: every coarse cell once, every fine cell
+        // m times (lbm_solver.cpp:1705-1713).
+        const double updates = static_cast<double>(ncells_) +
+                               (fine_.active ? static_cast<double>(fine_.factor) * fine_.ncells : 0.0);
+        if (lastStepMs_ > 0.0) p->mlups = updates / (lastStepMs_ * 1000.0);
+        p->population_bytes_per_step = 2.0 * kQ * storageBytes(storage_) * updates;
         return FC_OK;
     }
 
@@ -1060,45 +1082,65 @@ private:
         return true;
     }
 
-    int uploadFlags() {
-        // Kind bytes: flag + special / feed bits.
-        hostKind_.assign(static_cast<std::size_t>(ncells_), 0);
-        std::vector<std::uint8_t>& kind = hostKind_;
+    // Kind bytes (flag + special / feed bits) and the momentum-exchange link
+    // list of one level's flag field.
+    struct KindBuild {
+        std::vector<std::uint8_t> kind;
         std::vector<int> fcells;
         std::vector<std::uint32_t> fmasks;
-        zSlipWalls_ = false;
-        for (long long i = 0; i < static_cast<long long>(dims_.nx) * dims_.ny && i < ncells_; ++i)
-            if (hostFlags_[static_cast<std::size_t>(i)] == kSlipFront) zSlipWalls_ = true;
-        for (int z = 0; z < dims_.nz; ++z)
-            for (int y = 0; y < dims_.ny; ++y)
-                for (int x = 0; x < dims_.nx; ++x) {
-                    const std::size_t c = at(x, y, z);
-                    const std::uint8_t fl = hostFlags_[c];
-                    std::uint8_t k = fl;
-                    if (fl == kFluid) {
+    };
+    static void buildKinds(const std::vector<std::uint8_t>& fl, const Dims& d, bool px, bool py, KindBuild& out) {
+        const std::size_t n = static_cast<std::size_t>(d.cells());
+        out.kind.assign(n, 0);
+        out.fcells.clear();
+        out.fmasks.clear();
+        auto idx = [&](int x, int y, int z) {
+            return static_cast<std::size_t>(x) +
+                   static_cast<std::size_t>(d.nx) * (static_cast<std::size_t>(y) +
+                                                     static_cast<std::size_t>(d.ny) * static_cast<std::size_t>(z));
+        };
+        for (int z = 0; z < d.nz; ++z)
+            for (int y = 0; y < d.ny; ++y)
+                for (int x = 0; x < d.nx; ++x) {
+                    const std::size_t c = idx(x, y, z);
+                    const std::uint8_t f = fl[c];
+                    std::uint8_t k = f;
+                    if (f == kFluid) {
                         std::uint32_t solidMask = 0;
                         for (int q = 1; q < kQ; ++q) {
                             // pull neighbour c - e_q
-                            const int xn = stepAxis(x, -kCx[q], dims_.nx, periodicX_);
-                            const int yn = stepAxis(y, -kCy[q], dims_.ny, periodicY_);
-                            const int zn = stepAxis(z, -kCz[q], dims_.nz, true);
-                            const std::uint8_t nf = hostFlags_[at(xn, yn, zn)];
+                            const int xn = stepAxis(x, -kCx[q], d.nx, px);
+                            const int yn = stepAxis(y, -kCy[q], d.ny, py);
+                            const int zn = stepAxis(z, -kCz[q], d.nz, true);
+                            const std::uint8_t nf = fl[idx(xn, yn, zn)];
                             if (nf == kSolid || (nf >= kSlipTop && nf <= kSlipBack)) k |= kKindSpecial;
                             // push neighbour c + e_q for the momentum exchange
-                            const int xp = stepAxis(x, kCx[q], dims_.nx, periodicX_);
-                            const int yp = stepAxis(y, kCy[q], dims_.ny, periodicY_);
-                            const int zp = stepAxis(z, kCz[q], dims_.nz, true);
-                            if (hostFlags_[at(xp, yp, zp)] == kSolid) solidMask |= (1u << q);
+                            const int xp = stepAxis(x, kCx[q], d.nx, px);
+                            const int yp = stepAxis(y, kCy[q], d.ny, py);
+                            const int zp = stepAxis(z, kCz[q], d.nz, true);
+                            if (fl[idx(xp, yp, zp)] == kSolid) solidMask |= (1u << q);
                         }
                         if (solidMask) {
-                            fcells.push_back(static_cast<int>(c));
-                            fmasks.push_back(solidMask);
+                            out.fcells.push_back(static_cast<int>(c));
+                            out.fmasks.push_back(solidMask);
                         }
                     }
-                    const int xr = stepAxis(x, 1, dims_.nx, periodicX_);
-                    if (xr >= 0 && hostFlags_[at(xr, y, z)] == kOutlet) k |= kKindFeed;
-                    kind[c] = k;
+                    const int xr = stepAxis(x, 1, d.nx, px);
+                    if (xr >= 0 && fl[idx(xr, y, z)] == kOutlet) k |= kKindFeed;
+                    out.kind[c] = k;
                 }
+    }
+
+    int uploadFlags() {
+        KindBuild kb;
+        buildKinds(hostFlags_, dims_, periodicX_, periodicY_, kb);
+        hostKind_ = kb.kind;
+        std::vector<std::uint8_t>& kind = hostKind_;
+        std::vector<int>& fcells = kb.fcells;
+        std::vector<std::uint32_t>& fmasks = kb.fmasks;
+        zSlipWalls_ = false;
+        for (long long i = 0; i < static_cast<long long>(dims_.nx) * dims_.ny && i < ncells_; ++i)
+            if (hostFlags_[static_cast<std::size_t>(i)] == kSlipFront) zSlipWalls_ = true;
         if (hipMemcpyAsync(kind_, kind.data(), kind.size(), hipMemcpyHostToDevice, stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "flag upload failed");
         hipRelease(dForceCells_);
@@ -1124,11 +1166,292 @@ private:
     }
 
     int launchForceInto(double* dOut) {
+        // On the fine grid when the patch covers every coarse solid cell
+        // (lbm_solver.cpp:1337-1345): twice the wall resolution.
+        if (fine_.active && fine_.forcesFromFine) {
+            if (launchForce(fineView(), fine_.layout, fine_.dForceCells, fine_.dForceMasks, fine_.forceCount,
+                            fine_.dForcePartials, fine_.forceSlots, dOut, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "fine force kernel failed");
+            return FC_OK;
+        }
         if (launchForce(view(), layout_, dForceCells_, dForceMasks_, forceCount_, dForcePartials_,
                         forceSlots_, dOut, stream_) != 0)
             return fail(FC_ERR_DEVICE, "force kernel failed");
         return FC_OK;
     }
+
+public:
+    // ---- two-level refinement (reference initRefinement / stepN coupling,
+    // lbm_solver.cpp:1816-1895, :1172-1290) ----------------------------------
+    struct FineLevel {
+        bool active = false;
+        int factor = 0;
+        PatchBoxI box;
+        Dims dims;
+        Scaling scaling;
+        long long ncells = 0;
+        void* f = nullptr;
+        std::uint8_t* kind = nullptr;
+        int layout = kLayoutL;
+        std::vector<std::uint8_t> hostFlags;
+        int* dForceCells = nullptr;
+        std::uint32_t* dForceMasks = nullptr;
+        double* dForcePartials = nullptr;
+        int forceCount = 0, forceSlots = 0;
+        bool forcesFromFine = false;
+        int* dBandXY = nullptr;
+        int* dBandIndex = nullptr;
+        float* dBand = nullptr;
+        int nband = 0;
+        long long bytes = 0;
+    };
+
+    RefineGeom fineGeom() const {
+        RefineGeom g;
+        g.box = fine_.box;
+        g.factor = fine_.factor;
+        g.coarse = dims_;
+        g.fine = fine_.dims;
+        return g;
+    }
+
+    LatticeView fineView() const {
+        LatticeView l;
+        l.f = fine_.f;
+        l.storage = storage_;
+        l.kind = fine_.kind;
+        l.dims = fine_.dims;
+        return l;
+    }
+
+    void freeFine() {
+        hipRelease(fine_.f);
+        hipRelease(fine_.kind);
+        hipRelease(fine_.dForceCells);
+        hipRelease(fine_.dForceMasks);
+        hipRelease(fine_.dForcePartials);
+        hipRelease(fine_.dBandXY);
+        hipRelease(fine_.dBandIndex);
+        hipRelease(fine_.dBand);
+        fine_ = FineLevel{};
+    }
+
+    // Fine kind bytes + force list from fine_.hostFlags, and whether the patch
+    // covers every coarse solid cell outside the restriction band
+    // (updateForcesFromFine, lbm_solver.cpp:553-575).
+    int uploadFineFlags() {
+        KindBuild kb;
+        buildKinds(fine_.hostFlags, fine_.dims, false, false, kb);
+        hipRelease(fine_.dForceCells);
+        hipRelease(fine_.dForceMasks);
+        hipRelease(fine_.dForcePartials);
+        fine_.forceCount = static_cast<int>(kb.fcells.size());
+        fine_.forceSlots = forcePartialSlots(fine_.forceCount);
+        int rc = hipAlloc(&fine_.dForceCells, kb.fcells.size() * sizeof(int));
+        rc |= hipAlloc(&fine_.dForceMasks, kb.fmasks.size() * sizeof(std::uint32_t));
+        rc |= hipAlloc(&fine_.dForcePartials, static_cast<std::size_t>(fine_.forceSlots) * 3 * sizeof(double));
+        if (rc != FC_OK) return fail(FC_ERR_ALLOC, "fine force list allocation failed");
+        if (hipMemcpyAsync(fine_.kind, kb.kind.data(), kb.kind.size(), hipMemcpyHostToDevice, stream_) !=
+                hipSuccess ||
+            (fine_.forceCount > 0 &&
+             (hipMemcpyAsync(fine_.dForceCells, kb.fcells.data(), kb.fcells.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream_) != hipSuccess ||
+              hipMemcpyAsync(fine_.dForceMasks, kb.fmasks.data(), kb.fmasks.size() * sizeof(std::uint32_t),
+                             hipMemcpyHostToDevice, stream_) != hipSuccess)) ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "fine flag upload failed");
+        const int x0 = fine_.box.x0 + 2, x1 = fine_.box.x1 - 2, y0 = fine_.box.y0 + 2, y1 = fine_.box.y1 - 2;
+        fine_.forcesFromFine = true;
+        for (int z = 0; z < dims_.nz && fine_.forcesFromFine; ++z)
+            for (int y = 0; y < dims_.ny && fine_.forcesFromFine; ++y)
+                for (int x = 0; x < dims_.nx; ++x)
+                    if (hostFlags_[at(x, y, z)] == kSolid && (x < x0 || x >= x1 || y < y0 || y >= y1)) {
+                        fine_.forcesFromFine = false;
+                        break;
+                    }
+        return FC_OK;
+    }
+
+    // seedFineFromCoarse (lbm_solver.cpp:502-521): rest equilibrium, then the
+    // full-volume fill from the current coarse state, in layout L.
+    int seedFine() {
+        fine_.layout = kLayoutL;
+        const float tauC = effectiveTau();
+        const float tauF = fineTauFor(tauC, fine_.factor);
+        const float neqScale = tauF / (static_cast<float>(fine_.factor) * tauC);
+        if (launchInitEquilibrium(fineView(), 0.0f, stream_) != 0 ||
+            launchFineFill(view(), layout_, nullptr, nullptr, 0, fineView(), kLayoutL, fineGeom(), 0.0f, neqScale,
+                           1, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "fine level seeding failed");
+        return FC_OK;
+    }
+
+    // The m fine sub-steps of one coarse step plus the restriction into the
+    // coarse state just written in layout cLayout (lbm_solver.cpp:1172-1290).
+    int fineSubSteps(const StepArgs& coarseArgs, int cLayout) {
+        const int m = fine_.factor;
+        const float tauC = coarseArgs.tau0;
+        const float tauF = fineTauFor(tauC, m);
+        StepArgs fa = coarseArgs;
+        fa.tau0 = tauF;
+        fa.writeMacro = 0;
+        const float neqScale = tauF / (static_cast<float>(m) * tauC);
+        const RefineGeom g = fineGeom();
+        const LatticeView cv = view();
+        const LatticeView fv = fineView();
+        for (int k = 0; k < m; ++k) {
+            if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
+                               static_cast<float>(k) / static_cast<float>(m), neqScale, 0, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "fine interface fill failed");
+            if (launchStep(fv, fine_.layout, fa, stream_) != 0) return fail(FC_ERR_DEVICE, "fine step failed");
+            fine_.layout = 1 - fine_.layout;
+        }
+        const float neqScaleInv = (static_cast<float>(m) * tauC) / tauF;
+        const bool macro = coarseArgs.writeMacro != 0;
+        if (launchFineRestrict(fv, fine_.layout, cv, cLayout, g, neqScaleInv, macro ? rho_ : nullptr,
+                               macro ? u_ : nullptr, macro ? v_ : nullptr, macro ? w_ : nullptr, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "fine restriction failed");
+        return FC_OK;
+    }
+
+    int initRefinement(const fc_patch_box* b, int factor, const std::uint8_t* fineFlags) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        freeFine();
+        if (!b || !fineFlags) return fail(FC_ERR_INVALID, "null argument");
+        if (factor < 2 || factor > 4) return fail(FC_ERR_INVALID, "refinement factor out of range (2..4)");
+        const int w = b->x1 - b->x0, h = b->y1 - b->y0;
+        // One coarse cell more clearance from the domain faces than the
+        // reference asks (x0 >= 1): the fill's coarse ring may then never
+        // contain inlet / outlet / slip cells, whose in-place slots for
+        // outward directions do not exist.
+        if (w <= 8 || h <= 8 || b->x0 < 2 || b->y0 < 2 || b->x1 > dims_.nx - 2 || b->y1 > dims_.ny - 2)
+            return fail(FC_ERR_INVALID, "refinement patch box out of range");
+        if (periodicX_ || periodicY_) return fail(FC_ERR_INVALID, "refinement needs a non-periodic x / y domain");
+        FineLevel& fl = fine_;
+        fl.factor = factor;
+        fl.box = PatchBoxI{b->x0, b->y0, b->x1, b->y1};
+        fl.dims = Dims{factor * w, factor * h, factor * dims_.nz};
+        fl.ncells = fl.dims.cells();
+        if (fl.ncells >= (1LL << 31)) {
+            freeFine();
+            return fail(FC_ERR_INVALID, "fine grid exceeds 2^31 cells");
+        }
+        fl.scaling = refinedScaling(scaling_, factor);
+        fl.hostFlags.assign(fineFlags, fineFlags + fl.ncells);
+        for (int z = 0; z < fl.dims.nz; ++z)
+            for (int y = 0; y < fl.dims.ny; ++y)
+                for (int x = 0; x < fl.dims.nx; ++x) {
+                    const std::uint8_t v = fl.hostFlags[static_cast<std::size_t>(
+                        x + static_cast<long long>(fl.dims.nx) * (y + static_cast<long long>(fl.dims.ny) * z))];
+                    if (v != kFluid && v != kSolid && v != kInterface) {
+                        freeFine();
+                        return fail(FC_ERR_INVALID, "fine flags may only hold Fluid, Solid and Interface");
+                    }
+                    if (v == kFluid && (x == 0 || y == 0 || x == fl.dims.nx - 1 || y == fl.dims.ny - 1)) {
+                        freeFine();
+                        return fail(FC_ERR_INVALID, "fluid cell on a fine x / y face (missing Interface shell)");
+                    }
+                }
+        // Ring of the box expanded by one coarse cell that the shell fill reads.
+        const int ew = w + 2, eh = h + 2;
+        std::vector<int> bandXY, bandIndex(static_cast<std::size_t>(ew) * eh, -1);
+        for (int ey = 0; ey < eh; ++ey)
+            for (int ex = 0; ex < ew; ++ex) {
+                const int x = b->x0 - 1 + ex, y = b->y0 - 1 + ey;
+                if (x <= b->x0 + 1 || x >= b->x1 - 2 || y <= b->y0 + 1 || y >= b->y1 - 2) {
+                    bandIndex[static_cast<std::size_t>(ey) * ew + ex] = static_cast<int>(bandXY.size() / 2);
+                    bandXY.push_back(x);
+                    bandXY.push_back(y);
+                }
+            }
+        fl.nband = static_cast<int>(bandXY.size() / 2);
+        const std::size_t fBytes = static_cast<std::size_t>(kQ) * fl.ncells * storageBytes(storage_);
+        const std::size_t bandBytes = static_cast<std::size_t>(kQ) * dims_.nz * fl.nband * sizeof(float);
+        int rc = hipAlloc(&fl.f, fBytes);
+        rc |= hipAlloc(&fl.kind, static_cast<std::size_t>(fl.ncells));
+        rc |= hipAlloc(&fl.dBandXY, bandXY.size() * sizeof(int));
+        rc |= hipAlloc(&fl.dBandIndex, bandIndex.size() * sizeof(int));
+        rc |= hipAlloc(&fl.dBand, bandBytes);
+        if (rc != FC_OK) {
+            freeFine();
+            return fail(FC_ERR_ALLOC, "fine level allocation failed (" + std::to_string(fBytes) + " bytes)");
+        }
+        if (hipMemcpyAsync(fl.dBandXY, bandXY.data(), bandXY.size() * sizeof(int), hipMemcpyHostToDevice,
+                           stream_) != hipSuccess ||
+            hipMemcpyAsync(fl.dBandIndex, bandIndex.data(), bandIndex.size() * sizeof(int), hipMemcpyHostToDevice,
+                           stream_) != hipSuccess) {
+            freeFine();
+            return fail(FC_ERR_DEVICE, "band table upload failed");
+        }
+        fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes);
+        if (int e = uploadFineFlags(); e != FC_OK) {
+            freeFine();
+            return e;
+        }
+        fl.active = true;
+        if (int e = seedFine(); e != FC_OK) {
+            freeFine();
+            return e;
+        }
+        if (hipStreamSynchronize(stream_) != hipSuccess) {
+            freeFine();
+            return fail(FC_ERR_DEVICE, "fine level setup failed");
+        }
+        return FC_OK;
+    }
+
+    int setRefinedFlags(const std::uint8_t* fineFlags) {
+        if (!initialized_ || !fine_.active) return fail(FC_ERR_STATE, "no active refinement level");
+        if (!fineFlags) return fail(FC_ERR_INVALID, "null fine flags");
+        const fc_patch_box b{fine_.box.x0, fine_.box.y0, fine_.box.x1, fine_.box.y1};
+        return initRefinement(&b, fine_.factor, fineFlags);
+    }
+
+    int refinementInfo(fc_refinement_info* out) const {
+        *out = fc_refinement_info{};
+        if (!fine_.active) return FC_OK;
+        out->active = 1;
+        out->factor = fine_.factor;
+        out->box = fc_patch_box{fine_.box.x0, fine_.box.y0, fine_.box.x1, fine_.box.y1};
+        out->fine_dims = fc_dims{fine_.dims.nx, fine_.dims.ny, fine_.dims.nz};
+        out->fine_scaling = toC(fine_.scaling);
+        out->device_bytes = static_cast<double>(fine_.bytes);
+        out->forces_from_fine = fine_.forcesFromFine ? 1 : 0;
+        return FC_OK;
+    }
+
+    int downloadFineF(void* host, bool asF64) {
+        if (!fine_.active) return fail(FC_ERR_STATE, "no active refinement level");
+        if (!host) return fail(FC_ERR_INVALID, "null output");
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * fine_.ncells * (asF64 ? 8 : 4);
+        void* d = nullptr;
+        if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "download scratch");
+        int rc = FC_OK;
+        if (launchDownloadCanonical(fineView(), fine_.layout, d, asF64 ? 1 : 0, stream_) != 0 ||
+            hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            rc = fail(FC_ERR_DEVICE, "fine download failed");
+        hipRelease(d);
+        return rc;
+    }
+
+    int uploadFineF(const void* host, bool isF64) {
+        if (!fine_.active) return fail(FC_ERR_STATE, "no active refinement level");
+        if (!host) return fail(FC_ERR_INVALID, "null input");
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * fine_.ncells * (isF64 ? 8 : 4);
+        void* d = nullptr;
+        if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "upload scratch");
+        int rc = FC_OK;
+        if (hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            launchUploadCanonical(fineView(), d, isF64 ? 1 : 0, stream_) != 0 ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            rc = fail(FC_ERR_DEVICE, "fine upload failed");
+        hipRelease(d);
+        if (rc == FC_OK) fine_.layout = kLayoutL;
+        return rc;
+    }
+
+private:
 
     static bool eventDone(hipEvent_t ev) {
         const hipError_t st = hipEventQuery(ev);
@@ -1234,6 +1557,7 @@ private:
     void* f_ = nullptr;
     std::uint8_t* kind_ = nullptr;
     void* feed_ = nullptr;
+    FineLevel fine_;
     float *rho_ = nullptr, *u_ = nullptr, *v_ = nullptr, *w_ = nullptr;
     double* dForce_ = nullptr;
     double* dMass_ = nullptr;
@@ -1477,6 +1801,31 @@ int fc_set_smagorinsky_cs(fc_solver* s, float cs) {
     if (!s || cs < 0.0f) return FC_ERR_INVALID;
     s->impl.setCs(cs);
     return FC_OK;
+}
+
+int fc_init_refinement(fc_solver* s, const fc_patch_box* box, int factor, const uint8_t* fine_flags,
+                       char* err, int errlen) {
+    if (!s) return FC_ERR_INVALID;
+    const int rc = s->impl.initRefinement(box, factor, fine_flags);
+    if (rc != FC_OK) fc::copyErr(s->impl.error, err, errlen);
+    return rc;
+}
+int fc_shutdown_refinement(fc_solver* s) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.freeFine();
+    return FC_OK;
+}
+int fc_set_refined_flags(fc_solver* s, const uint8_t* fine_flags) {
+    return s ? s->impl.setRefinedFlags(fine_flags) : FC_ERR_INVALID;
+}
+int fc_get_refinement_info(fc_solver* s, fc_refinement_info* out) {
+    return (s && out) ? s->impl.refinementInfo(out) : FC_ERR_INVALID;
+}
+int fc_download_fine_f(fc_solver* s, void* host, int as_f64) {
+    return s ? s->impl.downloadFineF(host, as_f64 != 0) : FC_ERR_INVALID;
+}
+int fc_upload_fine_f(fc_solver* s, const void* host, int is_f64) {
+    return s ? s->impl.uploadFineF(host, is_f64 != 0) : FC_ERR_INVALID;
 }
 
 int fc_last_error(fc_solver* s, char* buf, int cap) {

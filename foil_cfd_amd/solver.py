@@ -292,6 +292,46 @@ class LBMSolver:
     def set_smagorinsky_cs(self, cs: float):
         check(self._lib.fc_set_smagorinsky_cs(self._h, cs), "fc_set_smagorinsky_cs")
 
+    # refinement -------------------------------------------------------------
+    def init_refinement(self, box, factor: int, fine_flags: np.ndarray) -> None:
+        """LBMSolver::initRefinement: box = (x0, y0, x1, y1) in coarse cells."""
+        b = FcPatchBox(*[int(v) for v in box])
+        ff = np.ascontiguousarray(fine_flags, dtype=np.uint8).reshape(-1)
+        err = C.create_string_buffer(512)
+        rc = self._lib.fc_init_refinement(self._h, C.byref(b), int(factor), _u8(ff), err, 512)
+        check(rc, "fc_init_refinement", err.value.decode())
+
+    def shutdown_refinement(self) -> None:
+        check(self._lib.fc_shutdown_refinement(self._h), "fc_shutdown_refinement")
+
+    def set_refined_flags(self, fine_flags: np.ndarray) -> None:
+        ff = np.ascontiguousarray(fine_flags, dtype=np.uint8).reshape(-1)
+        check(self._lib.fc_set_refined_flags(self._h, _u8(ff)), "fc_set_refined_flags")
+
+    def refinement_info(self) -> dict:
+        r = _abi.FcRefinementInfo()
+        check(self._lib.fc_get_refinement_info(self._h, C.byref(r)), "fc_get_refinement_info")
+        return {"active": bool(r.active), "factor": r.factor,
+                "box": (r.box.x0, r.box.y0, r.box.x1, r.box.y1),
+                "fine_dims": (r.fine_dims.nx, r.fine_dims.ny, r.fine_dims.nz),
+                "fine_tau": r.fine_scaling.tau, "device_bytes": r.device_bytes,
+                "forces_from_fine": bool(r.forces_from_fine)}
+
+    def download_fine_f(self, f64: bool = True) -> np.ndarray:
+        info = self.refinement_info()
+        n = Q * int(np.prod(info["fine_dims"]))
+        out = np.empty(n, dtype=np.float64 if f64 else np.float32)
+        check(self._lib.fc_download_fine_f(self._h, out.ctypes.data_as(C.c_void_p), 1 if f64 else 0),
+              "fc_download_fine_f")
+        return out
+
+    def upload_fine_f(self, f: np.ndarray) -> None:
+        a = np.ascontiguousarray(f)
+        is64 = a.dtype == np.float64
+        a = a if is64 else a.astype(np.float32)
+        check(self._lib.fc_upload_fine_f(self._h, a.ctypes.data_as(C.c_void_p), 1 if is64 else 0),
+              "fc_upload_fine_f")
+
     def set_step_variant(self, variant: str):
         """"auto" (library default), "single" (one cell per lane) or "pair"
         (two x-adjacent cells per lane where the grid allows it)."""

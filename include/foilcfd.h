@@ -248,6 +248,34 @@ int fc_device_fields(fc_solver* s, void** rho, void** u, void** v, void** w, voi
 /* Bytes of device memory owned by the handle. */
 long long fc_device_bytes(fc_solver* s);
 
+/* ---- two-level refinement (src/sim/lbm_refine.cuh, lbm_solver.h:192-258) ---- */
+/* RefinementInfo, lbm_solver.h:105-114 (coarse/fine level only). */
+typedef struct fc_refinement_info {
+    int active;               /* fine level allocated and stepping */
+    int factor;               /* refinement factor m (2..4) */
+    fc_patch_box box;         /* patch in coarse cells */
+    fc_dims fine_dims;        /* (m W, m H, m nz) */
+    fc_scaling fine_scaling;  /* refinedScaling(coarse, m) */
+    double device_bytes;      /* fine populations + kind bytes + coupling band */
+    int forces_from_fine;     /* momentum exchange on the fine grid */
+} fc_refinement_info;
+
+/* LBMSolver::initRefinement (lbm_solver.h:192-194): allocate the fine level
+ * over `box` (coarse cells, full span) at `factor` with its flags (Fluid /
+ * Solid / Interface, 2-cell Interface shell on the x / y faces, as
+ * buildFinePatchFlags makes them), seed it from the current coarse state and
+ * step it m times per coarse step from then on. The box must keep 2 coarse
+ * cells from the x / y domain faces (the reference: 1). */
+int fc_init_refinement(fc_solver* s, const fc_patch_box* box, int factor, const uint8_t* fine_flags,
+                       char* err, int errlen);
+int fc_shutdown_refinement(fc_solver* s);                          /* shutdownRefinement, :199 */
+int fc_set_refined_flags(fc_solver* s, const uint8_t* fine_flags);  /* setRefinedFlags, :205 */
+int fc_get_refinement_info(fc_solver* s, fc_refinement_info* out);  /* refinementInfo, :246 */
+/* Fine-level populations in canonical SoA (no reference counterpart: the
+ * reference exposes the padded fine buffer only to its tests). */
+int fc_download_fine_f(fc_solver* s, void* host, int as_f64);
+int fc_upload_fine_f(fc_solver* s, const void* host, int is_f64);
+
 /* ---- iMEM wall model (lbm_wallmodel.cuh, wall_model.h) -------------------- */
 #define FC_WALL_OFF 0
 #define FC_WALL_NEWTON 1   /* Reichardt inversion by Newton (the reference) */
