@@ -107,6 +107,21 @@ def cpu_baseline(case, seconds=12.0):
                       f"({dt:.1f} s wall)", "seconds": dt}
 
 
+def measured_traffic(storage):
+    """HBM bytes per step-kernel launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE on this workload (copy-kernel calibrated), from the committed
+    profile summary; None when absent."""
+    path = os.path.join(ROOT, "profiles", "stream_collide_wing_traffic.json")
+    key = {"f16c": "StoreF16C", "f16": "StoreF16", "bf16": "StoreBF16", "f32": "StoreF32",
+           "f64": "StoreF64"}[storage]
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else round(rec["traffic_bytes_per_launch_mean"])
+
+
 def copy_calibration(mib=1024, reps=5):
     """A plain device copy of known size (elementwise negate, N bytes read and
     N bytes written per launch) for scaling the HBM traffic counters."""
@@ -195,7 +210,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(r["kernel_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(head),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "kernel": "stepKernel (fused stream-collide)",
             "kernel_ms": round(r["kernel_ms"], 5),
             "algorithmic_bytes_per_cell": BYTES_PER_CELL[head],
