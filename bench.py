@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16c,f32")
-    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair"],
+    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe"],
                     help="step kernel: auto (library default), single or pair")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")

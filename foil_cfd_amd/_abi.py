@@ -24,6 +24,8 @@ FC_STORAGE_BF16 = 2
 FC_STORAGE_F64 = 3
 FC_STORAGE_F16C = 4
 
+FC_MLP_NPARAMS = 1217
+
 STORAGE_NAMES = {"f32": FC_STORAGE_F32, "f16": FC_STORAGE_F16, "bf16": FC_STORAGE_BF16,
                  "f64": FC_STORAGE_F64, "f16c": FC_STORAGE_F16C}
 STORAGE_BYTES = {FC_STORAGE_F32: 4, FC_STORAGE_F16: 2, FC_STORAGE_BF16: 2, FC_STORAGE_F64: 8,
@@ -198,6 +200,16 @@ SIGNATURES = {
     "fc_set_wall_harvest": (C.c_int, [_vp, C.c_int]),
     "fc_wall_harvest": (C.c_int, [_vp, _fp, C.c_int]),
     "fc_set_wall_mlp": (C.c_int, [_vp, C.POINTER(FcMlpParams)]),
+    "fc_mlp_trainer_create": (_vp, [_fp, C.c_longlong, C.c_uint32, C.c_char_p, C.c_int]),
+    "fc_mlp_trainer_destroy": (None, [_vp]),
+    "fc_mlp_trainer_samples": (C.c_longlong, [_vp]),
+    "fc_mlp_trainer_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _fp]),
+    "fc_mlp_trainer_gradient": (C.c_int, [_vp, _fp, _ip, C.c_int, _fp, _fp]),
+    "fc_mlp_trainer_get_params": (C.c_int, [_vp, _fp]),
+    "fc_mlp_trainer_set_params": (C.c_int, [_vp, _fp]),
+    "fc_mlp_trainer_data": (C.c_int, [_vp, _fp, _fp]),
+    "fc_mlp_trainer_export": (C.c_int, [_vp, C.POINTER(FcMlpParams)]),
+    "fc_mlp_infer": (C.c_int, [C.POINTER(FcMlpParams), _fp, C.c_int, _fp]),
     "fc_set_qlibb_enabled": (C.c_int, [_vp, C.c_int]),
     "fc_set_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
     "fc_qlibb_readout": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),

@@ -95,6 +95,12 @@ struct StoreF64 {
 
 // Fast reciprocal / square root for the FP32 compute path (1 ulp hardware
 // ops); the FP64 path keeps correctly rounded IEEE arithmetic.
+// a * b + c with one rounding. The step kernels compile with contraction
+// off and place every fused multiply-add explicitly, so each kernel variant
+// executes the same IEEE operations and they agree bit for bit.
+__device__ inline float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmac(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double frcp(double x) { return 1.0 / x; }
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }

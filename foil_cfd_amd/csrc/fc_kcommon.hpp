@@ -33,6 +33,8 @@ struct KArgs {
     int nwaves;   // nseg * ny * nz
     int nblocks;  // ceil(nwaves / kWavesPerBlock)
     int chunk;    // blocks per XCD after the remap
+    int pipeTasks;  // software-pipelined step: row-segment tasks per XCD slab
+    int pipeWaves;  // software-pipelined step: waves per XCD
     float tau0, lam, pre, uIn;
     int writeMacro;
 };
@@ -71,20 +73,15 @@ struct RowPos {
     bool yOk[3];
 };
 
+// Lane position of row-segment task gw (wave-uniform; segment of 64 x of
+// row gw / nseg). False for lanes past the end of the row.
 template <bool PX, bool PY>
-__device__ inline bool rowPos(const KArgs& a, RowPos& p) {
-    const int hb = blockIdx.x;
-    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
-    if (lb >= a.nblocks) return false;
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int gw = lb * kWavesPerBlock + wv;
-    if (gw >= a.nwaves) return false;
+__device__ inline bool taskPos(const KArgs& a, int gw, RowPos& p) {
     const int row = gw / a.nseg;
     const int seg = gw - row * a.nseg;
     const int z = row / a.ny;
     const int y = row - z * a.ny;
     const int x = seg * kWave + static_cast<int>(threadIdx.x & (kWave - 1));
-    if (x >= a.nx) return false;
     p.x = x;
     p.y = y;
     p.z = z;
@@ -110,7 +107,18 @@ __device__ inline bool rowPos(const KArgs& a, RowPos& p) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) p.R[i][j] = (ys[i] + a.ny * zs[j]) * a.nx;
-    return true;
+    return x < a.nx;
+}
+
+template <bool PX, bool PY>
+__device__ inline bool rowPos(const KArgs& a, RowPos& p) {
+    const int hb = blockIdx.x;
+    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
+    if (lb >= a.nblocks) return false;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int gw = lb * kWavesPerBlock + wv;
+    if (gw >= a.nwaves) return false;
+    return taskPos<PX, PY>(a, gw, p);
 }
 
 // Two-cells-per-lane decomposition: a wave covers 128 cells of a row, lane l
@@ -222,6 +230,7 @@ struct PopBuf {
     unsigned slab;       // bytes per direction slab (ncells * sizeof(T))
     unsigned vx[3];      // per-lane byte offsets of x-1, x, x+1
     unsigned row[3][3];  // wave-uniform byte offsets of the 9 row starts
+    PopBuf() = default;
     __device__ PopBuf(const KArgs& a, const RowPos& p) {
         // 0x00020000: raw (untyped) buffer descriptor word 3 of the gfx9 family
         rs = __builtin_amdgcn_make_buffer_rsrc(a.f, 0, 0x7FFFFFFF, 0x00020000);
@@ -241,6 +250,16 @@ struct PopBuf {
             return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0)));
         else
             return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0)));
+    }
+    // Raw storage bits of a slot (decoded later with S::fromBits), so a load
+    // can stay in flight across other work.
+    __device__ Bits ldBits(int q, int ex, int ey, int ez) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0));
+        else
+            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0));
     }
     __device__ void st(int q, int ex, int ey, int ez, C v) const {
         const int vo = static_cast<int>(vx[ex + 1]);
@@ -310,6 +329,23 @@ __device__ inline typename S::C postH(const KArgs& a, int layout, int q, int x, 
 __device__ inline double waveSum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+
+// Wave64 reductions of FP32 / int partials (the full sum lands in lane 0).
+__device__ inline float waveSumF(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+__device__ inline int waveSumI(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+__device__ inline int waveMaxI(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off, kWave));
     return v;
 }
 

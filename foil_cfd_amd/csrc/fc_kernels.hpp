@@ -17,8 +17,10 @@ namespace fc {
 // Step kernel variants: kStepAuto is the library's default (one cell per
 // lane, the fastest measured on MI355X), kStepSingle forces one cell per
 // lane, kStepPair runs two x-adjacent cells per lane with packed accesses
-// where the grid allows it (even nx, non-periodic x / y, array under 2 GiB).
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2 };
+// where the grid allows it (even nx, non-periodic x / y, array under 2 GiB),
+// kStepPipe runs the software-pipelined persistent kernel (one cell per lane,
+// next task's loads in flight during the collision; arrays under 2 GiB).
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3 };
 
 struct StepArgs {
     float tau0 = 0.6f;
@@ -127,6 +129,25 @@ struct WallStats {
 int launchWallUpdate(const LatticeView& lat, int layout, const WallListView& wl, std::uint16_t* uwx,
                      std::uint16_t* uwy, std::uint16_t* uwz, const WallParams& p, WallStats* stats,
                      float* harvest, void* stream);
+
+// ---- learned u_tau network: training and batched inference (fc_mlp.hip) ---
+// Master parameters: 1217 FP32 values, W1[32][3], b1[32], W2[32][32] (row =
+// output unit), b2[32], w3[32], b3. X: normalised features, SoA [3][n];
+// T: normalised targets [n].
+inline constexpr int kMlpParamCount = 1217;
+inline constexpr int kMlpTile = 32;  // samples per wavefront tile; batches are multiples
+// idx[batch] = counter-based hash of (seed, step, slot) mod n.
+int launchMlpBatch(int* idx, int batch, long long n, unsigned seed, unsigned step, void* stream);
+// Gradient of the batch-mean squared error at P into grad[1217]; loss[0] =
+// sum of squared errors. part: (batch/32) x 1217 floats, lossPart: batch/32.
+int launchMlpGradient(const float* X, long long n, const float* T, const int* idx, int batch, const float* P,
+                      float* part, float* lossPart, float* grad, float* loss, void* stream);
+// One Adam step (step counts from 1) on the master parameters.
+int launchMlpAdam(float* P, float* M, float* V, const float* grad, float lr, float beta1, float beta2, float eps,
+                  int step, void* stream);
+// out[i] = network u_tau of samples (4 floats each: y_s, |u_t|, nu, -), or -1
+// outside the envelope.
+int launchMlpInfer(const MlpParams* m, const float* samples, int count, float* out, void* stream);
 
 // ---- tracer particles (reference src/render/particles.cuh:29-46) -----------
 struct ParticleParams {

@@ -21,6 +21,10 @@
 
 #include "fc_kcommon.hpp"
 
+// No implicit contraction in this file: fused multiply-adds are written out
+// (fmac), so every variant of the step rounds identically.
+#pragma clang fp contract(off)
+
 namespace fc {
 namespace {
 
@@ -30,7 +34,7 @@ template <class C>
 __device__ inline C heqOf(int q, C K, C drho, C rho, C ux, C uy, C uz, C usq) {
     const C w = wOf<C>(q);
     const C cu = C(kCx[q]) * ux + C(kCy[q]) * uy + C(kCz[q]) * uz;
-    return K * w * (drho + rho * (C(3) * cu + C(4.5) * cu * cu - usq));
+    return K * w * fmac(rho, fmac(C(4.5) * cu, cu, fmac(C(3), cu, -usq)), drho);
 }
 
 // ---------------------------------------------------------------------------
@@ -70,13 +74,13 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
     const C invRho = frcp(rho);
     const C jscale = invK * invRho;
     C ux = jx * jscale, uy = jy * jscale, uz = jz * jscale;
-    C u2 = ux * ux + uy * uy + uz * uz;
+    C u2 = fmac(ux, ux, fmac(uy, uy, uz * uz));
     if (u2 > C(0.09)) {  // kMaxSimSpeed^2
         const C sc = C(0.3) * frsqrt(u2);
         ux *= sc;
         uy *= sc;
         uz *= sc;
-        u2 = ux * ux + uy * uy + uz * uz;
+        u2 = fmac(ux, ux, fmac(uy, uy, uz * uz));
     }
     const C usq = C(1.5) * u2;
 
@@ -85,31 +89,32 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
     if (pre > C(0)) {
         const C third = drho * C(1.0 / 3.0);
         const C rux = rho * ux, ruy = rho * uy, ruz = rho * uz;
-        const C nxx = ((s[0] + s[3]) + (s[4] + s[6]) + s[7]) * invK - third - rux * ux;
-        const C nyy = ((s[1] + s[3]) + (s[5] + s[6]) + s[8]) * invK - third - ruy * uy;
-        const C nzz = ((s[2] + s[4]) + (s[5] + s[7]) + s[8]) * invK - third - ruz * uz;
-        const C nxy = (s[3] - s[6]) * invK - rux * uy;
-        const C nxz = (s[4] - s[7]) * invK - rux * uz;
-        const C nyz = (s[5] - s[8]) * invK - ruy * uz;
-        const C q2 = nxx * nxx + nyy * nyy + nzz * nzz + C(2) * (nxy * nxy + nxz * nxz + nyz * nyz);
+        const C nxx = fmac(-rux, ux, fmac((s[0] + s[3]) + (s[4] + s[6]) + s[7], invK, -third));
+        const C nyy = fmac(-ruy, uy, fmac((s[1] + s[3]) + (s[5] + s[6]) + s[8], invK, -third));
+        const C nzz = fmac(-ruz, uz, fmac((s[2] + s[4]) + (s[5] + s[7]) + s[8], invK, -third));
+        const C nxy = fmac(-rux, uy, (s[3] - s[6]) * invK);
+        const C nxz = fmac(-rux, uz, (s[4] - s[7]) * invK);
+        const C nyz = fmac(-ruy, uz, (s[5] - s[8]) * invK);
+        const C off = fmac(nxy, nxy, fmac(nxz, nxz, nyz * nyz));
+        const C q2 = fmac(nxx, nxx, fmac(nyy, nyy, fmac(nzz, nzz, C(2) * off)));
         const C qbar = fsqrt(q2);
-        tauEff = C(0.5) * (tau0 + fsqrt(tau0 * tau0 + pre * qbar * invRho));
+        tauEff = C(0.5) * (tau0 + fsqrt(fmac(tau0, tau0, pre * qbar * invRho)));
     }
     const C omP = frcp(tauEff);
-    const C omM = frcp(C(0.5) + lam * frcp(tauEff - C(0.5)));
+    const C omM = frcp(fmac(lam, frcp(tauEff - C(0.5)), C(0.5)));
     const C P = C(0.5) * (C(1) - omP);
     const C M = C(0.5) * (C(1) - omM);
 
     // Equilibrium constants per weight class (w = 1/18 axis, 1/36 diagonal):
     // sym part A + B cu^2, antisym part Cc cu, pre-multiplied by omega.
-    const C base = drho - rho * usq;
+    const C base = fmac(-rho, usq, drho);
     const C a1 = omP * K * C(1.0 / 18.0) * base;
     const C b1 = omP * K * C(4.5 / 18.0) * rho;
     const C c1 = omM * K * C(3.0 / 18.0) * rho;
     const C a2 = C(0.5) * a1, b2 = C(0.5) * b1, c2 = C(0.5) * c1;
     const C a0 = omP * K * C(1.0 / 3.0) * base;
 
-    H[0] = (C(1) - omP) * H[0] + a0;
+    H[0] = fmac(C(1) - omP, H[0], a0);
     const C cu[9] = {ux, uy, uz, ux + uy, ux + uz, uy + uz, ux - uy, ux - uz, uy - uz};
 #pragma unroll
     for (int p = 0; p < 9; ++p) {
@@ -117,8 +122,8 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
         const C B = p < 3 ? b1 : b2;
         const C Cc = p < 3 ? c1 : c2;
         const C c = cu[p];
-        const C Sp = P * s[p] + (A + B * (c * c));
-        const C Dp = M * d[p] + Cc * c;
+        const C Sp = fmac(P, s[p], fmac(B, c * c, A));
+        const C Dp = fmac(M, d[p], Cc * c);
         H[2 * p + 1] = Sp + Dp;
         H[2 * p + 2] = Sp - Dp;
     }
@@ -132,12 +137,11 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
 // M is the population accessor (PopBuf for arrays under 2 GiB, else PopPtr).
 // ---------------------------------------------------------------------------
 template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
-__device__ inline void stepCell(const KArgs& a, const RowPos& p) {
+__device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop, std::uint8_t k,
+                                typename S::C (&H)[kQ]) {
     using T = typename S::T;
     using C = typename S::C;
-    const M<S> pop(a, p);
     const unsigned c = nb(p, 0, 0, 0);
-    const std::uint8_t k = a.kind[c];
     const std::uint8_t flag = k & kKindFlagMask;
     const C K = C(S::K);
     const int newLayout = FROM_L ? kLayoutS : kLayoutL;
@@ -200,15 +204,7 @@ __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
         return;
     }
 
-    // ---- fluid: pull the 19 incoming populations ---------------------------
-    C H[kQ];
-    staticFor<kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if (FROM_L)
-            H[q] = pop.ld(kOpp[q], -kCx[q], -kCy[q], -kCz[q]);
-        else
-            H[q] = pop.ld(q, 0, 0, 0);
-    });
+    // ---- fluid: H holds the 19 pulled populations --------------------------
     if (k & kKindSpecial) {
         // Per-link source select by the upstream neighbour's flag, as the
         // reference (lbm_core.cu:306-331), mapped onto the AA slots.
@@ -230,7 +226,7 @@ __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
                     H[q] = pop.ld(kOpp[q], -cx, -cy, -cz);
                 if (WM) {
                     const C cuw = C(cx) * uwx + C(cy) * uwy + C(cz) * uwz;
-                    H[q] += K * C(6) * wOf<C>(q) * cuw;
+                    H[q] = fmac(K * C(6) * wOf<C>(q), cuw, H[q]);
                 }
             } else if (nf == kSlipTop || nf == kSlipBottom) {
                 constexpr int m = kMirY[q];
@@ -264,6 +260,30 @@ __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
     }
 }
 
+// Slot of the incoming population q of this cell in the current layout: from
+// L, f*_opp(q) of the upstream cell; from S, the cell's own slot q.
+template <bool FROM_L, int q>
+struct PullSlot {
+    static constexpr int dir = FROM_L ? kOpp[q] : q;
+    static constexpr int ex = FROM_L ? -kCx[q] : 0;
+    static constexpr int ey = FROM_L ? -kCy[q] : 0;
+    static constexpr int ez = FROM_L ? -kCz[q] : 0;
+};
+
+template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
+__device__ inline void stepCell(const KArgs& a, const RowPos& p) {
+    using C = typename S::C;
+    const M<S> pop(a, p);
+    const std::uint8_t k = a.kind[nb(p, 0, 0, 0)];
+    C H[kQ];
+    if ((k & kKindFlagMask) == kFluid)
+        staticFor<kQ>([&](auto qc) {
+            using L = PullSlot<FROM_L, decltype(qc)::value>;
+            H[decltype(qc)::value] = pop.ld(L::dir, L::ex, L::ey, L::ez);
+        });
+    cellBody<S, M, FROM_L, WM, PX, PY>(a, p, pop, k, H);
+}
+
 template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
 __global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
     RowPos p;
@@ -281,6 +301,94 @@ stepKernelBuf(KArgs a) {
     RowPos p;
     if (!rowPos<PX, PY>(a, p)) return;
     stepCell<S, PopBuf, FROM_L, WM, PX, PY>(a, p);
+}
+
+// ---------------------------------------------------------------------------
+// Software-pipelined step. The row-segment tasks are split into eight
+// contiguous slabs, one per XCD (workgroups are dealt to XCDs round-robin);
+// wave i of an XCD takes tasks slab + i, slab + i + W, ... (W waves per XCD),
+// so the XCD sweeps its slab front by front. While a wave collides task j it
+// already has the kind bytes and the 19 population loads of task j+1 in
+// flight (raw storage bits, decoded only when used), so the collision's ALU
+// work overlaps the next task's memory latency instead of waiting for other
+// waves to cover it. Every slot an AA step reads is written by the same cell,
+// so prefetching across cells is race-free.
+// ---------------------------------------------------------------------------
+template <class S>
+struct PipeStage {
+    using Bits = typename S::Bits;
+    RowPos p;
+    PopBuf<S> pop;
+    Bits B[kQ];
+    std::uint8_t k;  // kind byte
+    bool v;          // lane inside the row
+};
+
+// Issues the kind byte and the 19 pull loads of task t for every lane with no
+// branch around them (lanes past the row end and x-1 / x+1 beyond a
+// non-periodic face read a clamped, in-domain slot; only fluid cells use the
+// values). Unconditional issue keeps the number of memory operations between
+// a load and its use fixed, so the waits the compiler inserts let the next
+// task's loads stay in flight.
+template <class S, bool FROM_L, bool PX, bool PY>
+__device__ inline void pipeIssue(const KArgs& a, int t, PipeStage<S>& st) {
+    st.v = taskPos<PX, PY>(a, t, st.p);
+    RowPos q = st.p;
+    if (!st.v) q.x = a.nx - 1;
+    if (!PX) {
+        q.xm = max(q.x - 1, 0);
+        q.xp = min(q.x + 1, a.nx - 1);
+    }
+    st.pop = PopBuf<S>(a, q);
+    st.k = a.kind[nb(q, 0, 0, 0)];
+    staticFor<kQ>([&](auto qc) {
+        using L = PullSlot<FROM_L, decltype(qc)::value>;
+        st.B[decltype(qc)::value] = st.pop.ldBits(L::dir, L::ex, L::ey, L::ez);
+    });
+}
+
+template <class S, bool FROM_L, bool WM, bool PX, bool PY>
+__device__ inline void pipeCompute(const KArgs& a, PipeStage<S>& st) {
+    using C = typename S::C;
+    if (!st.v) return;
+    C H[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(st.B[q]);
+    cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, st.p, st.pop, st.k, H);
+}
+
+template <class S, bool FROM_L, bool WM, bool PX, bool PY>
+__global__ void __launch_bounds__(kBlock) stepKernelPipe(KArgs a) {
+    const int hb = blockIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int W = a.pipeWaves;
+    const int i = (hb >> 3) * kWavesPerBlock + wv;  // wave index within the XCD
+    const int t0 = (hb & (kXcds - 1)) * a.pipeTasks + i;
+    const int tEnd = min((hb & (kXcds - 1)) * a.pipeTasks + a.pipeTasks, a.nwaves);
+    if (i >= W || t0 >= tEnd) return;
+    const int n = (tEnd - t0 + W - 1) / W;  // tasks of this wave: t0 + j W, j < n
+
+    // Two stages with fixed roles (no copies of registers with loads in
+    // flight): while one is collided the other's loads are outstanding.
+    PipeStage<S> A, B;
+    pipeIssue<S, FROM_L, PX, PY>(a, t0, A);
+    int j = 0;
+    for (;;) {
+        if (j + 1 >= n) {
+            pipeCompute<S, FROM_L, WM, PX, PY>(a, A);
+            return;
+        }
+        pipeIssue<S, FROM_L, PX, PY>(a, t0 + (j + 1) * W, B);
+        pipeCompute<S, FROM_L, WM, PX, PY>(a, A);
+        ++j;
+        if (j + 1 >= n) {
+            pipeCompute<S, FROM_L, WM, PX, PY>(a, B);
+            return;
+        }
+        pipeIssue<S, FROM_L, PX, PY>(a, t0 + (j + 1) * W, A);
+        pipeCompute<S, FROM_L, WM, PX, PY>(a, B);
+        ++j;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -627,6 +735,37 @@ struct BufStep {
     };
 };
 
+// The pipelined kernel runs as a persistent grid: per XCD as many workgroups
+// as its CUs hold at once (occupancy of the instantiation, queried once).
+template <class S>
+struct PipeStep {
+    template <bool FROM_L, bool WM, bool PX, bool PY>
+    struct K {
+        static void launch(dim3, dim3 b, hipStream_t st, KArgs a) {
+            static int occ = 0;
+            if (occ == 0) {
+                int o = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &o, reinterpret_cast<const void*>(&stepKernelPipe<S, FROM_L, WM, PX, PY>), kBlock, 0) !=
+                    hipSuccess)
+                    (void)hipGetLastError();
+                occ = o > 0 ? o : 1;
+            }
+            int dev = 0, cus = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+                (void)hipGetLastError();
+                cus = 256;
+            }
+            const int perXcd = (cus + kXcds - 1) / kXcds;
+            a.pipeTasks = (a.nwaves + kXcds - 1) / kXcds;
+            const int blocksPerXcd = min((a.pipeTasks + kWavesPerBlock - 1) / kWavesPerBlock, perXcd * occ);
+            a.pipeWaves = blocksPerXcd * kWavesPerBlock;
+            stepKernelPipe<S, FROM_L, WM, PX, PY><<<dim3(blocksPerXcd * kXcds), b, 0, st>>>(a);
+        }
+    };
+};
+
 // Buffer addressing whenever every population byte offset fits in 31 bits
 // (FP16 / BF16 up to 56 M cells, FP32 up to 28 M); FP64 and larger arrays
 // take the flat-pointer accessor. The two-cells-per-lane kernel needs an even
@@ -651,6 +790,7 @@ int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, int variant
                 }
                 return lastError();
             }
+            if (variant == kStepPipe) return dispatchStepK<PipeStep<S>::template K>(a, lat, layout, st);
             return dispatchStepK<BufStep<S>::template K>(a, lat, layout, st);
         }
     }

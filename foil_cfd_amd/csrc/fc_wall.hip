@@ -52,7 +52,7 @@ __device__ inline std::uint16_t toHalfBits(float v) {
     return b;
 }
 
-template <class S>
+template <class S, bool MLP>
 __global__ void __launch_bounds__(128) wallUpdateKernel(KArgs a, int layout, WallListView wl,
                                                         std::uint16_t* __restrict__ uwx,
                                                         std::uint16_t* __restrict__ uwy,
@@ -60,105 +60,132 @@ __global__ void __launch_bounds__(128) wallUpdateKernel(KArgs a, int layout, Wal
                                                         WallParams p, WallStats* __restrict__ st,
                                                         float* __restrict__ harvest) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= wl.count) return;
+    const bool listed = i < wl.count;
     using C = typename S::C;
     const C invK = C(1) / C(S::K);
-    const long long cell = wl.cellIdx[i];
-    const float nX = wl.nx[i], nY = wl.ny[i], nZ = wl.nz[i];
     const int nxny = a.nx * a.ny;
 
     // Moments at the sampling cell (same density floor as the collision).
-    const long long sc = wl.sampleIdx[i];
-    const int sz = static_cast<int>(sc / nxny);
-    const int srem = static_cast<int>(sc - static_cast<long long>(sz) * nxny);
-    const int sy = srem / a.nx, sx = srem - sy * a.nx;
-    float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    float nX = 0.0f, nY = 0.0f, nZ = 0.0f, ys = 1.0f;
+    if (listed) {
+        nX = wl.nx[i];
+        nY = wl.ny[i];
+        nZ = wl.nz[i];
+        ys = wl.ys[i];
+        const long long sc = wl.sampleIdx[i];
+        const int sz = static_cast<int>(sc / nxny);
+        const int srem = static_cast<int>(sc - static_cast<long long>(sz) * nxny);
+        const int sy = srem / a.nx, sx = srem - sy * a.nx;
+        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        rho = 0.0f;
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-        const float fq = wOf<float>(q) + static_cast<float>(postH<S>(a, layout, q, sx, sy, sz) * invK);
-        rho += fq;
-        jx += kCx[q] * fq;
-        jy += kCy[q] * fq;
-        jz += kCz[q] * fq;
+        for (int q = 0; q < kQ; ++q) {
+            const float fq = wOf<float>(q) + static_cast<float>(postH<S>(a, layout, q, sx, sy, sz) * invK);
+            rho += fq;
+            jx += kCx[q] * fq;
+            jy += kCy[q] * fq;
+            jz += kCz[q] * fq;
+        }
+        rho = fmaxf(rho, 0.05f);
+        const float inv = 1.0f / rho;
+        ux = jx * inv;
+        uy = jy * inv;
+        uz = jz * inv;
     }
-    rho = fmaxf(rho, 0.05f);
-    const float inv = 1.0f / rho;
-    const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
     const float un = ux * nX + uy * nY + uz * nZ;
     const float tx = ux - un * nX, ty = uy - un * nY, tz = uz - un * nZ;
     const float ut = sqrtf(tx * tx + ty * ty + tz * tz);
+    const bool active = listed && ut >= p.utCutoff;
 
-    if (ut < p.utCutoff) {  // stagnation / separation: plain bounce-back
-        wl.uTau[i] = 0.0f;
-        uwx[cell] = uwy[cell] = uwz[cell] = 0;
-        if (harvest) {
-            float* h = harvest + 4 * static_cast<long long>(i);
-            h[0] = wl.ys[i];
-            h[1] = ut;
-            h[2] = p.nuLat;
-            h[3] = -1.0f;  // no sample
-        }
-        return;
-    }
-    const float itx = tx / ut, ity = ty / ut, itz = tz / ut;
-
-    const float prev = wl.uTau[i];
-    const float ys = wl.ys[i];
-    float solved;
+    // Learned u_tau: the whole wavefront evaluates the network together
+    // (matrix-core tiles); lanes outside the envelope keep the Newton solve.
     bool usedMlp = false;
-    if (p.mode != kWallNewton && p.mlp && mlpInEnvelope(*p.mlp, ys, ut, p.nuLat)) {
-        solved = mlpUTau(*p.mlp, ys, ut, p.nuLat);
-        usedMlp = true;
-    } else {
-        solved = solveUTau(ut, ys, p.nuLat, prev);
+    float net = 0.0f;
+    if constexpr (MLP) {
+        usedMlp = active && mlpInEnvelope(*p.mlp, ys, ut, p.nuLat);
+        net = mlpWaveUTau(*p.mlp, usedMlp, ys, ut, p.nuLat);
     }
-    if (harvest) {
-        float* h = harvest + 4 * static_cast<long long>(i);
-        h[0] = ys;
-        h[1] = ut;
-        h[2] = p.nuLat;
-        h[3] = usedMlp ? solveUTau(ut, ys, p.nuLat, solved) : solved;
-    }
-    const float utau = prev > 0.0f ? prev + p.emaAlpha * (solved - prev) : solved;
-    wl.uTau[i] = utau;
+    // Per-lane statistics partials; the wave reduces them before one atomic each.
+    float ypSum = 0.0f;
+    int ypMaxBits = 0, nActive = 0, nClamped = 0, nMlp = 0;
+    if (listed) {
+        const long long cell = wl.cellIdx[i];
+        if (!active) {  // stagnation / separation: plain bounce-back
+            wl.uTau[i] = 0.0f;
+            uwx[cell] = uwy[cell] = uwz[cell] = 0;
+            if (harvest) {
+                float* h = harvest + 4 * static_cast<long long>(i);
+                h[0] = ys;
+                h[1] = ut;
+                h[2] = p.nuLat;
+                h[3] = -1.0f;  // no sample
+            }
+        } else {
+            const float itx = tx / ut, ity = ty / ut, itz = tz / ut;
+            const float prev = wl.uTau[i];
+            const float solved = usedMlp ? net : solveUTau(ut, ys, p.nuLat, prev);
+            if (harvest) {  // the Newton solution is the training target and the network's judge
+                float* h = harvest + 4 * static_cast<long long>(i);
+                h[0] = ys;
+                h[1] = ut;
+                h[2] = p.nuLat;
+                h[3] = usedMlp ? solveUTau(ut, ys, p.nuLat, solved) : solved;
+            }
+            const float utau = prev > 0.0f ? prev + p.emaAlpha * (solved - prev) : solved;
+            wl.uTau[i] = utau;
 
-    // Linear iMEM balance over the masked (solid) pull links: F_t = M0 - u_w D.
-    float m0 = 0.0f, dcoef = 0.0f;
-    const std::uint32_t mask = wl.mask[i];
-    const int cz = static_cast<int>(cell / nxny);
-    const int crem = static_cast<int>(cell - static_cast<long long>(cz) * nxny);
-    const int cy = crem / a.nx, cx = crem - cy * a.nx;
+            // Linear iMEM balance over the masked (solid) pull links: F_t = M0 - u_w D.
+            float m0 = 0.0f, dcoef = 0.0f;
+            const std::uint32_t mask = wl.mask[i];
+            const int cz = static_cast<int>(cell / nxny);
+            const int crem = static_cast<int>(cell - static_cast<long long>(cz) * nxny);
+            const int cy = crem / a.nx, cx = crem - cy * a.nx;
 #pragma unroll
-    for (int q = 1; q < kQ; ++q) {
-        if (!(mask & (1u << q))) continue;
-        const int o = kOpp[q];  // outgoing direction into the wall
-        const float ct = -(kCx[q] * itx + kCy[q] * ity + kCz[q] * itz);
-        const float fo = wOf<float>(o) + static_cast<float>(postH<S>(a, layout, o, cx, cy, cz) * invK);
-        m0 += 2.0f * fo * ct;
-        dcoef += 6.0f * wOf<float>(q) * ct * ct;
-    }
-    const float maxN = fmaxf(fabsf(nX), fmaxf(fabsf(nY), fabsf(nZ)));
-    const float aw = 1.0f / fmaxf(maxN, 0.1f);
-    float uw = 0.0f;
-    if (dcoef > 1e-6f) {
-        const float target = rho * utau * utau * aw;
-        const float sgn = m0 >= 0.0f ? 1.0f : -1.0f;
-        uw = (m0 - sgn * target) / dcoef;
-    }
-    const float ypS = ys * utau / p.nuLat;
-    const float fade = fminf(fmaxf((ypS - kFadeY0) / (kFadeY1 - kFadeY0), 0.0f), 1.0f);
-    uw *= fade;
-    const float cap = fminf(0.5f * ut, 0.5f * kMaxSimSpeed);
-    const float uwc = fminf(fmaxf(uw, -cap), cap);
-    uwx[cell] = toHalfBits(uwc * itx);
-    uwy[cell] = toHalfBits(uwc * ity);
-    uwz[cell] = toHalfBits(uwc * itz);
+            for (int q = 1; q < kQ; ++q) {
+                if (!(mask & (1u << q))) continue;
+                const int o = kOpp[q];  // outgoing direction into the wall
+                const float ct = -(kCx[q] * itx + kCy[q] * ity + kCz[q] * itz);
+                const float fo = wOf<float>(o) + static_cast<float>(postH<S>(a, layout, o, cx, cy, cz) * invK);
+                m0 += 2.0f * fo * ct;
+                dcoef += 6.0f * wOf<float>(q) * ct * ct;
+            }
+            const float maxN = fmaxf(fabsf(nX), fmaxf(fabsf(nY), fabsf(nZ)));
+            const float aw = 1.0f / fmaxf(maxN, 0.1f);
+            float uw = 0.0f;
+            if (dcoef > 1e-6f) {
+                const float target = rho * utau * utau * aw;
+                const float sgn = m0 >= 0.0f ? 1.0f : -1.0f;
+                uw = (m0 - sgn * target) / dcoef;
+            }
+            const float ypS = ys * utau / p.nuLat;
+            const float fade = fminf(fmaxf((ypS - kFadeY0) / (kFadeY1 - kFadeY0), 0.0f), 1.0f);
+            uw *= fade;
+            const float cap = fminf(0.5f * ut, 0.5f * kMaxSimSpeed);
+            const float uwc = fminf(fmaxf(uw, -cap), cap);
+            uwx[cell] = toHalfBits(uwc * itx);
+            uwy[cell] = toHalfBits(uwc * ity);
+            uwz[cell] = toHalfBits(uwc * itz);
 
-    atomicAdd(&st->sumYplus, ypS);
-    atomicMax(&st->maxYplusBits, __float_as_int(fmaxf(ypS, 0.0f)));
-    atomicAdd(&st->activeCells, 1);
-    if (uwc != uw) atomicAdd(&st->clampedCells, 1);
-    if (usedMlp) atomicAdd(&st->mlpCells, 1);
+            ypSum = ypS;
+            ypMaxBits = __float_as_int(fmaxf(ypS, 0.0f));  // non-negative floats order as ints
+            nActive = 1;
+            nClamped = uwc != uw ? 1 : 0;
+            nMlp = usedMlp ? 1 : 0;
+        }
+    }
+    ypSum = waveSumF(ypSum);
+    ypMaxBits = waveMaxI(ypMaxBits);
+    nActive = waveSumI(nActive);
+    nClamped = waveSumI(nClamped);
+    nMlp = waveSumI(nMlp);
+    if ((threadIdx.x & (kWave - 1)) == 0 && nActive > 0) {
+        atomicAdd(&st->sumYplus, ypSum);
+        atomicMax(&st->maxYplusBits, ypMaxBits);
+        atomicAdd(&st->activeCells, nActive);
+        if (nClamped) atomicAdd(&st->clampedCells, nClamped);
+        if (nMlp) atomicAdd(&st->mlpCells, nMlp);
+    }
 }
 
 __global__ void zeroStatsKernel(WallStats* st) {
@@ -172,9 +199,13 @@ struct WallOp {
                    float* harvest, hipStream_t s) {
         KArgs a = makeArgs(lat);
         zeroStatsKernel<<<1, 64, 0, s>>>(st);
-        if (wl.count > 0)
-            wallUpdateKernel<S><<<(wl.count + 127) / 128, 128, 0, s>>>(a, layout, wl, uwx, uwy, uwz, p, st,
-                                                                        harvest);
+        if (wl.count > 0) {
+            const dim3 grid((wl.count + 127) / 128);
+            if (p.mode == kWallMlp && p.mlp)
+                wallUpdateKernel<S, true><<<grid, 128, 0, s>>>(a, layout, wl, uwx, uwy, uwz, p, st, harvest);
+            else
+                wallUpdateKernel<S, false><<<grid, 128, 0, s>>>(a, layout, wl, uwx, uwy, uwz, p, st, harvest);
+        }
         return static_cast<int>(hipGetLastError());
     }
 };

@@ -333,9 +333,10 @@ class LBMSolver:
               "fc_upload_fine_f")
 
     def set_step_variant(self, variant: str):
-        """"auto" (library default), "single" (one cell per lane) or "pair"
-        (two x-adjacent cells per lane where the grid allows it)."""
-        v = {"auto": 0, "single": 1, "pair": 2}[variant]
+        """"auto" (library default), "single" (one cell per lane), "pair"
+        (two x-adjacent cells per lane where the grid allows it) or "pipe"
+        (persistent, software-pipelined one-cell-per-lane kernel)."""
+        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):

@@ -198,8 +198,11 @@ int fc_set_smagorinsky_cs(fc_solver* s, float cs);
  * cell per lane (the fastest measured on MI355X); FC_STEP_SINGLE forces it;
  * FC_STEP_PAIR runs two x-adjacent cells per lane with packed population
  * accesses where the grid allows it (even nx, x / y not periodic, population
- * array under 2 GiB). All give the same cell update. */
-enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2 };
+ * array under 2 GiB); FC_STEP_PIPE runs a persistent, software-pipelined
+ * kernel that keeps the next row segment's population loads in flight while
+ * it collides the current one (population array under 2 GiB). All give the
+ * same cell update. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2, FC_STEP_PIPE = 3 };
 int fc_set_step_variant(fc_solver* s, int variant);
 
 /* ---- stepping ------------------------------------------------------------ */
@@ -333,6 +336,45 @@ int fc_set_wall_harvest(fc_solver* s, int enabled);
 int fc_wall_harvest(fc_solver* s, float* out, int cap);
 /* Load (or with NULL, drop) the learned u_tau network used by FC_WALL_MLP. */
 int fc_set_wall_mlp(fc_solver* s, const fc_mlp_params* params);
+
+/* ---- learned u_tau network: GPU training (no reference counterpart) ------- */
+/* The trainer holds harvested samples on the GPU and fits the network there
+ * with the library's own forward / backward / Adam kernels (BF16 matrix-core
+ * tiles, FP32 accumulate and FP32 master parameters). Master parameters are
+ * FC_MLP_NPARAMS floats: W1[32][3], b1[32], W2[32][32] (row = output unit),
+ * b2[32], w3[32], b3. Inputs are the normalised (ln y_s, ln |u_t|, ln nu),
+ * the target the normalised ln(u_tau / |u_t|). */
+#define FC_MLP_NPARAMS 1217
+typedef struct fc_mlp_trainer fc_mlp_trainer;
+/* samples: count x 4 floats (y_s, |u_t|, nu, u_tau) as fc_wall_harvest writes
+ * them; entries with a non-positive or non-finite field are skipped. Sets the
+ * normalisation and the envelope from the kept samples and initialises the
+ * weights (Glorot-uniform, deterministic in seed). NULL on error (text in err). */
+fc_mlp_trainer* fc_mlp_trainer_create(const float* samples, long long count, uint32_t seed, char* err,
+                                      int errlen);
+void fc_mlp_trainer_destroy(fc_mlp_trainer* t);
+long long fc_mlp_trainer_samples(fc_mlp_trainer* t);
+/* steps Adam steps on minibatches of `batch` samples (a multiple of 32) drawn
+ * on the GPU; losses (may be NULL, steps entries) receive each step's batch
+ * mean squared error. The step counter persists across calls. */
+int fc_mlp_trainer_train(fc_mlp_trainer* t, int steps, int batch, float lr, float* losses);
+/* Gradient of the batch-mean squared error over samples idx[0..batch) (batch
+ * a multiple of 32) at params (NULL = the current master parameters) into
+ * grad[FC_MLP_NPARAMS]; *sse = sum of squared errors. */
+int fc_mlp_trainer_gradient(fc_mlp_trainer* t, const float* params, const int* idx, int batch, float* grad,
+                            float* sse);
+int fc_mlp_trainer_get_params(fc_mlp_trainer* t, float* params);
+/* Replaces the master parameters and restarts Adam (moments and counter). */
+int fc_mlp_trainer_set_params(fc_mlp_trainer* t, const float* params);
+/* The normalised training inputs (SoA, x[3][n]) and targets (t[n]); NULL skips. */
+int fc_mlp_trainer_data(fc_mlp_trainer* t, float* x, float* target);
+/* The network for fc_set_wall_mlp: BF16 (round to nearest even) weights,
+ * FP32 biases, normalisation and envelope. */
+int fc_mlp_trainer_export(fc_mlp_trainer* t, fc_mlp_params* out);
+/* Batched inference on the GPU with the wall model's own fused routine:
+ * out[i] = network u_tau of samples[4i..4i+2] (y_s, |u_t|, nu), or -1
+ * outside the envelope. */
+int fc_mlp_infer(const fc_mlp_params* params, const float* samples, int count, float* out);
 
 /* ---- interpolated bounce-back (q-LIBB), lbm_solver.h:311-346 -------------- */
 /* setQLIBBEnabled, :315 (off frees the link field; every link reverts to half-way). */

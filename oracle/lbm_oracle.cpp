@@ -429,9 +429,9 @@ float reichardtDuPlus(float yp) {
                       + (yp / 33.0f) * std::exp(-yp / 3.0f));
 }
 
-float solveUTau(float ut, float ys, float nu, float seed) {
+float solveUTauIters(float ut, float ys, float nu, float seed, int iters) {
     float utau = (seed > 0.0f) ? seed : std::max(0.04f * ut, std::sqrt(nu * ut / ys));
-    for (int it = 0; it < 6; ++it) {
+    for (int it = 0; it < iters; ++it) {
         const float yp = std::max(ys * utau / nu, 1e-8f);
         const float up = reichardtUPlus(yp);
         const float g = utau * up - ut;
@@ -441,6 +441,8 @@ float solveUTau(float ut, float ys, float nu, float seed) {
     }
     return utau;
 }
+
+float solveUTau(float ut, float ys, float nu, float seed) { return solveUTauIters(ut, ys, nu, seed, 6); }
 
 // IEEE binary32 -> binary16, round to nearest even (= __float2half_rn).
 uint16_t floatToHalfBits(float f) {
@@ -843,6 +845,106 @@ void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* f
     if (fs != fA) std::memcpy(fA, fs, sizeof(T) * (size_t)kQ * (size_t)fg.ncells());
 }
 
+
+// ---------------------------------------------------------------------------
+// Learned u_tau network (no reference counterpart; the reference solves
+// Reichardt's law by Newton only, lbm_wallmodel.cu:55-73). A restatement of
+// the network the HIP kernels train and evaluate, written from the
+// mathematics: 3 -> 32 -> 32 -> 1 with tanh hidden units, mean squared error,
+// reverse-mode gradient, Adam (Kingma & Ba 2015). Packed master parameters
+// (1217 floats): W1[32][3] @0, b1[32] @96, W2[32][32] @128 (row = output
+// unit), b2[32] @1152, w3[32] @1184, b3 @1216.
+// With emulate = true the operands the GPU feeds to BF16 matrix-core tiles
+// are rounded to BF16 here too (weights; first-layer activations into the
+// hidden layer; second-layer deltas and first-layer activations into the
+// weight-gradient product; deltas and weights into the input-delta product),
+// everything else in FP64. With emulate = false the whole network is FP64,
+// and the gradient is the exact derivative of the computed loss.
+// Parity pins: gradient vs central finite differences (FP64, emulate =
+// false); forward vs a closed form with one active path; the first Adam step
+// equals lr * sign(g).
+constexpr int kMlpH = 32;
+constexpr int kMlpNP = 1217;
+constexpr int kOW1 = 0, kOB1 = 96, kOW2 = 128, kOB2 = 1152, kOW3 = 1184, kOB3 = 1216;
+
+float bf16RoundF(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even at bit 16
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+inline double bfe(double v, bool emulate) { return emulate ? (double)bf16RoundF((float)v) : v; }
+
+struct MlpTape {
+    double h1[kMlpH], h2[kMlpH], y;
+};
+
+void mlpForward(const float* P, bool emu, const double x[3], MlpTape& t) {
+    for (int k = 0; k < kMlpH; ++k) {
+        double pre = P[kOB1 + k];
+        for (int c = 0; c < 3; ++c) pre += bfe(P[kOW1 + 3 * k + c], emu) * x[c];
+        t.h1[k] = std::tanh(pre);
+    }
+    for (int j = 0; j < kMlpH; ++j) {
+        double z = P[kOB2 + j];
+        for (int k = 0; k < kMlpH; ++k) z += bfe(P[kOW2 + j * kMlpH + k], emu) * bfe(t.h1[k], emu);
+        t.h2[j] = std::tanh(z);
+    }
+    double y = P[kOB3];
+    for (int j = 0; j < kMlpH; ++j) y += bfe(P[kOW3 + j], emu) * t.h2[j];
+    t.y = y;
+}
+
+// grad = d/dP of (1/batch) sum_i (y_i - T_i)^2 over the samples idx[0..batch);
+// returns the sum of squared errors. X is SoA [3][n].
+double mlpGradient(const float* P, bool emu, const float* X, long long n, const float* T, const int* idx,
+                   int batch, double* grad) {
+    std::fill(grad, grad + kMlpNP, 0.0);
+    double sse = 0.0;
+    for (int b = 0; b < batch; ++b) {
+        const long long g = idx[b];
+        const double x[3] = {X[g], X[n + g], X[2 * n + g]};
+        MlpTape t;
+        mlpForward(P, emu, x, t);
+        const double e = t.y - T[g];
+        sse += e * e;
+        const double dy = 2.0 * e / batch;
+        grad[kOB3] += dy;
+        double dz2[kMlpH];
+        for (int j = 0; j < kMlpH; ++j) {
+            grad[kOW3 + j] += dy * t.h2[j];
+            dz2[j] = dy * bfe(P[kOW3 + j], emu) * (1.0 - t.h2[j] * t.h2[j]);
+            grad[kOB2 + j] += dz2[j];
+        }
+        for (int j = 0; j < kMlpH; ++j)
+            for (int k = 0; k < kMlpH; ++k) grad[kOW2 + j * kMlpH + k] += bfe(dz2[j], emu) * bfe(t.h1[k], emu);
+        for (int k = 0; k < kMlpH; ++k) {
+            double dh = 0.0;
+            for (int j = 0; j < kMlpH; ++j) dh += bfe(dz2[j], emu) * bfe(P[kOW2 + j * kMlpH + k], emu);
+            const double dz1 = dh * (1.0 - t.h1[k] * t.h1[k]);
+            grad[kOB1 + k] += dz1;
+            for (int c = 0; c < 3; ++c) grad[kOW1 + 3 * k + c] += dz1 * x[c];
+        }
+    }
+    return sse;
+}
+
+void mlpAdam(float* P, float* M, float* V, const double* grad, double lr, double b1, double b2, double eps,
+             int step) {
+    const double c1 = 1.0 - std::pow(b1, step), c2 = 1.0 - std::pow(b2, step);
+    for (int p = 0; p < kMlpNP; ++p) {
+        const double m = b1 * M[p] + (1.0 - b1) * grad[p];
+        const double v = b2 * V[p] + (1.0 - b2) * grad[p] * grad[p];
+        M[p] = (float)m;
+        V[p] = (float)v;
+        P[p] = (float)(P[p] - lr * (m / c1) / (std::sqrt(v / c2) + eps));
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1191,6 +1293,31 @@ void orc_particles_advect(float* pos4, int count, const float* u, const float* v
         }
         pos4[4 * i] = p.x; pos4[4 * i + 1] = p.y; pos4[4 * i + 2] = p.z; pos4[4 * i + 3] = age;
     }
+}
+
+float orc_solve_utau_iters(float ut, float ys, float nu, float seed, int iters) {
+    return solveUTauIters(ut, ys, nu, seed, iters);
+}
+
+float orc_bf16_round(float x) { return bf16RoundF(x); }
+
+void orc_mlp_forward(const float* P, int emulate, const float* X, long long n, double* y) {
+    for (long long i = 0; i < n; ++i) {
+        const double x[3] = {X[i], X[n + i], X[2 * n + i]};
+        MlpTape t;
+        mlpForward(P, emulate != 0, x, t);
+        y[i] = t.y;
+    }
+}
+
+double orc_mlp_gradient(const float* P, int emulate, const float* X, long long n, const float* T, const int* idx,
+                        int batch, double* grad) {
+    return mlpGradient(P, emulate != 0, X, n, T, idx, batch, grad);
+}
+
+void orc_mlp_adam(float* P, float* M, float* V, const double* grad, double lr, double b1, double b2, double eps,
+                  int step) {
+    mlpAdam(P, M, V, grad, lr, b1, b2, eps, step);
 }
 
 }  // extern "C"

@@ -115,6 +115,15 @@ def lib():
                                            C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint32]
         L.orc_moments_f32.argtypes = [_fp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
         L.orc_moments_f64.argtypes = [_dp, C.c_int, C.c_int, C.c_int] + [_dp] * 4
+        L.orc_solve_utau_iters.restype = C.c_float
+        L.orc_solve_utau_iters.argtypes = [C.c_float] * 4 + [C.c_int]
+        L.orc_bf16_round.restype = C.c_float
+        L.orc_bf16_round.argtypes = [C.c_float]
+        _ip = C.POINTER(C.c_int)
+        L.orc_mlp_forward.argtypes = [_fp, C.c_int, _fp, C.c_longlong, _dp]
+        L.orc_mlp_gradient.restype = C.c_double
+        L.orc_mlp_gradient.argtypes = [_fp, C.c_int, _fp, C.c_longlong, _fp, _ip, C.c_int, _dp]
+        L.orc_mlp_adam.argtypes = [_fp, _fp, _fp, _dp] + [C.c_double] * 4 + [C.c_int]
         _lib = L
     return _lib
 
@@ -304,6 +313,55 @@ def reichardt_uplus(yp):
 
 def solve_utau(ut, ys, nu, seed=0.0):
     return lib().orc_solve_utau(ut, ys, nu, seed)
+
+
+def solve_utau_iters(ut, ys, nu, seed=0.0, iters=6):
+    """Newton on Reichardt's law with a given iteration count (6 = solve_utau)."""
+    return lib().orc_solve_utau_iters(ut, ys, nu, seed, iters)
+
+
+# ---- learned u_tau network (restatement of the GPU network) ------------------
+MLP_NPARAMS = 1217
+MLP_OFFSETS = {"w1": (0, (32, 3)), "b1": (96, (32,)), "w2": (128, (32, 32)),
+               "b2": (1152, (32,)), "w3": (1184, (32,)), "b3": (1216, ())}
+
+
+def bf16_round(x):
+    return lib().orc_bf16_round(x)
+
+
+def mlp_forward(params, x, emulate=True):
+    """Network output for normalised features x (shape (n, 3)); FP64 (with
+    the GPU's BF16 operand roundings when emulate)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    assert p.size == MLP_NPARAMS
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float32).T)
+    n = xs.shape[1]
+    y = np.zeros(n, dtype=np.float64)
+    lib().orc_mlp_forward(_p(p, _fp), int(emulate), _p(xs, _fp), n, _p(y, _dp))
+    return y
+
+
+def mlp_gradient(params, x, t, idx, emulate=True):
+    """(gradient of the batch-mean squared error, sum of squared errors)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float32).T)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    ii = np.ascontiguousarray(idx, dtype=np.int32)
+    n = xs.shape[1]
+    assert tt.size == n and ii.min() >= 0 and ii.max() < n
+    g = np.zeros(MLP_NPARAMS, dtype=np.float64)
+    sse = lib().orc_mlp_gradient(_p(p, _fp), int(emulate), _p(xs, _fp), n, _p(tt, _fp),
+                                 ii.ctypes.data_as(C.POINTER(C.c_int)), ii.size, _p(g, _dp))
+    return g, sse
+
+
+def mlp_adam(params, m, v, grad, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays (params, m, v)."""
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.size == MLP_NPARAMS
+    lib().orc_mlp_adam(_p(params, _fp), _p(m, _fp), _p(v, _fp), _p(g, _dp), lr, beta1, beta2, eps, step)
 
 
 def float_to_half_bits(x):
