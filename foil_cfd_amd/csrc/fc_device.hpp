@@ -58,28 +58,31 @@ struct StoreBF16 {
 // FP16C: a 16-bit float of the unscaled deviation with 1 sign, 4 exponent
 // and 11 mantissa bits (one mantissa bit more than IEEE FP16, whose fifth
 // exponent bit the bounded deviations never need). Normals cover
-// [2^-14, 2), a subnormal quantum of 2^-25 below; values at or above 2,
-// infinities and NaN saturate to the largest code. Decoding places the 15
-// magnitude bits under an FP32 exponent/mantissa and rescales by 2^112 (FP32
-// denormals are preserved in these kernels, so subnormal codes decode
-// exactly); encoding rescales by 2^-112 and rounds the FP32 bits to nearest
-// even at bit 12.
+// [2^-14, 2), a subnormal quantum of 2^-25 below; values at or above the
+// largest code (1 + 2047/2048) and infinities saturate to it, NaN encodes as
+// the largest negative code. Decoding sign-extends the 16 bits (a signed
+// 16-bit load), places the 15 magnitude bits under an FP32 exponent/mantissa
+// and rescales by 2^112 (3 VALU; FP32 denormals are preserved in these
+// kernels, so subnormal codes decode exactly). Encoding rescales by 2^-112,
+// clamps, and rounds the FP32 bits to nearest even at bit 12 with the sign
+// bit carried along (bits 27-30 are zero after the clamp, so no carry reaches
+// it), then packs (7 VALU).
 struct StoreF16C {
     using T = unsigned short;
     using C = float;
     using Bits = unsigned short;
     static constexpr float K = 1.0f;
     __device__ static inline C fromBits(Bits b) {
-        const unsigned x = b;
-        const unsigned u = ((x & 0x7FFFu) << 12) | ((x & 0x8000u) << 16);
+        const unsigned sx = static_cast<unsigned>(static_cast<int>(static_cast<short>(b)));
+        const unsigned u = (sx << 12) & 0x87FFF000u;
         return __builtin_bit_cast(float, u) * 0x1p112f;
     }
     __device__ static inline Bits toBits(C v) {
-        const unsigned bits = __builtin_bit_cast(unsigned, v * 0x1p-112f);
-        const unsigned mag = bits & 0x7FFFFFFFu;
-        unsigned r = (mag + 0x7FFu + ((mag >> 12) & 1u)) >> 12;
-        r = r < 0x7FFFu ? r : 0x7FFFu;
-        return static_cast<Bits>(r | ((bits >> 16) & 0x8000u));
+        constexpr float lim = 0x1.FFEp0f * 0x1p-112f;  // code 0x7FFF
+        const float t = __builtin_fminf(__builtin_fmaxf(v * 0x1p-112f, -lim), lim);
+        const unsigned b = __builtin_bit_cast(unsigned, t);
+        const unsigned r = b + 0x7FFu + ((b >> 12) & 1u);
+        return static_cast<Bits>(((r >> 12) & 0x7FFFu) | ((r >> 16) & 0x8000u));
     }
     __device__ static inline C load(const T* p) { return fromBits(*p); }
     __device__ static inline void store(T* p, C v) { *p = toBits(v); }

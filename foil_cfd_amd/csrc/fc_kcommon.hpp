@@ -33,11 +33,33 @@ struct KArgs {
     int nwaves;   // nseg * ny * nz
     int nblocks;  // ceil(nwaves / kWavesPerBlock)
     int chunk;    // blocks per XCD after the remap
+    // Reciprocal multipliers for the task -> (row, z) divisions: n / d as
+    // (n * m) >> 40 with m = ceil(2^40 / d), exact for n < 2^24, d < 2^16
+    // (0 when the grid exceeds those ranges: plain division).
+    unsigned long long segMagic, nyMagic;
     int pipeTasks;  // software-pipelined step: row-segment tasks per XCD slab
     int pipeWaves;  // software-pipelined step: waves per XCD
     float tau0, lam, pre, uIn;
     int writeMacro;
 };
+
+inline unsigned long long divMagic(long long d) {
+    return d > 0 && d < (1 << 16) ? ((1ull << 40) + static_cast<unsigned long long>(d) - 1) /
+                                        static_cast<unsigned long long>(d)
+                                  : 0ull;
+}
+
+inline void setDivMagic(KArgs& a) {
+    const bool ok = a.nwaves < (1 << 24);
+    a.segMagic = ok ? divMagic(a.nseg) : 0ull;
+    a.nyMagic = ok ? divMagic(a.ny) : 0ull;
+    if (!a.segMagic || !a.nyMagic) a.segMagic = a.nyMagic = 0ull;
+}
+
+// n / d for a wave-uniform task index (reciprocal multiply when available).
+__device__ inline int taskDiv(int n, int d, unsigned long long m) {
+    return m ? static_cast<int>((static_cast<unsigned long long>(n) * m) >> 40) : n / d;
+}
 
 KArgs makeArgs(const LatticeView& lat) {
     KArgs a{};
@@ -59,6 +81,7 @@ KArgs makeArgs(const LatticeView& lat) {
     a.nwaves = a.nseg * a.ny * a.nz;
     a.nblocks = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
     a.chunk = (a.nblocks + kXcds - 1) / kXcds;
+    setDivMagic(a);
     return a;
 }
 
@@ -77,9 +100,9 @@ struct RowPos {
 // row gw / nseg). False for lanes past the end of the row.
 template <bool PX, bool PY>
 __device__ inline bool taskPos(const KArgs& a, int gw, RowPos& p) {
-    const int row = gw / a.nseg;
+    const int row = taskDiv(gw, a.nseg, a.segMagic);
     const int seg = gw - row * a.nseg;
-    const int z = row / a.ny;
+    const int z = taskDiv(row, a.ny, a.nyMagic);
     const int y = row - z * a.ny;
     const int x = seg * kWave + static_cast<int>(threadIdx.x & (kWave - 1));
     p.x = x;
@@ -130,6 +153,7 @@ inline KArgs pairArgs(const KArgs& a) {
     b.nwaves = b.nseg * a.ny * a.nz;
     b.nblocks = (b.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
     b.chunk = (b.nblocks + kXcds - 1) / kXcds;
+    setDivMagic(b);
     return b;
 }
 
@@ -140,9 +164,9 @@ __device__ inline bool rowPosPair(const KArgs& a, RowPos& p) {
     const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int gw = lb * kWavesPerBlock + wv;
     if (gw >= a.nwaves) return false;
-    const int row = gw / a.nseg;
+    const int row = taskDiv(gw, a.nseg, a.segMagic);
     const int seg = gw - row * a.nseg;
-    const int z = row / a.ny;
+    const int z = taskDiv(row, a.ny, a.nyMagic);
     const int y = row - z * a.ny;
     const int x = seg * (2 * kWave) + 2 * static_cast<int>(threadIdx.x & (kWave - 1));
     if (x >= a.nx) return false;

@@ -317,44 +317,55 @@ stepKernelBuf(KArgs a) {
 template <class S>
 struct PipeStage {
     using Bits = typename S::Bits;
-    RowPos p;
-    PopBuf<S> pop;
-    Bits B[kQ];
+    int t;           // task (wave-uniform); row geometry is recomputed from it
+    Bits B[kQ];      // raw pulled populations
     std::uint8_t k;  // kind byte
-    bool v;          // lane inside the row
 };
 
-// Issues the kind byte and the 19 pull loads of task t for every lane with no
-// branch around them (lanes past the row end and x-1 / x+1 beyond a
-// non-periodic face read a clamped, in-domain slot; only fluid cells use the
-// values). Unconditional issue keeps the number of memory operations between
-// a load and its use fixed, so the waits the compiler inserts let the next
-// task's loads stay in flight.
-template <class S, bool FROM_L, bool PX, bool PY>
-__device__ inline void pipeIssue(const KArgs& a, int t, PipeStage<S>& st) {
-    st.v = taskPos<PX, PY>(a, t, st.p);
-    RowPos q = st.p;
-    if (!st.v) q.x = a.nx - 1;
+// Row geometry of task t for the loads: lanes past the row end and x-1 / x+1
+// beyond a non-periodic face are clamped onto an in-domain slot (only fluid
+// cells use what they read).
+template <bool PX, bool PY>
+__device__ inline void pipeLoadPos(const KArgs& a, int t, RowPos& q) {
+    if (!taskPos<PX, PY>(a, t, q)) q.x = a.nx - 1;
     if (!PX) {
         q.xm = max(q.x - 1, 0);
         q.xp = min(q.x + 1, a.nx - 1);
     }
-    st.pop = PopBuf<S>(a, q);
+}
+
+// Issues the kind byte and the 19 pull loads of task t for every lane with no
+// branch around them. Unconditional issue keeps the number of memory
+// operations between a load and its use fixed, so the waits the compiler
+// inserts let the next task's loads stay in flight.
+template <class S, bool FROM_L, bool PX, bool PY>
+__device__ inline void pipeIssue(const KArgs& a, int t, PipeStage<S>& st) {
+    RowPos q;
+    pipeLoadPos<PX, PY>(a, t, q);
+    const PopBuf<S> pop(a, q);
+    st.t = t;
     st.k = a.kind[nb(q, 0, 0, 0)];
     staticFor<kQ>([&](auto qc) {
         using L = PullSlot<FROM_L, decltype(qc)::value>;
-        st.B[decltype(qc)::value] = st.pop.ldBits(L::dir, L::ex, L::ey, L::ez);
+        st.B[decltype(qc)::value] = pop.ldBits(L::dir, L::ex, L::ey, L::ez);
     });
 }
 
 template <class S, bool FROM_L, bool WM, bool PX, bool PY>
 __device__ inline void pipeCompute(const KArgs& a, PipeStage<S>& st) {
     using C = typename S::C;
-    if (!st.v) return;
+    RowPos p;
+    if (!taskPos<PX, PY>(a, st.t, p)) return;
+    RowPos q = p;
+    if (!PX) {
+        q.xm = max(q.x - 1, 0);
+        q.xp = min(q.x + 1, a.nx - 1);
+    }
+    const PopBuf<S> pop(a, q);
     C H[kQ];
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(st.B[q]);
-    cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, st.p, st.pop, st.k, H);
+    for (int i = 0; i < kQ; ++i) H[i] = S::fromBits(st.B[i]);
+    cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, p, pop, st.k, H);
 }
 
 template <class S, bool FROM_L, bool WM, bool PX, bool PY>

@@ -1087,13 +1087,14 @@ static double roundFormat(double x, int mant, int emin) {
 // magnitude, saturating), then decoded back (restatement of the device's
 // StoreF16C, which has no reference counterpart).
 static uint16_t f16cEncode(float v) {
-    const float y = v * 0x1p-112f;
+    // saturate at the largest code (1 + 2047/2048), NaN to the largest
+    // negative code, then round to nearest even at bit 12 of the FP32 bits
+    const float lim = 0x1.FFEp0f * 0x1p-112f;
+    const float y = std::fmin(std::fmax(v * 0x1p-112f, -lim), lim);
     uint32_t bits;
     std::memcpy(&bits, &y, 4);
-    const uint32_t mag = bits & 0x7FFFFFFFu;
-    uint32_t r = (mag + 0x7FFu + ((mag >> 12) & 1u)) >> 12;
-    if (r > 0x7FFFu) r = 0x7FFFu;
-    return static_cast<uint16_t>(r | ((bits >> 16) & 0x8000u));
+    const uint32_t r = bits + 0x7FFu + ((bits >> 12) & 1u);
+    return static_cast<uint16_t>(((r >> 12) & 0x7FFFu) | ((r >> 16) & 0x8000u));
 }
 static float f16cDecode(uint16_t b) {
     const uint32_t x = b;

@@ -1,8 +1,8 @@
 """FP16C storage codec (1 sign, 4 exponent, 11 mantissa bits; the device's
 StoreF16C, restated in the oracle) pinned by the definition of the format:
 value = (-1)^s 2^(e-15) (1 + m/2^11) for e >= 1, (-1)^s 2^-14 (m/2^11) for
-e = 0; encoding rounds to the nearest code (ties to even mantissa) and
-saturates at the largest code."""
+e = 0; encoding rounds to the nearest code (ties to even mantissa),
+saturates at the largest code and sends NaN to the largest negative code."""
 import math
 
 import numpy as np
@@ -59,6 +59,10 @@ def test_saturation_and_relative_precision():
     assert orc.f16c_encode(5.0) == 0x7FFF
     assert orc.f16c_encode(-5.0) == 0xFFFF
     assert orc.f16c_encode(float("inf")) == 0x7FFF
+    assert orc.f16c_encode(float("-inf")) == 0xFFFF
+    assert orc.f16c_encode(float("nan")) == 0xFFFF
+    assert orc.f16c_encode(value_of(0x7FFF)) == 0x7FFF
+    assert orc.f16c_encode(-value_of(0x7FFF)) == 0xFFFF
     # relative rounding error of normal values is at most 2^-12
     rng = np.random.default_rng(3)
     for v in rng.uniform(2.0 ** -14, 1.99, 5000).astype(np.float32):
