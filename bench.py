@@ -137,6 +137,102 @@ def copy_calibration(mib=1024, reps=5):
                       "bytes_written": n * 4, "launches": reps}), flush=True)
 
 
+def polar_bench(args):
+    """Batched polar sweep: 16 NACA 2412 cases (AoA -4..16 deg, chords from a
+    fixed-seed draw in [96, 192], grids 2c x 0.75c x 0.9c) stepped as one
+    group (one batched launch per step, graph replay, autotuned launch shape).
+    Aggregate GLUPS of the group, and each case's GLUPS when stepped alone."""
+    import foil_cfd_amd as fc
+    from foil_cfd_amd import cases, group
+    fc.load()
+    t_setup = time.perf_counter()
+    cs = group.polar_sweep(16)
+    log(f"[polar] 16 cases built in {time.perf_counter() - t_setup:.1f} s, "
+        f"{sum(c.flags.size for c in cs) / 1e6:.1f} M cells")
+    solvers = [cases.make_solver(c, args.polar_storage) for c in cs]
+    if args.polar_wall != "off":
+        for s in solvers:
+            s.set_wall_model("newton")
+    g = group.SolverGroup(solvers)
+    g.step(args.warmup)
+    trained = None
+    if args.polar_wall == "mlp":
+        import numpy as np
+        from foil_cfd_amd import mlp
+        for s in solvers:
+            s.set_wall_harvest(True)
+        got = []
+        for _ in range(4):
+            g.step(5)
+            for s in solvers:
+                h = s.wall_harvest()
+                got.append(h[h[:, 3] > 0])
+        for s in solvers:
+            s.set_wall_harvest(False)
+        smp = np.concatenate(got)
+        tr = mlp.MlpTrainer(smp, seed=7)
+        t0 = time.perf_counter()
+        tr.fit(2000, 1024, 3e-3, 1e-4)
+        trained = {"samples": int(len(smp)), "train_s": round(time.perf_counter() - t0, 3)}
+        prm = tr.export()
+        for s in solvers:
+            s.set_wall_mlp(prm)
+            s.set_wall_model("mlp")
+        tr.close()
+    best = g.autotune(steps_per_trial=args.polar_trial)
+    log(f"[polar] autotuned launch shape: {best['waves_per_block']} waves/workgroup, "
+        f"{'XCD slab' if best['slab_order'] else 'linear'} order, {best['glups']:.2f} GLUPS")
+    for s in solvers:
+        s.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(args.per_call, args.steps - done)
+        g.step(k)
+        done += k
+    for s in solvers:
+        s.synchronize()
+    wall = time.perf_counter() - t0
+    ncells = sum(s.ncells for s in solvers)
+    agg = ncells * args.steps / wall / 1e9
+    g.close()
+    per_case = []
+    for c, s in zip(cs, solvers):
+        s.step_n(args.polar_trial)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step_n(args.polar_solo_steps)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        per_case.append({"dims": list(c.dims), "aoa": round(float(c.name.split("_a")[-1]), 3)
+                         if "_a" in c.name else None,
+                         "glups_alone": round(s.ncells * args.polar_solo_steps / dt / 1e9, 3)})
+        s.close()
+    seq_ms = sum(c["dims"][0] * c["dims"][1] * c["dims"][2] / (c["glups_alone"] * 1e6) for c in per_case)
+    out = {
+        "metric": "polar sweep aggregate GLUPS (16 uneven NACA 2412 cases, one batched launch per step)",
+        "value": round(agg, 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "naca2412_polar16_aoa-4to16_chords96to192_grid2c_0.75c_0.9c",
+                   "cells": ncells, "storage": args.polar_storage, "wall_model": args.polar_wall,
+                   "launch": {k: best[k] for k in ("waves_per_block", "slab_order", "use_graph")},
+                   "network": trained},
+        "per_case": per_case,
+        "sequential_ms_per_step": round(seq_ms, 4),
+        "batched_vs_sequential": round(seq_ms / (wall / args.steps * 1e3), 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +245,12 @@ def main():
     ap.add_argument("--storages", default="f16c,f32")
     ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe"],
                     help="step kernel: auto (library default), single or pair")
+    ap.add_argument("--polar", action="store_true",
+                    help="batched 16-case polar sweep instead of the wing (separate JSON line)")
+    ap.add_argument("--polar-storage", default="f16c")
+    ap.add_argument("--polar-wall", default="mlp", choices=["off", "newton", "mlp"])
+    ap.add_argument("--polar-trial", type=int, default=10)
+    ap.add_argument("--polar-solo-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -160,6 +262,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.polar:
+        polar_bench(args)
         return
 
     import foil_cfd_amd as fc

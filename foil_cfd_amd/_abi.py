@@ -89,6 +89,13 @@ class FcMlpParams(C.Structure):
                 ("w3", C.c_uint16 * 32), ("b3", C.c_float)]
 
 
+class FcGroupInfo(C.Structure):
+    _fields_ = [("members", C.c_int), ("cells", C.c_longlong), ("waves_per_block", C.c_int),
+                ("slab_order", C.c_int), ("use_graph", C.c_int), ("cache_budget", C.c_longlong),
+                ("sub_batches", C.c_int), ("last_step_ms", C.c_double),
+                ("glups", C.c_double)]
+
+
 class FcVGParams(C.Structure):
     _fields_ = [("type", C.c_int), ("x_c", C.c_float), ("height_c", C.c_float),
                 ("length_h", C.c_float), ("beta_deg", C.c_float), ("pitch_c", C.c_float),
@@ -210,6 +217,15 @@ SIGNATURES = {
     "fc_mlp_trainer_data": (C.c_int, [_vp, _fp, _fp]),
     "fc_mlp_trainer_export": (C.c_int, [_vp, C.POINTER(FcMlpParams)]),
     "fc_mlp_infer": (C.c_int, [C.POINTER(FcMlpParams), _fp, C.c_int, _fp]),
+    "fc_group_create": (_vp, [C.POINTER(_vp), C.c_int, C.c_char_p, C.c_int]),
+    "fc_group_destroy": (None, [_vp]),
+    "fc_group_step": (C.c_int, [_vp, C.c_int]),
+    "fc_group_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "fc_group_set_cache_budget": (C.c_int, [_vp, C.c_longlong]),
+    "fc_group_last_step_ms": (C.c_int, [_vp, _dp]),
+    "fc_group_get_info": (C.c_int, [_vp, C.POINTER(FcGroupInfo)]),
+    "fc_group_last_error": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "fc_group_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(FcGroupInfo)]),
     "fc_set_qlibb_enabled": (C.c_int, [_vp, C.c_int]),
     "fc_set_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
     "fc_qlibb_readout": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),

@@ -72,6 +72,25 @@ int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, i
                    int* flag, void* stream);
 int launchFillFloat(float* dst, long long count, float value, void* stream);
 
+// ---- lattice groups (one batched step launch for several lattices) --------
+inline constexpr int kGroupMaxMembers = 64;
+// Size of one device record of a (step, member) pair; built on the host.
+std::size_t groupEntryBytes();
+// Row-segment tasks of a lattice (its share of a group launch).
+int groupTasks(const LatticeView& lat);
+// 1 if the lattice can join a group step (16-/32-bit storage, array under
+// 2 GiB, x and y not periodic).
+int groupEligible(const LatticeView& lat);
+// Fills one record (groupEntryBytes() bytes at out) for a member stepping
+// from `layout` with `args`; taskBase = first task of the member.
+int buildGroupEntry(const LatticeView& lat, int layout, const StepArgs& args, int taskBase, void* out);
+// One batched step of `members` lattices sharing `storage`: table = device
+// array of their records; bounds[0..members] = task prefix sums (host).
+// fromL / wm: the members' common layout parity (1: layout L) and wall-model
+// setting, or -1 when they differ (per-wave dispatch).
+int launchGroupStep(int storage, const void* table, int members, const int* bounds, int wavesPerBlock,
+                    int slabOrder, int fromL, int wm, void* stream);
+
 // ---- wall model -------------------------------------------------------------
 // Device view of a level's wall-cell list (SoA, reference WallCellListView,
 // src/sim/lbm_wallmodel.cuh:47-57); uTau is the persistent EMA state.

@@ -19,6 +19,8 @@
 //    plane that the x-1 cells publish every step.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "fc_kcommon.hpp"
 
 // No implicit contraction in this file: fused multiply-adds are written out
@@ -399,6 +401,80 @@ __global__ void __launch_bounds__(kBlock) stepKernelPipe(KArgs a) {
         pipeIssue<S, FROM_L, PX, PY>(a, t0 + (j + 1) * W, A);
         pipeCompute<S, FROM_L, WM, PX, PY>(a, B);
         ++j;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched step of a group of independent lattices (e.g. the cases of a polar
+// sweep, each with its own grid size, layout parity, relaxation time and
+// inlet speed) in one launch. The row-segment tasks of all members are
+// concatenated; a wave finds its member from the task bounds (kernel
+// arguments), reads that member's argument record with scalar loads and runs
+// the same cell update as the single-lattice kernel. Launch geometry (waves
+// per workgroup, XCD slab or linear task order) is a tuning choice that does
+// not change the result.
+// ---------------------------------------------------------------------------
+struct GroupEntry {
+    KArgs a;
+    int fromL;  // member array in layout L this step
+    int wm;     // member has the wall-model slip field
+    int taskBase;
+    int pad;
+};
+
+struct GroupLaunch {
+    int members;
+    int totalTasks;
+    int wavesPerBlock;
+    int slabOrder;  // 1: each XCD takes a contiguous slab of the task range
+    int nblocks;
+    int chunk;      // workgroups per XCD slab
+    int bound[kGroupMaxMembers + 1];  // task prefix: member m owns [bound[m], bound[m+1])
+};
+
+// Member and local task of this wave; false past the end of the launch.
+__device__ inline bool groupTask(const GroupLaunch& g, int& m, int& local) {
+    const int hb = blockIdx.x;
+    const int lb = g.slabOrder ? (hb & (kXcds - 1)) * g.chunk + (hb >> 3) : hb;
+    if (lb >= g.nblocks) return false;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int gw = lb * g.wavesPerBlock + wv;
+    if (gw >= g.totalTasks) return false;
+    m = 0;
+    while (m + 1 < g.members && gw >= g.bound[m + 1]) ++m;
+    local = gw - g.bound[m];
+    return true;
+}
+
+// Launch whose members share the layout parity and the wall-model setting
+// (the usual case: members stepped together from their initial state).
+template <class S, bool FROM_L, bool WM>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
+stepKernelGroup(const GroupEntry* __restrict__ tab, GroupLaunch g) {
+    int m, local;
+    if (!groupTask(g, m, local)) return;
+    const KArgs& a = tab[m].a;
+    RowPos p;
+    if (!taskPos<false, false>(a, local, p)) return;
+    stepCell<S, PopBuf, FROM_L, WM, false, false>(a, p);
+}
+
+// Members with mixed parity or wall-model settings: per-wave dispatch.
+template <class S>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
+stepKernelGroupMixed(const GroupEntry* __restrict__ tab, GroupLaunch g) {
+    int m, local;
+    if (!groupTask(g, m, local)) return;
+    const GroupEntry& e = tab[m];
+    const KArgs& a = e.a;
+    RowPos p;
+    if (!taskPos<false, false>(a, local, p)) return;
+    if (e.fromL) {
+        if (e.wm) stepCell<S, PopBuf, true, true, false, false>(a, p);
+        else stepCell<S, PopBuf, true, false, false, false>(a, p);
+    } else {
+        if (e.wm) stepCell<S, PopBuf, false, true, false, false>(a, p);
+        else stepCell<S, PopBuf, false, false, false, false>(a, p);
     }
 }
 
@@ -976,6 +1052,73 @@ int launchFillFloat(float* dst, long long count, float value, void* stream) {
     if (!dst || count <= 0) return static_cast<int>(hipErrorInvalidValue);
     const unsigned blocks = static_cast<unsigned>((count + kBlock - 1) / kBlock);
     fillFloatKernel<<<blocks, kBlock, 0, static_cast<hipStream_t>(stream)>>>(dst, count, value);
+    return lastError();
+}
+
+
+// ---- lattice groups ---------------------------------------------------------
+std::size_t groupEntryBytes() { return sizeof(GroupEntry); }
+
+int groupTasks(const LatticeView& lat) { return makeArgs(lat).nwaves; }
+
+int groupEligible(const LatticeView& lat) {
+    if (lat.storage == kStoreF64 || lat.periodicX || lat.periodicY) return 0;
+    const unsigned long long bytes = static_cast<unsigned long long>(kQ) *
+                                     static_cast<unsigned long long>(lat.dims.cells()) * storageBytes(lat.storage);
+    return bytes < (1ull << 31) ? 1 : 0;
+}
+
+int buildGroupEntry(const LatticeView& lat, int layout, const StepArgs& sa, int taskBase, void* out) {
+    if (!out || !groupEligible(lat)) return static_cast<int>(hipErrorInvalidValue);
+    GroupEntry e{};
+    e.a = makeArgs(lat);
+    e.a.tau0 = sa.tau0;
+    e.a.lam = sa.magicLambda;
+    e.a.pre = sa.smagPrefactor;
+    e.a.uIn = sa.uInlet;
+    e.a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+    e.fromL = layout == kLayoutL ? 1 : 0;
+    e.wm = (lat.uwx && lat.uwy && lat.uwz) ? 1 : 0;
+    e.taskBase = taskBase;
+    std::memcpy(out, &e, sizeof(e));
+    return 0;
+}
+
+template <class S>
+void groupLaunch(dim3 grid, dim3 b, hipStream_t st, const GroupEntry* tab, const GroupLaunch& g, int fromL,
+                 int wm) {
+    if (fromL < 0 || wm < 0) stepKernelGroupMixed<S><<<grid, b, 0, st>>>(tab, g);
+    else if (fromL && wm) stepKernelGroup<S, true, true><<<grid, b, 0, st>>>(tab, g);
+    else if (fromL) stepKernelGroup<S, true, false><<<grid, b, 0, st>>>(tab, g);
+    else if (wm) stepKernelGroup<S, false, true><<<grid, b, 0, st>>>(tab, g);
+    else stepKernelGroup<S, false, false><<<grid, b, 0, st>>>(tab, g);
+}
+
+int launchGroupStep(int storage, const void* table, int members, const int* bounds, int wavesPerBlock,
+                    int slabOrder, int fromL, int wm, void* stream) {
+    if (!table || !bounds || members < 1 || members > kGroupMaxMembers ||
+        (wavesPerBlock != 1 && wavesPerBlock != 2 && wavesPerBlock != 4 && wavesPerBlock != 8))
+        return static_cast<int>(hipErrorInvalidValue);
+    GroupLaunch g{};
+    g.members = members;
+    for (int m = 0; m <= members; ++m) g.bound[m] = bounds[m];
+    g.totalTasks = bounds[members];
+    g.wavesPerBlock = wavesPerBlock;
+    g.slabOrder = slabOrder ? 1 : 0;
+    g.nblocks = (g.totalTasks + wavesPerBlock - 1) / wavesPerBlock;
+    g.chunk = (g.nblocks + kXcds - 1) / kXcds;
+    if (g.totalTasks <= 0) return 0;
+    const unsigned grid = static_cast<unsigned>(g.slabOrder ? g.chunk * kXcds : g.nblocks);
+    const dim3 b(static_cast<unsigned>(wavesPerBlock * kWave));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const auto* tab = static_cast<const GroupEntry*>(table);
+    switch (storage) {
+        case kStoreF32: groupLaunch<StoreF32>(grid, b, st, tab, g, fromL, wm); break;
+        case kStoreF16: groupLaunch<StoreF16>(grid, b, st, tab, g, fromL, wm); break;
+        case kStoreBF16: groupLaunch<StoreBF16>(grid, b, st, tab, g, fromL, wm); break;
+        case kStoreF16C: groupLaunch<StoreF16C>(grid, b, st, tab, g, fromL, wm); break;
+        default: return static_cast<int>(hipErrorInvalidValue);
+    }
     return lastError();
 }
 

@@ -397,6 +397,77 @@ public:
         return FC_OK;
     }
 
+    // ---- group stepping (fc_group_*): one batched launch per step steps
+    // every member; everything else of a batch stays per member and matches
+    // stepN (ramped tau / inlet speed per step, macro store on the batch's
+    // last step, one force sample and one wall-model update per batch). The
+    // NaN watchdog samples at the batch end once 200 or more steps have passed.
+    bool groupReady(std::string& why) const {
+        if (!initialized_) why = "member not initialized";
+        else if (fine_.active) why = "member has a refinement patch";
+        else if (qCount_ > 0 && qlibbEnabled_) why = "member has q-LIBB links";
+        else if (!groupEligible(view())) why = "member needs 16/32-bit storage under 2 GiB, x / y not periodic";
+        else return true;
+        return false;
+    }
+    hipStream_t stream() const { return stream_; }
+    long long cells() const { return ncells_; }
+    int groupTaskCount() const { return groupTasks(view()); }
+    // bit 0: array in layout L now; bit 1: wall-model slip field present
+    int groupFlags() const {
+        const LatticeView l = view();
+        return (layout_ == kLayoutL ? 1 : 0) | ((l.uwx && l.uwy && l.uwz) ? 2 : 0);
+    }
+    // Starts a group batch; false when the member is paused on a NaN.
+    bool groupBegin() {
+        pollAsync();
+        if (nanLatched_) return false;
+        groupTimed_ = !timingPending_;
+        if (groupTimed_) (void)hipEventRecord(evT0_, stream_);
+        return true;
+    }
+    // Record of step i (0-based) of an n-step batch starting now.
+    int groupEntry(int i, int n, int taskBase, void* out) const {
+        StepArgs a;
+        a.tau0 = tauAt(steps_ + i);
+        a.magicLambda = kTRTMagicLambda;
+        a.smagPrefactor = smagPrefactor(cs_);
+        a.uInlet = uAt(steps_ + i);
+        a.writeMacro = (i == n - 1) ? 1 : 0;
+        const int lay = (i & 1) ? 1 - layout_ : layout_;
+        return buildGroupEntry(view(), lay, a, taskBase, out);
+    }
+    // Finishes a group batch of n steps (call after the member's stream has
+    // been ordered behind the batched launches).
+    int groupEnd(int n) {
+        if (n & 1) layout_ = 1 - layout_;
+        steps_ += n;
+        stepsSinceWatchdog_ += n;
+        if (stepsSinceWatchdog_ >= kWatchdogPeriodSteps && !nanPending_) {
+            stepsSinceWatchdog_ = 0;
+            (void)hipMemsetAsync(dNan_, 0, sizeof(int), stream_);
+            launchWatchdog(view(), layout_, steps_ * 977, 0, dNan_, stream_);
+            (void)hipMemcpyAsync(hNan_, dNan_, sizeof(int), hipMemcpyDeviceToHost, stream_);
+            (void)hipEventRecord(evNan_, stream_);
+            nanPending_ = true;
+        }
+        if (groupTimed_) {
+            (void)hipEventRecord(evT1_, stream_);
+            timingPending_ = true;
+            timingBatchN_ = n;
+        }
+        if (!forcePending_) {
+            if (launchForceInto(dForce_) == FC_OK) {
+                (void)hipMemcpyAsync(hForce_, dForce_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+                (void)hipEventRecord(evForce_, stream_);
+                forcePending_ = true;
+                stepsAtForceLaunch_ = steps_;
+            }
+        }
+        if (wm_.active) return wallUpdate();
+        return FC_OK;
+    }
+
     // ---- q-LIBB (reference setQLIBBEnabled / setCoarseQLinks, lbm_solver.cpp:1027-1080)
     void freeQLinks() {
         hipRelease(dQCells_);
@@ -883,14 +954,14 @@ private:
     }
 
     float flowThroughSteps() const { return scaling_.flowThroughSteps(dims_.nx); }
-    float effectiveTau() const {
-        return (rampActive_ && rampEnabled_) ? rampedTau(scaling_, steps_ - rampStartStep_, dims_.nx)
-                                             : scaling_.tau;
+    float tauAt(long long step) const {
+        return (rampActive_ && rampEnabled_) ? rampedTau(scaling_, step - rampStartStep_, dims_.nx) : scaling_.tau;
     }
-    float effectiveU() const {
-        return (rampActive_ && rampEnabled_) ? rampedU(scaling_, steps_ - rampStartStep_, dims_.nx)
-                                             : scaling_.u_lat;
+    float uAt(long long step) const {
+        return (rampActive_ && rampEnabled_) ? rampedU(scaling_, step - rampStartStep_, dims_.nx) : scaling_.u_lat;
     }
+    float effectiveTau() const { return tauAt(steps_); }
+    float effectiveU() const { return uAt(steps_); }
 
     LatticeView view() const {
         LatticeView l;
@@ -1591,6 +1662,7 @@ private:
     int nanKind_ = 0;
     int stepsSinceWatchdog_ = 0;
     bool timingPending_ = false;
+    bool groupTimed_ = false;
     int timingBatchN_ = 0;
     double emaStepMs_ = 0.0, lastStepMs_ = 0.0;
 
@@ -2112,6 +2184,360 @@ float fc_ray_obb_entry(const float* origin3, const float* dir3, const fc_vane_sl
     if (!origin3 || !dir3 || !slab) return -1.0f;
     return fc::rayOBBEntry(fc::Vec3{origin3[0], origin3[1], origin3[2]}, fc::Vec3{dir3[0], dir3[1], dir3[2]},
                            slabFromC(*slab));
+}
+
+}  // extern "C"
+
+// ---- lattice groups: batched stepping of independent solvers ----------------
+// No reference counterpart (the reference steps one solver at a time). A
+// group steps its members with one batched launch per step; the launches of
+// a batch are captured once into a HIP graph per (sub-batch, chunk length,
+// launch shape) and replayed, with the per-step member records (ramped tau /
+// inlet speed, layout parity, macro flag) refreshed in a pinned table before
+// each replay. Members are split into consecutive sub-batches whose
+// population footprint fits a byte budget; a sub-batch runs its steps of a
+// chunk before the next one starts, so a footprint that fits the GPU's
+// last-level (Infinity) cache stays resident across those steps. Members are
+// independent, so the split changes no result; the autotuner picks it.
+namespace fc {
+namespace {
+
+constexpr int kGroupChunkSteps = 256;  // steps per uploaded table / graph
+
+struct GroupKey {
+    int first, count, steps, wpb, slab, fromL0, wm;
+    bool operator==(const GroupKey& o) const {
+        return first == o.first && count == o.count && steps == o.steps && wpb == o.wpb && slab == o.slab &&
+               fromL0 == o.fromL0 && wm == o.wm;
+    }
+};
+
+struct SubBatch {
+    int first = 0, count = 0;
+    std::vector<int> bounds;  // task prefix sums of its members
+};
+
+struct Group {
+    std::vector<Solver*> members;
+    int storage = kStoreF32;
+    std::vector<int> tasks;           // row-segment tasks per member
+    std::vector<long long> footprint;  // population + kind bytes per member
+    long long budget = 0;              // sub-batch footprint budget (0: one batch)
+    std::vector<SubBatch> subs;
+    long long cells = 0;
+    std::size_t entryBytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t tBegin = nullptr, tEnd = nullptr, done = nullptr;
+    std::vector<hipEvent_t> memberReady;
+    void* dTable = nullptr;
+    unsigned char* hTable = nullptr;  // pinned
+    int wpb = 4, slab = 1;
+    bool useGraph = true;
+    std::vector<std::pair<GroupKey, hipGraphExec_t>> graphs;
+    double lastStepMs = 0.0;
+    long long lastSteps = 0;
+    std::string error;
+
+    ~Group() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        dropGraphs();
+        if (dTable) (void)hipFree(dTable);
+        if (hTable) (void)hipHostFree(hTable);
+        for (hipEvent_t e : memberReady)
+            if (e) (void)hipEventDestroy(e);
+        if (tBegin) (void)hipEventDestroy(tBegin);
+        if (tEnd) (void)hipEventDestroy(tEnd);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void dropGraphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    int fail(int code, const std::string& why) {
+        error = why;
+        return code;
+    }
+
+    // Consecutive sub-batches: a member joins the current one while the
+    // running footprint stays within the budget (every sub-batch has one).
+    void partition() {
+        subs.clear();
+        const int M = static_cast<int>(members.size());
+        long long used = 0;
+        for (int m = 0; m < M; ++m) {
+            if (subs.empty() || (budget > 0 && used + footprint[m] > budget)) {
+                subs.push_back(SubBatch{m, 0, {0}});
+                used = 0;
+            }
+            SubBatch& s = subs.back();
+            s.bounds.push_back(s.bounds.back() + tasks[m]);
+            ++s.count;
+            used += footprint[m];
+        }
+    }
+
+    // Common layout parity (1 = L) of a sub-batch's members at the chunk's
+    // first step and their common wall-model setting; -1 where they differ.
+    void subFlags(const SubBatch& sb, int chunkStart, int& fromL0, int& wm) const {
+        fromL0 = wm = -2;
+        for (int m = sb.first; m < sb.first + sb.count; ++m) {
+            const int f = members[m]->groupFlags();
+            const int l = (f & 1) ^ (chunkStart & 1);
+            const int w = (f >> 1) & 1;
+            fromL0 = fromL0 == -2 ? l : (fromL0 == l ? l : -1);
+            wm = wm == -2 ? w : (wm == w ? w : -1);
+        }
+    }
+
+    // The chunk's launches of one sub-batch on `stream` (recorded into a
+    // graph or issued directly).
+    int issue(const SubBatch& sb, int steps, int fromL0, int wm) {
+        const int M = static_cast<int>(members.size());
+        const char* base = static_cast<const char*>(dTable);
+        for (int i = 0; i < steps; ++i) {
+            const int fromL = fromL0 < 0 ? -1 : (fromL0 ^ (i & 1));
+            if (launchGroupStep(storage, base + (static_cast<std::size_t>(i) * M + sb.first) * entryBytes, sb.count,
+                                sb.bounds.data(), wpb, slab, fromL, wm, stream) != 0)
+                return FC_ERR_DEVICE;
+        }
+        return FC_OK;
+    }
+
+    int launchChunk(const SubBatch& sb, int steps, int chunkStart) {
+        int fromL0, wm;
+        subFlags(sb, chunkStart, fromL0, wm);
+        if (!useGraph) return issue(sb, steps, fromL0, wm);
+        const GroupKey key{sb.first, sb.count, steps, wpb, slab, fromL0, wm};
+        hipGraphExec_t exec = nullptr;
+        for (auto& kv : graphs)
+            if (kv.first == key) exec = kv.second;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "graph capture could not start");
+            const int rc = issue(sb, steps, fromL0, wm);
+            const hipError_t ec = hipStreamEndCapture(stream, &graph);
+            if (rc != FC_OK || ec != hipSuccess || !graph) {
+                if (graph) (void)hipGraphDestroy(graph);
+                (void)hipGetLastError();
+                return fail(FC_ERR_DEVICE, "graph capture failed");
+            }
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(FC_ERR_DEVICE, "graph instantiation failed");
+            }
+            graphs.emplace_back(key, exec);
+        }
+        return hipGraphLaunch(exec, stream) == hipSuccess ? FC_OK : fail(FC_ERR_DEVICE, "graph launch failed");
+    }
+
+    int step(int n) {
+        if (n <= 0) return FC_OK;
+        const int M = static_cast<int>(members.size());
+        // a member paused on a NaN keeps its state: the whole group waits
+        for (Solver* s : members)
+            if (!s->groupBegin()) return fail(FC_ERR_STATE, "a member is paused on a NaN");
+        for (int m = 0; m < M; ++m) {
+            if (hipEventRecord(memberReady[m], members[m]->stream()) != hipSuccess ||
+                hipStreamWaitEvent(stream, memberReady[m], 0) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "member ordering failed");
+        }
+        (void)hipEventRecord(tBegin, stream);
+        for (int done0 = 0; done0 < n;) {
+            const int k = std::min(kGroupChunkSteps, n - done0);
+            // the pinned table is rewritten only after the previous chunk's upload finished
+            if (hipStreamSynchronize(stream) != hipSuccess) return fail(FC_ERR_DEVICE, "group stream failed");
+            for (int i = 0; i < k; ++i)
+                for (int m = 0; m < M; ++m)
+                    if (members[m]->groupEntry(done0 + i, n, 0,
+                                               hTable + (static_cast<std::size_t>(i) * M + m) * entryBytes) != 0)
+                        return fail(FC_ERR_INVALID, "member record");
+            // records are indexed from the batch start (members advance in groupEnd)
+            if (hipMemcpyAsync(dTable, hTable, static_cast<std::size_t>(k) * M * entryBytes, hipMemcpyHostToDevice,
+                               stream) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "table upload failed");
+            for (const SubBatch& sb : subs)
+                if (int rc = launchChunk(sb, k, done0); rc != FC_OK) return rc;
+            done0 += k;
+        }
+        (void)hipEventRecord(tEnd, stream);
+        (void)hipEventRecord(done, stream);
+        for (int m = 0; m < M; ++m)
+            if (hipStreamWaitEvent(members[m]->stream(), done, 0) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "member ordering failed");
+        int rc = FC_OK;
+        for (Solver* s : members)
+            if (int e = s->groupEnd(n); e != FC_OK && rc == FC_OK) rc = fail(e, "member batch end failed");
+        lastSteps = n;
+        return rc;
+    }
+
+    int timing(double* msPerStep) {
+        if (hipEventSynchronize(tEnd) != hipSuccess) return fail(FC_ERR_DEVICE, "timing event failed");
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, tBegin, tEnd) != hipSuccess) return fail(FC_ERR_DEVICE, "timing failed");
+        lastStepMs = lastSteps > 0 ? ms / static_cast<double>(lastSteps) : 0.0;
+        if (msPerStep) *msPerStep = lastStepMs;
+        return FC_OK;
+    }
+};
+
+}  // namespace
+}  // namespace fc
+
+struct fc_group {
+    fc::Group impl;
+};
+
+extern "C" {
+
+fc_group* fc_group_create(fc_solver* const* members, int count, char* err, int errlen) {
+    if (!members || count < 1 || count > fc::kGroupMaxMembers) {
+        fc::copyErr("group needs 1..64 members", err, errlen);
+        return nullptr;
+    }
+    auto* g = new fc_group;
+    fc::Group& G = g->impl;
+    std::string why;
+    for (int m = 0; m < count; ++m) {
+        if (!members[m]) why = "null member";
+        for (int k = 0; k < m && why.empty(); ++k)
+            if (members[k] == members[m]) why = "a solver appears twice";
+        if (why.empty() && !members[m]->impl.groupReady(why)) why = "member " + std::to_string(m) + ": " + why;
+        if (why.empty() && m > 0 && members[m]->impl.storage() != members[0]->impl.storage())
+            why = "members use different storage formats";
+        if (!why.empty()) break;
+        G.members.push_back(&members[m]->impl);
+    }
+    if (why.empty()) {
+        G.storage = G.members[0]->storage();
+        G.entryBytes = fc::groupEntryBytes();
+        for (fc::Solver* s : G.members) {
+            G.tasks.push_back(s->groupTaskCount());
+            G.footprint.push_back(s->cells() * (19LL * fc::storageBytes(G.storage) + 1));
+            G.cells += s->cells();
+        }
+        G.partition();
+        const std::size_t bytes = static_cast<std::size_t>(fc::kGroupChunkSteps) * count * G.entryBytes;
+        bool ok = hipStreamCreate(&G.stream) == hipSuccess && hipEventCreate(&G.tBegin) == hipSuccess &&
+                  hipEventCreate(&G.tEnd) == hipSuccess &&
+                  hipEventCreateWithFlags(&G.done, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc(&G.dTable, bytes) == hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&G.hTable), bytes, hipHostMallocDefault) == hipSuccess;
+        for (int m = 0; ok && m < count; ++m) {
+            hipEvent_t e = nullptr;
+            ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+            G.memberReady.push_back(e);
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            why = "group device resources could not be created";
+        }
+    }
+    if (!why.empty()) {
+        fc::copyErr(why, err, errlen);
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+void fc_group_destroy(fc_group* g) { delete g; }
+
+int fc_group_step(fc_group* g, int n) { return g ? g->impl.step(n) : FC_ERR_INVALID; }
+
+int fc_group_set_launch(fc_group* g, int waves_per_block, int slab_order, int use_graph) {
+    if (!g || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 && waves_per_block != 8))
+        return FC_ERR_INVALID;
+    g->impl.wpb = waves_per_block;
+    g->impl.slab = slab_order ? 1 : 0;
+    g->impl.useGraph = use_graph != 0;
+    return FC_OK;
+}
+
+int fc_group_set_cache_budget(fc_group* g, long long bytes) {
+    if (!g || bytes < 0) return FC_ERR_INVALID;
+    fc::Group& G = g->impl;
+    if (hipStreamSynchronize(G.stream) != hipSuccess) return G.fail(FC_ERR_DEVICE, "group stream failed");
+    G.budget = bytes;
+    G.dropGraphs();
+    G.partition();
+    return FC_OK;
+}
+
+int fc_group_get_info(fc_group* g, fc_group_info* out) {
+    if (!g || !out) return FC_ERR_INVALID;
+    const fc::Group& G = g->impl;
+    out->members = static_cast<int>(G.members.size());
+    out->cells = G.cells;
+    out->waves_per_block = G.wpb;
+    out->slab_order = G.slab;
+    out->use_graph = G.useGraph ? 1 : 0;
+    out->cache_budget = G.budget;
+    out->sub_batches = static_cast<int>(G.subs.size());
+    out->last_step_ms = G.lastStepMs;
+    out->glups = G.lastStepMs > 0.0 ? static_cast<double>(G.cells) / (G.lastStepMs * 1e6) : 0.0;
+    return FC_OK;
+}
+
+int fc_group_last_step_ms(fc_group* g, double* ms) { return g ? g->impl.timing(ms) : FC_ERR_INVALID; }
+
+int fc_group_last_error(fc_group* g, char* buf, int cap) {
+    if (!g) return FC_ERR_INVALID;
+    fc::copyErr(g->impl.error, buf, cap);
+    return FC_OK;
+}
+
+int fc_group_autotune(fc_group* g, int steps_per_trial, fc_group_info* best) {
+    if (!g || steps_per_trial < 1) return FC_ERR_INVALID;
+    fc::Group& G = g->impl;
+    // one trial: a batch that builds the shape's graphs, then a timed batch
+    auto trial = [&](double* ms) -> int {
+        if (int rc = G.step(steps_per_trial); rc != FC_OK) return rc;
+        if (int rc = G.step(steps_per_trial); rc != FC_OK) return rc;
+        return G.timing(ms);
+    };
+    // 1) sub-batch footprint budget (whole group, one member each, and
+    //    budgets around the 256 MB last-level cache) at the default shape
+    const long long MiB = 1LL << 20;
+    const long long budgets[5] = {0, 1, 128 * MiB, 256 * MiB, 512 * MiB};
+    G.wpb = 4;
+    G.slab = 1;
+    double bestMs = -1.0;
+    long long bb = 0;
+    for (long long b : budgets) {
+        if (int rc = fc_group_set_cache_budget(g, b); rc != FC_OK) return rc;
+        double ms = 0.0;
+        if (int rc = trial(&ms); rc != FC_OK) return rc;
+        if (bestMs < 0.0 || ms < bestMs) {
+            bestMs = ms;
+            bb = b;
+        }
+    }
+    if (int rc = fc_group_set_cache_budget(g, bb); rc != FC_OK) return rc;
+    // 2) launch shape: wavefronts per workgroup x task order
+    const int wpbs[4] = {1, 2, 4, 8};
+    int bw = 4, bs = 1;
+    bestMs = -1.0;
+    for (int slab = 0; slab <= 1; ++slab)
+        for (int w : wpbs) {
+            G.wpb = w;
+            G.slab = slab;
+            double ms = 0.0;
+            if (int rc = trial(&ms); rc != FC_OK) return rc;
+            if (bestMs < 0.0 || ms < bestMs) {
+                bestMs = ms;
+                bw = w;
+                bs = slab;
+            }
+        }
+    G.wpb = bw;
+    G.slab = bs;
+    G.lastStepMs = bestMs;
+    return best ? fc_group_get_info(g, best) : FC_OK;
 }
 
 }  // extern "C"

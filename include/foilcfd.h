@@ -376,6 +376,49 @@ int fc_mlp_trainer_export(fc_mlp_trainer* t, fc_mlp_params* out);
  * outside the envelope. */
 int fc_mlp_infer(const fc_mlp_params* params, const float* samples, int count, float* out);
 
+/* ---- lattice groups: batched stepping (no reference counterpart) ---------- */
+/* A group steps several initialized solvers (e.g. the cases of an angle-of-
+ * attack polar sweep, each with its own grid) with one batched step launch
+ * per step; the launches of a batch are captured into a HIP graph and
+ * replayed. fc_group_step(g, n) is equivalent to fc_step(member, n) for
+ * every member (same cell updates, ramps, macro store, force sample and
+ * wall-model update per batch; the NaN watchdog samples at the batch end
+ * once 200 or more steps have passed). Members must share the storage format
+ * (16- or 32-bit, population array under 2 GiB), have x / y not periodic, no
+ * refinement patch and no q-LIBB links; up to 64 members. */
+typedef struct fc_group fc_group;
+typedef struct fc_group_info {
+    int members;
+    long long cells;
+    int waves_per_block;  /* launch shape: 1, 2, 4 or 8 wavefronts per workgroup */
+    int slab_order;       /* 1: each XCD sweeps a contiguous slab of the task range */
+    int use_graph;
+    long long cache_budget; /* sub-batch footprint budget in bytes (0: all members per launch) */
+    int sub_batches;
+    double last_step_ms;  /* batched step time of the last timed batch */
+    double glups;         /* aggregate: all members' cells / last_step_ms */
+} fc_group_info;
+fc_group* fc_group_create(fc_solver* const* members, int count, char* err, int errlen);
+void fc_group_destroy(fc_group* g);
+int fc_group_step(fc_group* g, int n);
+int fc_group_set_launch(fc_group* g, int waves_per_block, int slab_order, int use_graph);
+/* Splits the members into consecutive sub-batches whose populations (plus
+ * kind bytes) fit `bytes`; each sub-batch runs its steps before the next, so
+ * a footprint that fits the last-level cache stays resident. 0 = one batch,
+ * 1 = one member per sub-batch. Results do not depend on it. */
+int fc_group_set_cache_budget(fc_group* g, long long bytes);
+/* Waits for the last batch and reports its time per step (also stored for
+ * fc_group_get_info). */
+int fc_group_last_step_ms(fc_group* g, double* ms);
+int fc_group_get_info(fc_group* g, fc_group_info* out);
+int fc_group_last_error(fc_group* g, char* buf, int cap);
+/* Times the sub-batch budgets (0, 1, 128 / 256 / 512 MiB) and then every
+ * launch shape (waves per workgroup x task order), each over two batches of
+ * steps_per_trial real steps (the members advance 26 x 2 x steps_per_trial
+ * steps; results do not depend on the choices), keeps the fastest and
+ * reports it. */
+int fc_group_autotune(fc_group* g, int steps_per_trial, fc_group_info* best);
+
 /* ---- interpolated bounce-back (q-LIBB), lbm_solver.h:311-346 -------------- */
 /* setQLIBBEnabled, :315 (off frees the link field; every link reverts to half-way). */
 int fc_set_qlibb_enabled(fc_solver* s, int enabled);
