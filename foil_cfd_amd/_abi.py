@@ -170,6 +170,12 @@ SIGNATURES = {
     "fc_shutdown_refinement": (C.c_int, [_vp]),
     "fc_set_refined_flags": (C.c_int, [_vp, _u8p]),
     "fc_get_refinement_info": (C.c_int, [_vp, C.POINTER(FcRefinementInfo)]),
+    "fc_init_finer_refinement": (C.c_int, [_vp, C.POINTER(FcPatchBox), C.c_int, _u8p, C.c_char_p,
+                                           C.c_int]),
+    "fc_shutdown_finer_refinement": (C.c_int, [_vp]),
+    "fc_get_finer_refinement_info": (C.c_int, [_vp, C.POINTER(FcRefinementInfo)]),
+    "fc_download_finer_f": (C.c_int, [_vp, _vp, C.c_int]),
+    "fc_upload_finer_f": (C.c_int, [_vp, _vp, C.c_int]),
     "fc_download_fine_f": (C.c_int, [_vp, _vp, C.c_int]),
     "fc_upload_fine_f": (C.c_int, [_vp, _vp, C.c_int]),
     "fc_step": (C.c_int, [_vp, C.c_int]),

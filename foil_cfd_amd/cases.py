@@ -111,6 +111,18 @@ def refinement_patch(case: Case, factor: int = 2, upstream_c: float = 0.35, wake
     return (box.x0, box.y0, box.x1, box.y1), fine
 
 
+def finer_patch(case: Case, box, factor: int, finer_box, finer_factor: int = 2, digits: str = "2412",
+                aoa: float = 8.0):
+    """Flags of a nested third level: finer_box (x0, y0, x1, y1) in cells of the
+    fine grid of `box` at `factor`, voxelized from the fine level's layout
+    (the reference's nested VG box, main.cpp:664-745)."""
+    nx, ny, nz = case.dims
+    layout = S.make_layout(nx, ny, nz, case.scaling.chord_cells)
+    fine_layout = S.make_fine_layout(layout, S.FcPatchBox(*box), factor)
+    pts, _ = S.generate_naca4(digits, 200)
+    return S.build_fine_patch_flags(pts, aoa, fine_layout, S.FcPatchBox(*finer_box), finer_factor)
+
+
 def make_solver(case: Case, storage: str = "f32") -> S.LBMSolver:
     s = S.LBMSolver()
     s.set_startup_ramp_enabled(case.ramp)

@@ -293,6 +293,9 @@ public:
         if (fine_.active) {
             if (int fe = seedFine(); fe != FC_OK) return fe;
         }
+        if (finer_.active) {
+            if (int fe = seedFiner(); fe != FC_OK) return fe;
+        }
         // Wall-model EMA state and slip field restart with the flow.
         if (wm_.active) {
             const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
@@ -369,6 +372,7 @@ public:
                 launchWatchdog(lat, layout_, steps_ * 977, 0, dNan_, stream_);
                 // The fine level is sampled too (verdict 3 / 4 names it).
                 if (fine_.active) launchWatchdog(fineView(), fine_.layout, steps_ * 977, 2, dNan_, stream_);
+                if (finer_.active) launchWatchdog(viewOf(finer_), finer_.layout, steps_ * 977, 4, dNan_, stream_);
                 (void)hipMemcpyAsync(hNan_, dNan_, sizeof(int), hipMemcpyDeviceToHost, stream_);
                 (void)hipEventRecord(evNan_, stream_);
                 nanPending_ = true;
@@ -879,8 +883,9 @@ public:
         p->last_step_ms = lastStepMs_;
         // Cell updates per coarse step: every coarse cell once, every fine cell
         // m times (lbm_solver.cpp:1705-1713).
-        const double updates = static_cast<double>(ncells_) +
-                               (fine_.active ? static_cast<double>(fine_.factor) * fine_.ncells : 0.0);
+        const double updates =
+            static_cast<double>(ncells_) + (fine_.active ? static_cast<double>(fine_.factor) * fine_.ncells : 0.0) +
+            (finer_.active ? static_cast<double>(fine_.factor) * finer_.factor * finer_.ncells : 0.0);
         if (lastStepMs_ > 0.0) p->mlups = updates / (lastStepMs_ * 1000.0);
         p->population_bytes_per_step = 2.0 * kQ * storageBytes(storage_) * updates;
         return FC_OK;
@@ -1277,34 +1282,41 @@ public:
         long long bytes = 0;
     };
 
-    RefineGeom fineGeom() const {
+    static RefineGeom geomOf(const Dims& parent, const FineLevel& l) {
         RefineGeom g;
-        g.box = fine_.box;
-        g.factor = fine_.factor;
-        g.coarse = dims_;
-        g.fine = fine_.dims;
+        g.box = l.box;
+        g.factor = l.factor;
+        g.coarse = parent;
+        g.fine = l.dims;
         return g;
     }
+    RefineGeom fineGeom() const { return geomOf(dims_, fine_); }
 
-    LatticeView fineView() const {
+    LatticeView viewOf(const FineLevel& lv) const {
         LatticeView l;
-        l.f = fine_.f;
+        l.f = lv.f;
         l.storage = storage_;
-        l.kind = fine_.kind;
-        l.dims = fine_.dims;
+        l.kind = lv.kind;
+        l.dims = lv.dims;
         return l;
     }
+    LatticeView fineView() const { return viewOf(fine_); }
 
+    static void freeLevel(FineLevel& l) {
+        hipRelease(l.f);
+        hipRelease(l.kind);
+        hipRelease(l.dForceCells);
+        hipRelease(l.dForceMasks);
+        hipRelease(l.dForcePartials);
+        hipRelease(l.dBandXY);
+        hipRelease(l.dBandIndex);
+        hipRelease(l.dBand);
+        l = FineLevel{};
+    }
+    // The finer level nests in the fine one: both go together.
     void freeFine() {
-        hipRelease(fine_.f);
-        hipRelease(fine_.kind);
-        hipRelease(fine_.dForceCells);
-        hipRelease(fine_.dForceMasks);
-        hipRelease(fine_.dForcePartials);
-        hipRelease(fine_.dBandXY);
-        hipRelease(fine_.dBandIndex);
-        hipRelease(fine_.dBand);
-        fine_ = FineLevel{};
+        freeLevel(finer_);
+        freeLevel(fine_);
     }
 
     // Fine kind bytes + force list from fine_.hostFlags, and whether the patch
@@ -1374,7 +1386,15 @@ public:
             if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
                                static_cast<float>(k) / static_cast<float>(m), neqScale, 0, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "fine interface fill failed");
+            // The fine step overwrites fine time t in place: keep the ring the
+            // finer shell fill interpolates from first.
+            if (finer_.active && launchBandSnapshot(fv, fine_.layout, geomOf(fine_.dims, finer_), finer_.dBandXY,
+                                                    finer_.nband, finer_.dBand, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "finer band snapshot failed");
             if (launchStep(fv, fine_.layout, fa, stream_) != 0) return fail(FC_ERR_DEVICE, "fine step failed");
+            if (finer_.active) {
+                if (int e = finerSubSteps(fa, 1 - fine_.layout); e != FC_OK) return e;
+            }
             fine_.layout = 1 - fine_.layout;
         }
         const float neqScaleInv = (static_cast<float>(m) * tauC) / tauF;
@@ -1385,45 +1405,37 @@ public:
         return FC_OK;
     }
 
-    int initRefinement(const fc_patch_box* b, int factor, const std::uint8_t* fineFlags) {
-        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
-        freeFine();
-        if (!b || !fineFlags) return fail(FC_ERR_INVALID, "null argument");
+    // Validation, band tables and allocation of a refined level inside a
+    // parent grid of `parent` dims. `clearance`: parent cells the box keeps
+    // from the parent's x / y faces (2 under the coarse grid, whose faces are
+    // inlet / outlet / slip cells; 3 under a fine grid, whose 2-cell
+    // Interface shell must stay out of the fill's ring).
+    int allocLevel(FineLevel& fl, const Dims& parent, const Scaling& parentScaling, const fc_patch_box* b,
+                   int factor, const std::uint8_t* flags, int clearance) {
+        if (!b || !flags) return fail(FC_ERR_INVALID, "null argument");
         if (factor < 2 || factor > 4) return fail(FC_ERR_INVALID, "refinement factor out of range (2..4)");
         const int w = b->x1 - b->x0, h = b->y1 - b->y0;
-        // One coarse cell more clearance from the domain faces than the
-        // reference asks (x0 >= 1): the fill's coarse ring may then never
-        // contain inlet / outlet / slip cells, whose in-place slots for
-        // outward directions do not exist.
-        if (w <= 8 || h <= 8 || b->x0 < 2 || b->y0 < 2 || b->x1 > dims_.nx - 2 || b->y1 > dims_.ny - 2)
+        if (w <= 8 || h <= 8 || b->x0 < clearance || b->y0 < clearance || b->x1 > parent.nx - clearance ||
+            b->y1 > parent.ny - clearance)
             return fail(FC_ERR_INVALID, "refinement patch box out of range");
-        if (periodicX_ || periodicY_) return fail(FC_ERR_INVALID, "refinement needs a non-periodic x / y domain");
-        FineLevel& fl = fine_;
         fl.factor = factor;
         fl.box = PatchBoxI{b->x0, b->y0, b->x1, b->y1};
-        fl.dims = Dims{factor * w, factor * h, factor * dims_.nz};
+        fl.dims = Dims{factor * w, factor * h, factor * parent.nz};
         fl.ncells = fl.dims.cells();
-        if (fl.ncells >= (1LL << 31)) {
-            freeFine();
-            return fail(FC_ERR_INVALID, "fine grid exceeds 2^31 cells");
-        }
-        fl.scaling = refinedScaling(scaling_, factor);
-        fl.hostFlags.assign(fineFlags, fineFlags + fl.ncells);
+        if (fl.ncells >= (1LL << 31)) return fail(FC_ERR_INVALID, "refined grid exceeds 2^31 cells");
+        fl.scaling = refinedScaling(parentScaling, factor);
+        fl.hostFlags.assign(flags, flags + fl.ncells);
         for (int z = 0; z < fl.dims.nz; ++z)
             for (int y = 0; y < fl.dims.ny; ++y)
                 for (int x = 0; x < fl.dims.nx; ++x) {
                     const std::uint8_t v = fl.hostFlags[static_cast<std::size_t>(
                         x + static_cast<long long>(fl.dims.nx) * (y + static_cast<long long>(fl.dims.ny) * z))];
-                    if (v != kFluid && v != kSolid && v != kInterface) {
-                        freeFine();
-                        return fail(FC_ERR_INVALID, "fine flags may only hold Fluid, Solid and Interface");
-                    }
-                    if (v == kFluid && (x == 0 || y == 0 || x == fl.dims.nx - 1 || y == fl.dims.ny - 1)) {
-                        freeFine();
-                        return fail(FC_ERR_INVALID, "fluid cell on a fine x / y face (missing Interface shell)");
-                    }
+                    if (v != kFluid && v != kSolid && v != kInterface)
+                        return fail(FC_ERR_INVALID, "refined flags may only hold Fluid, Solid and Interface");
+                    if (v == kFluid && (x == 0 || y == 0 || x == fl.dims.nx - 1 || y == fl.dims.ny - 1))
+                        return fail(FC_ERR_INVALID, "fluid cell on a refined x / y face (missing Interface shell)");
                 }
-        // Ring of the box expanded by one coarse cell that the shell fill reads.
+        // Ring of the box expanded by one parent cell that the shell fill reads.
         const int ew = w + 2, eh = h + 2;
         std::vector<int> bandXY, bandIndex(static_cast<std::size_t>(ew) * eh, -1);
         for (int ey = 0; ey < eh; ++ey)
@@ -1437,29 +1449,62 @@ public:
             }
         fl.nband = static_cast<int>(bandXY.size() / 2);
         const std::size_t fBytes = static_cast<std::size_t>(kQ) * fl.ncells * storageBytes(storage_);
-        const std::size_t bandBytes = static_cast<std::size_t>(kQ) * dims_.nz * fl.nband * sizeof(float);
+        const std::size_t bandBytes = static_cast<std::size_t>(kQ) * parent.nz * fl.nband * sizeof(float);
         int rc = hipAlloc(&fl.f, fBytes);
         rc |= hipAlloc(&fl.kind, static_cast<std::size_t>(fl.ncells));
         rc |= hipAlloc(&fl.dBandXY, bandXY.size() * sizeof(int));
         rc |= hipAlloc(&fl.dBandIndex, bandIndex.size() * sizeof(int));
         rc |= hipAlloc(&fl.dBand, bandBytes);
-        if (rc != FC_OK) {
-            freeFine();
-            return fail(FC_ERR_ALLOC, "fine level allocation failed (" + std::to_string(fBytes) + " bytes)");
-        }
+        if (rc != FC_OK)
+            return fail(FC_ERR_ALLOC, "refined level allocation failed (" + std::to_string(fBytes) + " bytes)");
         if (hipMemcpyAsync(fl.dBandXY, bandXY.data(), bandXY.size() * sizeof(int), hipMemcpyHostToDevice,
                            stream_) != hipSuccess ||
             hipMemcpyAsync(fl.dBandIndex, bandIndex.data(), bandIndex.size() * sizeof(int), hipMemcpyHostToDevice,
-                           stream_) != hipSuccess) {
-            freeFine();
+                           stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "band table upload failed");
-        }
         fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes);
+        return FC_OK;
+    }
+
+    // The m2 finer sub-steps of one fine sub-step plus the restriction into
+    // the fine state just written in layout fLayout (lbm_solver.cpp:1209-1267).
+    int finerSubSteps(const StepArgs& fineArgs, int fLayout) {
+        const int m2 = finer_.factor;
+        const float tauF = fineArgs.tau0;
+        const float tauG = fineTauFor(tauF, m2);
+        StepArgs ga = fineArgs;
+        ga.tau0 = tauG;
+        ga.writeMacro = 0;
+        const float neqScale = tauG / (static_cast<float>(m2) * tauF);
+        const RefineGeom g = geomOf(fine_.dims, finer_);
+        const LatticeView fv = viewOf(fine_), gv = viewOf(finer_);
+        for (int q = 0; q < m2; ++q) {
+            if (launchFineFill(fv, fLayout, finer_.dBand, finer_.dBandIndex, finer_.nband, gv, finer_.layout, g,
+                               static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "finer interface fill failed");
+            if (launchStep(gv, finer_.layout, ga, stream_) != 0) return fail(FC_ERR_DEVICE, "finer step failed");
+            finer_.layout = 1 - finer_.layout;
+        }
+        const float neqScaleInv = (static_cast<float>(m2) * tauF) / tauG;
+        if (launchFineRestrict(gv, finer_.layout, fv, fLayout, g, neqScaleInv, nullptr, nullptr, nullptr, nullptr,
+                               stream_) != 0)
+            return fail(FC_ERR_DEVICE, "finer restriction failed");
+        return FC_OK;
+    }
+
+    int initRefinement(const fc_patch_box* b, int factor, const std::uint8_t* fineFlags) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        freeFine();
+        if (periodicX_ || periodicY_) return fail(FC_ERR_INVALID, "refinement needs a non-periodic x / y domain");
+        if (int e = allocLevel(fine_, dims_, scaling_, b, factor, fineFlags, 2); e != FC_OK) {
+            freeFine();
+            return e;
+        }
         if (int e = uploadFineFlags(); e != FC_OK) {
             freeFine();
             return e;
         }
-        fl.active = true;
+        fine_.active = true;
         if (int e = seedFine(); e != FC_OK) {
             freeFine();
             return e;
@@ -1468,6 +1513,70 @@ public:
             freeFine();
             return fail(FC_ERR_DEVICE, "fine level setup failed");
         }
+        return FC_OK;
+    }
+
+    // ---- nested third level (initFinerRefinement, lbm_solver.h:223-225) ----
+    // Box in fine cells; the finer level covers the fine grid's full z at
+    // factor m2 and couples to the fine level as the fine level couples to
+    // the coarse one (fill from the fine band at t and the live fine array at
+    // t+1 for each of its m2 sub-steps, restriction into the fine t+1 state),
+    // nested inside every fine sub-step (lbm_solver.cpp:1209-1267).
+    int initFinerRefinement(const fc_patch_box* b, int factor, const std::uint8_t* finerFlags) {
+        if (!initialized_ || !fine_.active) return fail(FC_ERR_STATE, "no active refinement level to nest in");
+        freeLevel(finer_);
+        if (int e = allocLevel(finer_, fine_.dims, fine_.scaling, b, factor, finerFlags, 3); e != FC_OK) {
+            freeLevel(finer_);
+            return e;
+        }
+        KindBuild kb;
+        buildKinds(finer_.hostFlags, finer_.dims, false, false, kb);
+        if (hipMemcpyAsync(finer_.kind, kb.kind.data(), kb.kind.size(), hipMemcpyHostToDevice, stream_) !=
+            hipSuccess) {
+            freeLevel(finer_);
+            return fail(FC_ERR_DEVICE, "finer flag upload failed");
+        }
+        finer_.active = true;
+        if (int e = seedFiner(); e != FC_OK) {
+            freeLevel(finer_);
+            return e;
+        }
+        if (hipStreamSynchronize(stream_) != hipSuccess) {
+            freeLevel(finer_);
+            return fail(FC_ERR_DEVICE, "finer level setup failed");
+        }
+        return FC_OK;
+    }
+
+    // seedFineFromCoarse one rung down: rest equilibrium, then the full-volume
+    // fill from the current fine state, in layout L.
+    int seedFiner() {
+        finer_.layout = kLayoutL;
+        const float tauF = fineTauFor(effectiveTau(), fine_.factor);
+        const float tauG = fineTauFor(tauF, finer_.factor);
+        const float neqScale = tauG / (static_cast<float>(finer_.factor) * tauF);
+        if (launchInitEquilibrium(viewOf(finer_), 0.0f, stream_) != 0 ||
+            launchFineFill(viewOf(fine_), fine_.layout, nullptr, nullptr, 0, viewOf(finer_), kLayoutL,
+                           geomOf(fine_.dims, finer_), 0.0f, neqScale, 1, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "finer level seeding failed");
+        return FC_OK;
+    }
+
+    void shutdownFiner() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        freeLevel(finer_);
+    }
+
+    int finerInfo(fc_refinement_info* out) const {
+        *out = fc_refinement_info{};
+        if (!finer_.active) return FC_OK;
+        out->active = 1;
+        out->factor = finer_.factor;
+        out->box = fc_patch_box{finer_.box.x0, finer_.box.y0, finer_.box.x1, finer_.box.y1};
+        out->fine_dims = fc_dims{finer_.dims.nx, finer_.dims.ny, finer_.dims.nz};
+        out->fine_scaling = toC(finer_.scaling);
+        out->device_bytes = static_cast<double>(finer_.bytes);
+        out->forces_from_fine = 0;
         return FC_OK;
     }
 
@@ -1491,34 +1600,39 @@ public:
         return FC_OK;
     }
 
-    int downloadFineF(void* host, bool asF64) {
-        if (!fine_.active) return fail(FC_ERR_STATE, "no active refinement level");
+    int downloadFineF(void* host, bool asF64) { return downloadLevelF(fine_, host, asF64); }
+    int uploadFineF(const void* host, bool isF64) { return uploadLevelF(fine_, host, isF64); }
+    int downloadFinerF(void* host, bool asF64) { return downloadLevelF(finer_, host, asF64); }
+    int uploadFinerF(const void* host, bool isF64) { return uploadLevelF(finer_, host, isF64); }
+
+    int downloadLevelF(FineLevel& lv, void* host, bool asF64) {
+        if (!lv.active) return fail(FC_ERR_STATE, "no active refinement level");
         if (!host) return fail(FC_ERR_INVALID, "null output");
-        const std::size_t bytes = static_cast<std::size_t>(kQ) * fine_.ncells * (asF64 ? 8 : 4);
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * lv.ncells * (asF64 ? 8 : 4);
         void* d = nullptr;
         if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "download scratch");
         int rc = FC_OK;
-        if (launchDownloadCanonical(fineView(), fine_.layout, d, asF64 ? 1 : 0, stream_) != 0 ||
+        if (launchDownloadCanonical(viewOf(lv), lv.layout, d, asF64 ? 1 : 0, stream_) != 0 ||
             hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
             hipStreamSynchronize(stream_) != hipSuccess)
-            rc = fail(FC_ERR_DEVICE, "fine download failed");
+            rc = fail(FC_ERR_DEVICE, "refined level download failed");
         hipRelease(d);
         return rc;
     }
 
-    int uploadFineF(const void* host, bool isF64) {
-        if (!fine_.active) return fail(FC_ERR_STATE, "no active refinement level");
+    int uploadLevelF(FineLevel& lv, const void* host, bool isF64) {
+        if (!lv.active) return fail(FC_ERR_STATE, "no active refinement level");
         if (!host) return fail(FC_ERR_INVALID, "null input");
-        const std::size_t bytes = static_cast<std::size_t>(kQ) * fine_.ncells * (isF64 ? 8 : 4);
+        const std::size_t bytes = static_cast<std::size_t>(kQ) * lv.ncells * (isF64 ? 8 : 4);
         void* d = nullptr;
         if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "upload scratch");
         int rc = FC_OK;
         if (hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream_) != hipSuccess ||
-            launchUploadCanonical(fineView(), d, isF64 ? 1 : 0, stream_) != 0 ||
+            launchUploadCanonical(viewOf(lv), d, isF64 ? 1 : 0, stream_) != 0 ||
             hipStreamSynchronize(stream_) != hipSuccess)
-            rc = fail(FC_ERR_DEVICE, "fine upload failed");
+            rc = fail(FC_ERR_DEVICE, "refined level upload failed");
         hipRelease(d);
-        if (rc == FC_OK) fine_.layout = kLayoutL;
+        if (rc == FC_OK) lv.layout = kLayoutL;
         return rc;
     }
 
@@ -1629,6 +1743,7 @@ private:
     std::uint8_t* kind_ = nullptr;
     void* feed_ = nullptr;
     FineLevel fine_;
+    FineLevel finer_;  // nested third level (inside fine_)
     float *rho_ = nullptr, *u_ = nullptr, *v_ = nullptr, *w_ = nullptr;
     double* dForce_ = nullptr;
     double* dMass_ = nullptr;
@@ -1892,6 +2007,27 @@ int fc_set_refined_flags(fc_solver* s, const uint8_t* fine_flags) {
 }
 int fc_get_refinement_info(fc_solver* s, fc_refinement_info* out) {
     return (s && out) ? s->impl.refinementInfo(out) : FC_ERR_INVALID;
+}
+int fc_init_finer_refinement(fc_solver* s, const fc_patch_box* box, int factor, const uint8_t* finer_flags,
+                             char* err, int errlen) {
+    if (!s) return FC_ERR_INVALID;
+    const int rc = s->impl.initFinerRefinement(box, factor, finer_flags);
+    if (rc != FC_OK) fc::copyErr(s->impl.error, err, errlen);
+    return rc;
+}
+int fc_shutdown_finer_refinement(fc_solver* s) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.shutdownFiner();
+    return FC_OK;
+}
+int fc_get_finer_refinement_info(fc_solver* s, fc_refinement_info* out) {
+    return (s && out) ? s->impl.finerInfo(out) : FC_ERR_INVALID;
+}
+int fc_download_finer_f(fc_solver* s, void* host, int as_f64) {
+    return s ? s->impl.downloadFinerF(host, as_f64 != 0) : FC_ERR_INVALID;
+}
+int fc_upload_finer_f(fc_solver* s, const void* host, int is_f64) {
+    return s ? s->impl.uploadFinerF(host, is_f64 != 0) : FC_ERR_INVALID;
 }
 int fc_download_fine_f(fc_solver* s, void* host, int as_f64) {
     return s ? s->impl.downloadFineF(host, as_f64 != 0) : FC_ERR_INVALID;

@@ -332,6 +332,40 @@ class LBMSolver:
         check(self._lib.fc_upload_fine_f(self._h, a.ctypes.data_as(C.c_void_p), 1 if is64 else 0),
               "fc_upload_fine_f")
 
+    def init_finer_refinement(self, box, factor: int, finer_flags: np.ndarray) -> None:
+        """LBMSolver::initFinerRefinement: nested patch, box in fine cells."""
+        b = FcPatchBox(*[int(v) for v in box])
+        gf = np.ascontiguousarray(finer_flags, dtype=np.uint8).reshape(-1)
+        err = C.create_string_buffer(512)
+        rc = self._lib.fc_init_finer_refinement(self._h, C.byref(b), int(factor), _u8(gf), err, 512)
+        check(rc, "fc_init_finer_refinement", err.value.decode())
+
+    def shutdown_finer_refinement(self) -> None:
+        check(self._lib.fc_shutdown_finer_refinement(self._h), "fc_shutdown_finer_refinement")
+
+    def finer_refinement_info(self) -> dict:
+        r = _abi.FcRefinementInfo()
+        check(self._lib.fc_get_finer_refinement_info(self._h, C.byref(r)), "fc_get_finer_refinement_info")
+        return {"active": bool(r.active), "factor": r.factor,
+                "box": (r.box.x0, r.box.y0, r.box.x1, r.box.y1),
+                "finer_dims": (r.fine_dims.nx, r.fine_dims.ny, r.fine_dims.nz),
+                "finer_tau": r.fine_scaling.tau, "device_bytes": r.device_bytes}
+
+    def download_finer_f(self, f64: bool = True) -> np.ndarray:
+        info = self.finer_refinement_info()
+        n = Q * int(np.prod(info["finer_dims"]))
+        out = np.empty(n, dtype=np.float64 if f64 else np.float32)
+        check(self._lib.fc_download_finer_f(self._h, out.ctypes.data_as(C.c_void_p), 1 if f64 else 0),
+              "fc_download_finer_f")
+        return out
+
+    def upload_finer_f(self, f: np.ndarray) -> None:
+        a = np.ascontiguousarray(f)
+        is64 = a.dtype == np.float64
+        a = a if is64 else a.astype(np.float32)
+        check(self._lib.fc_upload_finer_f(self._h, a.ctypes.data_as(C.c_void_p), 1 if is64 else 0),
+              "fc_upload_finer_f")
+
     def set_step_variant(self, variant: str):
         """"auto" (library default), "single" (one cell per lane), "pair"
         (two x-adjacent cells per lane where the grid allows it) or "pipe"

@@ -274,6 +274,20 @@ int fc_init_refinement(fc_solver* s, const fc_patch_box* box, int factor, const 
 int fc_shutdown_refinement(fc_solver* s);                          /* shutdownRefinement, :199 */
 int fc_set_refined_flags(fc_solver* s, const uint8_t* fine_flags);  /* setRefinedFlags, :205 */
 int fc_get_refinement_info(fc_solver* s, fc_refinement_info* out);  /* refinementInfo, :246 */
+/* Nested third level (initFinerRefinement, lbm_solver.h:223-225; sequencing
+ * lbm_solver.cpp:1209-1267): a patch inside the fine grid (box in fine cells,
+ * at least 3 fine cells from the fine x / y faces), factor 2..4, full fine z.
+ * Flags hold Fluid, Solid and Interface (an Interface shell on its x / y
+ * faces, as for the fine level). Each fine sub-step then runs factor finer
+ * sub-steps and the finer-to-fine restriction. Re-initialising or shutting
+ * down the fine level drops it. The info struct reports the finer box,
+ * dims and scaling in its fine_* fields. */
+int fc_init_finer_refinement(fc_solver* s, const fc_patch_box* box, int factor, const uint8_t* finer_flags,
+                             char* err, int errlen);
+int fc_shutdown_finer_refinement(fc_solver* s);
+int fc_get_finer_refinement_info(fc_solver* s, fc_refinement_info* out);
+int fc_download_finer_f(fc_solver* s, void* host, int as_f64);
+int fc_upload_finer_f(fc_solver* s, const void* host, int is_f64);
 /* Fine-level populations in canonical SoA (no reference counterpart: the
  * reference exposes the padded fine buffer only to its tests). */
 int fc_download_fine_f(fc_solver* s, void* host, int as_f64);

@@ -806,19 +806,37 @@ void fineToCoarseRestrict(const T* ff, const uint8_t* fFlags, const Grid& fg, T*
 // fineTauFor, lbm_refine.cuh:96-98 (FP32 as the reference computes it).
 float fineTauFor(float tauC, int m) { return (float)m * (tauC - 0.5f) + 0.5f; }
 
-// The refined step sequence of stepN (lbm_solver.cpp:1144-1290, coarse +
-// fine levels): coarse step t -> t+1; for each of the m fine sub-steps the
-// Interface fill from coarse (t, t+1) at weight k/m, then the fine step;
-// then the restriction of the fine state into the coarse t+1 buffer. Taus and
-// rescale factors are formed in FP32 as the reference does.
+// Optional third level (the nested "finer" patch): grid, flags, buffers and
+// box in fine cells.
+template <class T>
+struct FinerLevel {
+    T* gA = nullptr;
+    T* gB = nullptr;
+    const uint8_t* flags = nullptr;
+    Grid grid{};
+    PBox box{};
+    int m2 = 0;
+};
+
+// The refined step sequence of stepN (lbm_solver.cpp:1144-1290): coarse step
+// t -> t+1; for each of the m fine sub-steps the Interface fill from coarse
+// (t, t+1) at weight k/m, then the fine step; with a finer level, m2 finer
+// sub-steps follow each fine step (fill from fine t / t+1 at q/m2, finer
+// step) and the restriction of the finer state into the fine t+1 buffer,
+// before the fine swap (lbm_solver.cpp:1209-1267); then the restriction of
+// the fine state into the coarse t+1 buffer. Taus and rescale factors are
+// formed in FP32 as the reference does.
 template <class T>
 void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* fB,
                 const uint8_t* fFlags, const Grid& fg, const PBox& box, int m, int nsteps,
-                const T* taus, const T* uins, T lam, T pre, T* rho, T* u, T* v, T* w) {
+                const T* taus, const T* uins, T lam, T pre, T* rho, T* u, T* v, T* w,
+                const FinerLevel<T>* finer = nullptr) {
     T* cs = cA;
     T* cd = cB;
     T* fs = fA;
     T* fd = fB;
+    T* gs = finer ? finer->gA : nullptr;
+    T* gd = finer ? finer->gB : nullptr;
     for (int i = 0; i < nsteps; ++i) {
         const float tauC = (float)taus[i];
         const float tauF = fineTauFor(tauC, m);
@@ -833,6 +851,22 @@ void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* f
                                 (T)((float)k / (float)m), (T)neqScale, false);
             streamCollide<T, T>(fs, fd, fFlags, fg, af, (T*)nullptr, (T*)nullptr, (T*)nullptr,
                                 (T*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            if (finer) {
+                const int m2 = finer->m2;
+                const float tauG = fineTauFor(tauF, m2);
+                StepArgs ag{(T)tauG, lam, pre, uins[i], 0};
+                const float neqG = tauG / ((float)m2 * tauF);
+                for (int q = 0; q < m2; ++q) {
+                    coarseToFineFill<T>(fs, fd, fFlags, fg, finer->flags, gs, finer->grid, finer->box, m2,
+                                        (T)((float)q / (float)m2), (T)neqG, false);
+                    streamCollide<T, T>(gs, gd, finer->flags, finer->grid, ag, (T*)nullptr, (T*)nullptr,
+                                        (T*)nullptr, (T*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                    std::swap(gs, gd);
+                }
+                const float neqGInv = ((float)m2 * tauF) / tauG;
+                fineToCoarseRestrict<T, T>(gs, finer->flags, finer->grid, fd, fFlags, fg, finer->box, m2,
+                                           (T)neqGInv, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
+            }
             std::swap(fs, fd);
         }
         const float neqScaleInv = ((float)m * tauC) / tauF;
@@ -843,6 +877,8 @@ void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* f
     }
     if (cs != cA) std::memcpy(cA, cs, sizeof(T) * (size_t)kQ * (size_t)cg.ncells());
     if (fs != fA) std::memcpy(fA, fs, sizeof(T) * (size_t)kQ * (size_t)fg.ncells());
+    if (finer && gs != finer->gA)
+        std::memcpy(finer->gA, gs, sizeof(T) * (size_t)kQ * (size_t)finer->grid.ncells());
 }
 
 
@@ -1187,6 +1223,26 @@ void orc_run_refined_f32(float* cA, float* cB, const uint8_t* cFlags, int cnx, i
     const Grid fg = makeGrid(m * (x1 - x0), m * (y1 - y0), m * cnz, 0, 0);
     runRefined<float>(cA, cB, cFlags, cg, fA, fB, fFlags, fg, PBox{x0, y0, x1, y1}, m, nsteps,
                       taus, uins, lam, pre, rho, u, v, w);
+}
+
+// Three levels: coarse, fine (box in coarse cells, factor m) and finer (box
+// gx0..gy1 in fine cells, factor m2); g* buffers hold the finer populations.
+void orc_run_refined3_f64(double* cA, double* cB, const uint8_t* cFlags, int cnx, int cny, int cnz,
+                          double* fA, double* fB, const uint8_t* fFlags, int x0, int y0, int x1, int y1,
+                          int m, double* gA, double* gB, const uint8_t* gFlags, int gx0, int gy0, int gx1,
+                          int gy1, int m2, int nsteps, const double* taus, const double* uins, double lam,
+                          double pre, double* rho, double* u, double* v, double* w) {
+    const Grid cg = makeGrid(cnx, cny, cnz, 0, 0);
+    const Grid fg = makeGrid(m * (x1 - x0), m * (y1 - y0), m * cnz, 0, 0);
+    FinerLevel<double> g;
+    g.gA = gA;
+    g.gB = gB;
+    g.flags = gFlags;
+    g.grid = makeGrid(m2 * (gx1 - gx0), m2 * (gy1 - gy0), m2 * fg.nz, 0, 0);
+    g.box = PBox{gx0, gy0, gx1, gy1};
+    g.m2 = m2;
+    runRefined<double>(cA, cB, cFlags, cg, fA, fB, fFlags, fg, PBox{x0, y0, x1, y1}, m, nsteps, taus, uins,
+                       lam, pre, rho, u, v, w, &g);
 }
 
 uint16_t orc_f16c_encode(float v) { return f16cEncode(v); }

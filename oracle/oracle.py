@@ -93,6 +93,9 @@ def lib():
         L.orc_run_refined_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                           _u8p] + [C.c_int] * 6 + [_dp, _dp, C.c_double,
                                                                    C.c_double] + [_dp] * 4
+        L.orc_run_refined3_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                           _u8p] + [C.c_int] * 5 + [_dp, _dp, _u8p] + [C.c_int] * 5 + \
+            [C.c_int, _dp, _dp, C.c_double, C.c_double] + [_dp] * 4
         L.orc_run_refined_f32.argtypes = [_fp, _fp, _u8p, C.c_int, C.c_int, C.c_int, _fp, _fp,
                                           _u8p] + [C.c_int] * 6 + [_fp, _fp, C.c_float,
                                                                    C.c_float] + [_fp] * 4
@@ -439,6 +442,26 @@ def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.
        _p(ff, pt), _p(ftmp, pt), _p(np.ascontiguousarray(fflags, np.uint8), _u8p), *box, m,
        len(taus), _p(taus, pt), _p(uins, pt), lam, float(smag_prefactor(cs)),
        *[_p(a, pt) for a in m4])
+    return tuple(m4)
+
+
+def run_refined3(cf, ff, gf, cflags, fflags, gflags, cdims, box, m, gbox, m2, taus, uins,
+                 lam=3.0 / 16.0, cs=0.1733):
+    """Three-level refined steps (coarse, fine, nested finer patch; FP64),
+    lbm_solver.cpp:1144-1290. cf / ff / gf are advanced in place."""
+    nx, ny, nz = cdims
+    n = nx * ny * nz
+    for a in (cf, ff, gf):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    taus = np.ascontiguousarray(taus, dtype=np.float64)
+    uins = np.ascontiguousarray(uins, dtype=np.float64)
+    m4 = [np.zeros(n) for _ in range(4)]
+    u8 = lambda a: _p(np.ascontiguousarray(a, np.uint8), _u8p)
+    lib().orc_run_refined3_f64(_p(cf, _dp), _p(np.empty_like(cf), _dp), u8(cflags), nx, ny, nz,
+                               _p(ff, _dp), _p(np.empty_like(ff), _dp), u8(fflags), *box, m,
+                               _p(gf, _dp), _p(np.empty_like(gf), _dp), u8(gflags), *gbox, m2,
+                               len(taus), _p(taus, _dp), _p(uins, _dp), lam, float(smag_prefactor(cs)),
+                               *[_p(a, _dp) for a in m4])
     return tuple(m4)
 
 

@@ -170,3 +170,87 @@ def test_refined_run_stays_finite_and_counts_updates(gpu):
     n_f = int(np.prod(s.refinement_info()["fine_dims"]))
     assert p.population_bytes_per_step == pytest.approx(76.0 * (n_c + 2 * n_f))
     s.close()
+
+
+# ---- nested third level --------------------------------------------------------
+def test_three_level_uniform_freestream_stays_uniform(gpu):
+    """Body-free tunnel with a factor-2 fine patch and a factor-2 finer patch
+    nested in it (the reference's m4_finer 3-level gate): all three levels
+    stay at the inlet's uniform equilibrium."""
+    say("case: 3-level uniform freestream 96x48x8, fine (20,12,60,36), finer (20,10,60,38), f32, 40 steps")
+    dims, box, m, u0 = (96, 48, 8), (20, 12, 60, 36), 2, 0.05
+    gbox, m2 = (20, 10, 60, 38), 2
+    flags = fc.build_boundary_flags(*dims)
+    sc = fc.scaling_for_tau(0.56, u0, 32)
+    feq = orc.equilibrium(1.0, u0, 0.0, 0.0)
+    n = dims[0] * dims[1] * dims[2]
+    case = cases.Case("uniform", dims, flags, sc, cs=0.1, ramp=False,
+                      f0=np.repeat(feq[:, None], n, axis=1).reshape(-1))
+    fd = orc.fine_dims(box, dims, m)
+    fine = np.zeros(fd[0] * fd[1] * fd[2], np.uint8)
+    fc.stamp_interface_shell(*fd, fine)
+    gd = orc.fine_dims(gbox, fd, m2)
+    finer = np.zeros(gd[0] * gd[1] * gd[2], np.uint8)
+    fc.stamp_interface_shell(*gd, finer)
+    s = cases.make_solver(case, "f32")
+    s.init_refinement(box, m, fine)
+    s.init_finer_refinement(gbox, m2, finer)
+    info = s.finer_refinement_info()
+    assert info["active"] and info["finer_dims"] == gd
+    s.step_n(40)
+    s.synchronize()
+    rho, u, v, w = s.read_macro()
+    fl = flags == fc.FLUID
+    assert np.abs(u[fl] - u0).max() < 2e-6 and np.abs(v[fl]).max() < 2e-6
+    for got, ff in ((s.download_fine_f(f64=True), fine), (s.download_finer_f(f64=True), finer)):
+        g = got.reshape(Q, -1)[:, ff == fc.FLUID]
+        say(f"  level max |f - feq| {np.abs(g - feq[:, None]).max():.2e}")
+        # FP32 rounding of the coupling (2.3e-7 measured on the fine level);
+        # a wrong slot, layout or time level gives O(1e-3)
+        assert np.abs(g - feq[:, None]).max() < 1e-6
+    assert s.perf_stats().population_bytes_per_step > 0
+    assert not s.nan_detected()
+    s.close()
+
+
+def nested_case():
+    case, box, fine = refined_case()
+    fd = orc.fine_dims(box, case.dims, 2)
+    # finer box around the foil's front half, 3+ fine cells inside the fine shell
+    gbox = (6, max(3, fd[1] // 2 - 14), min(fd[0] - 3, 6 + 40), min(fd[1] - 3, fd[1] // 2 + 14))
+    finer = cases.finer_patch(case, box, 2, gbox, 2)
+    return case, box, fine, gbox, finer
+
+
+@pytest.mark.parametrize("steps", [1, 6])
+def test_three_level_step_parity(gpu, steps):
+    """Coupled three-level steps on the refined tunnel (foil in all three
+    levels) against the FP64 oracle's restatement of the nested sequencing,
+    from the same coarse, fine and finer state (FP64 storage; the coupling
+    kernels compute in FP32)."""
+    case, box, fine, gbox, finer = nested_case()
+    say(f"case: 3-level tunnel 160x64x8, fine {box}, finer {gbox}, f64 steps={steps}")
+    s = cases.make_solver(case, "f64")
+    s.step_n(10)
+    s.init_refinement(box, 2, fine)
+    s.step_n(3)
+    s.init_finer_refinement(gbox, 2, finer)
+    assert (finer == fc.SOLID).any()
+    cf = s.download_f(f64=True)
+    ff = s.download_fine_f(f64=True)
+    gf = s.download_finer_f(f64=True)
+    start = s.step_count()
+    s.step_n(steps)
+    s.synchronize()
+    got = [s.download_f(f64=True), s.download_fine_f(f64=True), s.download_finer_f(f64=True)]
+    sc = case.scaling
+    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, case.dims[0], start + steps, ramp=case.ramp)
+    orc.run_refined3(cf, ff, gf, case.flags, fine, finer, case.dims, box, 2, gbox, 2,
+                     taus[start:].astype(np.float64), uins[start:].astype(np.float64), cs=case.cs)
+    for name, g, w, fl in (("coarse", got[0], cf, case.flags), ("fine", got[1], ff, fine),
+                           ("finer", got[2], gf, finer)):
+        live = fl == fc.FLUID
+        e = rel_l2(g.reshape(Q, -1)[:, live], w.reshape(Q, -1)[:, live])
+        say(f"  {name} populations rel L2 {e:.3e}")
+        assert e < 1e-6, name
+    s.close()

@@ -190,3 +190,34 @@ def test_uniform_freestream_stays_uniform_through_coupled_steps():
     assert np.abs(v[fluid]).max() < 1e-13 and np.abs(rho[fluid] - 1.0).max() < 1e-13
     g = ff.reshape(Q, -1)[:, fflags == fc.FLUID]
     assert np.abs(g - feq[:, None]).max() < 1e-14
+
+
+def test_uniform_freestream_stays_uniform_through_three_levels():
+    """The nested third level (reference m4_finer's 3-level uniform-freestream
+    gate): coarse tunnel without a body, a factor-2 fine patch and inside it a
+    factor-2 finer patch; 10 coupled steps (coarse + 2 x (fine + 2 finer
+    sub-steps + finer restriction) + fine restriction) keep all three levels
+    at the uniform equilibrium to rounding."""
+    cdims, box, m = (40, 24, 4), (10, 5, 30, 19), 2
+    nx, ny, nz = cdims
+    cflags = fc.build_boundary_flags(*cdims)
+    fd = orc.fine_dims(box, cdims, m)
+    fflags = np.zeros(fd[0] * fd[1] * fd[2], np.uint8)
+    fc.stamp_interface_shell(*fd, fflags)
+    gbox, m2 = (10, 8, 30, 20), 2  # in fine cells, 3+ cells inside the fine shell
+    gd = orc.fine_dims(gbox, fd, m2)
+    gflags = np.zeros(gd[0] * gd[1] * gd[2], np.uint8)
+    fc.stamp_interface_shell(*gd, gflags)
+    u0, tau = 0.05, 0.56
+    feq = orc.equilibrium(1.0, u0, 0.0, 0.0)
+    cf = np.repeat(feq[:, None], nx * ny * nz, axis=1).reshape(-1).copy()
+    ff = orc.seed_fine(cf, cflags, cdims, fflags, box, m, tau)
+    gf = orc.seed_fine(ff, fflags, fd, gflags, gbox, m2, orc.fine_tau_for(tau, m))
+    steps = 10
+    rho, u, v, w = orc.run_refined3(cf, ff, gf, cflags, fflags, gflags, cdims, box, m, gbox, m2,
+                                    np.full(steps, tau), np.full(steps, u0), cs=0.1)
+    fluid = cflags == fc.FLUID
+    assert np.abs(u[fluid] - u0).max() < 1e-13 and np.abs(rho[fluid] - 1.0).max() < 1e-13
+    for arr, fl in ((ff, fflags), (gf, gflags)):
+        g = arr.reshape(Q, -1)[:, fl == fc.FLUID]
+        assert g.shape[1] > 0 and np.abs(g - feq[:, None]).max() < 1e-13
