@@ -17,6 +17,9 @@ FC_ERR_INVALID = 1
 FC_ERR_ALLOC = 2
 FC_ERR_DEVICE = 3
 FC_ERR_STATE = 4
+FC_ERR_IO = 5
+
+FC_SOLVER_VERSION = 1
 
 FC_STORAGE_F32 = 0
 FC_STORAGE_F16 = 1
@@ -107,6 +110,11 @@ class FcVaneSlab(C.Structure):
     _fields_ = [("center", C.c_float * 3), ("n_axis", C.c_float * 3), ("d_axis", C.c_float * 3),
                 ("b_axis", C.c_float * 3), ("half_height", C.c_float),
                 ("half_length", C.c_float), ("half_thick", C.c_float)]
+
+
+class FcSnapshotKey(C.Structure):
+    _fields_ = [("geometry_id", C.c_char_p), ("aoa_bucket", C.c_int), ("u_lat", C.c_float),
+                ("dims", FcDims), ("solver_version", C.c_int)]
 
 
 class FcInitOptions(C.Structure):
@@ -242,6 +250,45 @@ SIGNATURES = {
     "fc_particles_write": (C.c_int, [_vp, _fp]),
     "fc_particles_count": (C.c_int, [_vp]),
     "fc_particles_set_params": (C.c_int, [_vp, C.c_float, C.c_float]),
+    "fc_apply_edited_flags": (C.c_int, [_vp, _u8p]),
+    "fc_last_edit_count": (C.c_longlong, [_vp]),
+    "fc_restore_from_macroscopic": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fc_snapshot_aoa_bucket": (C.c_int, [C.c_float]),
+    "fc_snapshot_key_hash": (C.c_uint64, [C.POINTER(FcSnapshotKey)]),
+    "fc_snapshot_key_string": (C.c_int, [C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_snapshot_key_file_name": (C.c_int, [C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_snapshot_key_equal": (C.c_int, [C.POINTER(FcSnapshotKey), C.POINTER(FcSnapshotKey)]),
+    "fc_hash_flags": (C.c_uint64, [_u8p, C.c_uint64]),
+    "fc_vram_snapshot_create": (_vp, []),
+    "fc_vram_snapshot_destroy": (None, [_vp]),
+    "fc_vram_snapshot_capture": (C.c_int, [_vp, _vp, C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_vram_snapshot_restore": (C.c_int, [_vp, _vp]),
+    "fc_vram_snapshot_has_data": (C.c_int, [_vp]),
+    "fc_vram_snapshot_get_key": (C.c_int, [_vp, C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_vram_snapshot_flags_hash": (C.c_uint64, [_vp]),
+    "fc_vram_snapshot_bytes": (C.c_longlong, [_vp]),
+    "fc_vram_snapshot_release": (None, [_vp]),
+    "fc_compact_snapshot_create": (_vp, []),
+    "fc_compact_snapshot_destroy": (None, [_vp]),
+    "fc_compact_snapshot_capture": (C.c_int, [_vp, _vp, C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_compact_snapshot_set": (C.c_int, [_vp, C.POINTER(FcSnapshotKey), C.c_uint64, C.POINTER(FcScaling),
+                                          _fp, _fp, _fp, _fp, C.c_longlong]),
+    "fc_compact_snapshot_restore": (C.c_int, [_vp, _vp]),
+    "fc_compact_snapshot_save": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_int]),
+    "fc_compact_snapshot_load": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_int]),
+    "fc_compact_snapshot_has_data": (C.c_int, [_vp]),
+    "fc_compact_snapshot_get_key": (C.c_int, [_vp, C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_compact_snapshot_flags_hash": (C.c_uint64, [_vp]),
+    "fc_compact_snapshot_scaling": (C.c_int, [_vp, C.POINTER(FcScaling)]),
+    "fc_compact_snapshot_fields": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fc_snapshot_cache_open": (_vp, [C.c_char_p, C.c_uint64]),
+    "fc_snapshot_cache_close": (None, [_vp]),
+    "fc_snapshot_cache_find": (C.c_int, [_vp, C.POINTER(FcSnapshotKey), C.c_char_p, C.c_int]),
+    "fc_snapshot_cache_store": (C.c_int, [_vp, _vp, C.c_char_p, C.c_int]),
+    "fc_snapshot_cache_store_async": (C.c_int, [_vp, _vp]),
+    "fc_snapshot_cache_flush": (C.c_longlong, [_vp]),
+    "fc_snapshot_cache_set_budget": (C.c_int, [_vp, C.c_uint64]),
+    "fc_snapshot_cache_used_bytes": (C.c_uint64, [_vp]),
     "fc_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

@@ -72,6 +72,23 @@ int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, i
                    int* flag, void* stream);
 int launchFillFloat(float* dst, long long count, float value, void* stream);
 
+// ---- warm restart -----------------------------------------------------------
+// f = feq(rho, u) of the view's macro arrays in layout L (solid cells at
+// rest) and both outlet feed-plane halves: the compact-snapshot restore.
+int launchInitFromMacro(const LatticeView& lat, void* stream);
+// One cell of a flag edit. kind 1: newly solid, every population set to
+// f = 0. kind 2: newly fluid, f = feq(mean rho, mean u) over the neighbours
+// c + e_q whose bit q is set in nbMask (the host sets a bit when that
+// neighbour is Fluid under the new flags, not edited itself, inside x / y,
+// and has every population addressable in the current layout); no bit set
+// gives the rest state. The populations are written in layout `layout`.
+struct FlagEdit {
+    long long cell;
+    std::uint32_t nbMask;
+    std::int32_t kind;
+};
+int launchFlagEdits(const LatticeView& lat, int layout, const FlagEdit* edits, int count, void* stream);
+
 // ---- lattice groups (one batched step launch for several lattices) --------
 inline constexpr int kGroupMaxMembers = 64;
 // Size of one device record of a (step, member) pair; built on the host.

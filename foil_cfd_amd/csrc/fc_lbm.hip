@@ -664,6 +664,157 @@ __global__ void __launch_bounds__(kBlock) downloadKernel(KArgs a, int layout, CT
     }
 }
 
+// Compact-snapshot restore (reference initFromMacroKernel, lbm_core.cu:162-198):
+// f = feq(rho, u) from the macro arrays, solid cells at rest, written in
+// layout L plus both halves of the outlet feed planes.
+template <class S>
+__global__ void __launch_bounds__(kBlock) macroInitKernel(KArgs a) {
+    using T = typename S::T;
+    using C = typename S::C;
+    RowPos p;
+    if (!rowPos<false, false>(a, p)) return;
+    T* const f = static_cast<T*>(a.f);
+    const unsigned c = nb(p, 0, 0, 0);
+    const std::uint8_t k = a.kind[c];
+    const bool solid = (k & kKindFlagMask) == kSolid;
+    const C r = solid ? C(1) : C(a.rho[c]);
+    const C ux = solid ? C(0) : C(a.u[c]);
+    const C uy = solid ? C(0) : C(a.v[c]);
+    const C uz = solid ? C(0) : C(a.w[c]);
+    const C K = C(S::K);
+    const C usq = C(1.5) * (ux * ux + uy * uy + uz * uz);
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+    const unsigned yz = static_cast<unsigned>(p.y + a.ny * p.z);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const C h = heqOf<C>(q, K, r - C(1), r, ux, uy, uz, usq);
+        S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
+        if (k & kKindFeed) {
+            T* fd = static_cast<T*>(a.feed);
+            S::store(fd + q * plane + yz, h);
+            S::store(fd + (kQ + q) * plane + yz, h);
+        }
+    }
+}
+
+// Cell index of (x + ex, y + ey, z + ez): z wraps, x / y wrap when periodic,
+// -1 when the position leaves a non-periodic x / y range.
+__device__ inline long long shiftedCell(const KArgs& a, int x, int y, int z, int ex, int ey, int ez,
+                                        bool px, bool py) {
+    int xx = x + ex, yy = y + ey, zz = z + ez;
+    if (xx < 0 || xx >= a.nx) {
+        if (!px) return -1;
+        xx = xx < 0 ? xx + a.nx : xx - a.nx;
+    }
+    if (yy < 0 || yy >= a.ny) {
+        if (!py) return -1;
+        yy = yy < 0 ? yy + a.ny : yy - a.ny;
+    }
+    zz = zz < 0 ? zz + a.nz : (zz >= a.nz ? zz - a.nz : zz);
+    return xx + static_cast<long long>(a.nx) * (yy + static_cast<long long>(a.ny) * zz);
+}
+
+// Storage slot of population q of cell (x, y, z) in layout L (in place,
+// direction reversed) or S (at the downstream neighbour), -1 when layout S
+// puts it outside a non-periodic domain.
+__device__ inline long long popSlot(const KArgs& a, int layout, int q, int x, int y, int z, bool px,
+                                    bool py) {
+    if (layout == kLayoutL)
+        return static_cast<long long>(kOpp[q]) * a.ncells + x +
+               static_cast<long long>(a.nx) * (y + static_cast<long long>(a.ny) * z);
+    const long long n = shiftedCell(a, x, y, z, kCx[q], kCy[q], kCz[q], px, py);
+    return n < 0 ? -1 : static_cast<long long>(q) * a.ncells + n;
+}
+
+// Warm-restart flag-edit fixup (reference applyFlagEditsKernel,
+// lbm_core.cu:644-717), one thread per edited cell of a host-built list. A
+// newly solid cell gets f = 0; a newly fluid cell the equilibrium at the mean
+// of rho and u = j / rho over its valid neighbours (rest state when none).
+// Every neighbour read is an unedited cell and every write a slot of an
+// edited one, so the threads never touch the same slot.
+template <class S>
+__global__ void __launch_bounds__(kBlock) flagEditKernel(KArgs a, int layout, int px, int py,
+                                                         const FlagEdit* edits, int count) {
+    using T = typename S::T;
+    using C = typename S::C;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const FlagEdit e = edits[i];
+    if (e.cell < 0 || e.cell >= a.ncells) return;
+    T* const f = static_cast<T*>(a.f);
+    const long long nxny = static_cast<long long>(a.nx) * a.ny;
+    const int z = static_cast<int>(e.cell / nxny);
+    const long long rem = e.cell - z * nxny;
+    const int y = static_cast<int>(rem / a.nx);
+    const int x = static_cast<int>(rem - static_cast<long long>(y) * a.nx);
+    const C K = C(S::K);
+    const C invK = C(1) / K;
+    C hOut[kQ];
+    if (e.kind == 1) {
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) hOut[q] = -K * wOf<C>(q);
+    } else {
+        C rho = 0, ux = 0, uy = 0, uz = 0;
+        int found = 0;
+        for (int q = 1; q < kQ; ++q) {
+            if (!(e.nbMask & (1u << q))) continue;
+            const long long n = shiftedCell(a, x, y, z, kCx[q], kCy[q], kCz[q], false, false);
+            if (n < 0) continue;
+            const int nz = static_cast<int>(n / nxny);
+            const long long nr = n - nz * nxny;
+            const int ny = static_cast<int>(nr / a.nx);
+            const int nx = static_cast<int>(nr - static_cast<long long>(ny) * a.nx);
+            C r = 0, jx = 0, jy = 0, jz = 0;
+            bool ok = true;
+#pragma unroll
+            for (int p = 0; p < kQ; ++p) {
+                const long long s = popSlot(a, layout, p, nx, ny, nz, px != 0, py != 0);
+                if (s < 0) {
+                    ok = false;
+                    continue;
+                }
+                const C fq = wOf<C>(p) + S::load(f + s) * invK;
+                r += fq;
+                jx += C(kCx[p]) * fq;
+                jy += C(kCy[p]) * fq;
+                jz += C(kCz[p]) * fq;
+            }
+            if (!ok) continue;
+            rho += r;
+            ux += jx / r;
+            uy += jy / r;
+            uz += jz / r;
+            ++found;
+        }
+        if (found > 0) {
+            const C inv = C(1) / C(found);
+            rho *= inv;
+            ux *= inv;
+            uy *= inv;
+            uz *= inv;
+        } else {
+            rho = C(1);
+            ux = uy = uz = C(0);
+        }
+        const C usq = C(1.5) * (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) hOut[q] = heqOf<C>(q, K, rho - C(1), rho, ux, uy, uz, usq);
+    }
+    const bool feed = (a.kind[e.cell] & kKindFeed) != 0;
+    T* const fd = static_cast<T*>(a.feed);
+    const long long plane = static_cast<long long>(a.ny) * a.nz;
+    const long long yz = y + static_cast<long long>(a.ny) * z;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const long long s = popSlot(a, layout, q, x, y, z, px != 0, py != 0);
+        if (s >= 0) S::store(f + s, hOut[q]);
+        if (feed) {
+            S::store(fd + q * plane + yz, hOut[q]);
+            S::store(fd + (kQ + q) * plane + yz, hOut[q]);
+        }
+    }
+}
+
 // Momentum exchange (reference forceReductionKernel<WM>, lbm_core.cu:573-637):
 // F = sum over fluid->solid links of c_q (2 f*_q - 6 w_q c_q.u_w), per-link
 // terms in the compute type, sums in FP64 in a fixed order.
@@ -960,6 +1111,27 @@ struct DownloadOp {
 };
 
 template <class S>
+struct MacroInitOp {
+    static int run(const LatticeView& lat, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        if (!a.rho || !a.u || !a.v || !a.w) return static_cast<int>(hipErrorInvalidValue);
+        macroInitKernel<S><<<rowGrid(a), kBlock, 0, st>>>(a);
+        return lastError();
+    }
+};
+
+template <class S>
+struct FlagEditOp {
+    static int run(const LatticeView& lat, int layout, const FlagEdit* edits, int count, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        if (count <= 0) return 0;
+        const int blocks = (count + kBlock - 1) / kBlock;
+        flagEditKernel<S><<<blocks, kBlock, 0, st>>>(a, layout, lat.periodicX, lat.periodicY, edits, count);
+        return lastError();
+    }
+};
+
+template <class S>
 struct ForceOp {
     static int run(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks,
                    int count, double* partials, int slots, double* out3, hipStream_t st) {
@@ -1017,6 +1189,14 @@ int launchDownloadCanonical(const LatticeView& lat, int layout, void* canon, int
                             void* stream) {
     return byStorage<DownloadOp>(lat.storage, lat, layout, canon, canonIsF64,
                                  static_cast<hipStream_t>(stream));
+}
+
+int launchInitFromMacro(const LatticeView& lat, void* stream) {
+    return byStorage<MacroInitOp>(lat.storage, lat, static_cast<hipStream_t>(stream));
+}
+
+int launchFlagEdits(const LatticeView& lat, int layout, const FlagEdit* edits, int count, void* stream) {
+    return byStorage<FlagEditOp>(lat.storage, lat, layout, edits, count, static_cast<hipStream_t>(stream));
 }
 
 int launchStep(const LatticeView& lat, int layout, const StepArgs& args, void* stream) {

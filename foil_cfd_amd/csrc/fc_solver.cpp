@@ -866,6 +866,127 @@ public:
                                static_cast<long long>(kWarmGateFlowThroughs * flowThroughSteps());
     }
 
+    // Warm-restart flag edit (lbm_solver.cpp:1086-1126): cells whose solid
+    // state flips are fixed in place (newly solid: f = 0; newly fluid:
+    // equilibrium at the mean rho / u of the neighbours that were fluid
+    // before the edit), every other population is left as it is. The field is
+    // developed, so no viscosity ramp; the force EMA restarts behind a gate.
+    int applyEditedFlags(const std::uint8_t* flagsIn) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!flagsIn) return fail(FC_ERR_INVALID, "flags is null");
+        std::vector<std::uint8_t> nf(flagsIn, flagsIn + ncells_);
+        std::string why;
+        if (!validateFlags(nf, why)) return fail(FC_ERR_INVALID, why);
+        std::vector<std::uint8_t> edit(static_cast<std::size_t>(ncells_), 0);
+        std::vector<FlagEdit> list;
+        for (long long i = 0; i < ncells_; ++i) {
+            const bool was = hostFlags_[static_cast<std::size_t>(i)] == kSolid;
+            const bool is = nf[static_cast<std::size_t>(i)] == kSolid;
+            if (was == is) continue;
+            edit[static_cast<std::size_t>(i)] = is ? 1 : 2;
+            list.push_back(FlagEdit{i, 0u, is ? 1 : 2});
+        }
+        // A neighbour's populations are all addressable unless layout S
+        // pushed some of them across a non-periodic x / y face.
+        const bool layoutS = layout_ != kLayoutL;
+        const long long nxny = static_cast<long long>(dims_.nx) * dims_.ny;
+        for (FlagEdit& e : list) {
+            if (e.kind != 2) continue;
+            const int z = static_cast<int>(e.cell / nxny);
+            const long long rem = e.cell - z * nxny;
+            const int y = static_cast<int>(rem / dims_.nx);
+            const int x = static_cast<int>(rem - static_cast<long long>(y) * dims_.nx);
+            for (int q = 1; q < kQ; ++q) {
+                const int xn = x + kCx[q], yn = y + kCy[q];
+                if (xn < 0 || xn >= dims_.nx || yn < 0 || yn >= dims_.ny) continue;
+                const int zn = (z + kCz[q] + dims_.nz) % dims_.nz;
+                const std::size_t n = at(xn, yn, zn);
+                if (nf[n] != kFluid || edit[n] != 0) continue;
+                if (layoutS && ((!periodicX_ && (xn == 0 || xn == dims_.nx - 1)) ||
+                                (!periodicY_ && (yn == 0 || yn == dims_.ny - 1))))
+                    continue;
+                e.nbMask |= 1u << q;
+            }
+        }
+        hostFlags_.swap(nf);
+        if (int e = uploadFlags(); e != FC_OK) return e;
+        if (!list.empty()) {
+            FlagEdit* d = nullptr;
+            const std::size_t bytes = list.size() * sizeof(FlagEdit);
+            if (hipAlloc(&d, bytes) != FC_OK) return fail(FC_ERR_ALLOC, "flag edit list");
+            int rc = FC_OK;
+            if (hipMemcpyAsync(d, list.data(), bytes, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                launchFlagEdits(view(), layout_, d, static_cast<int>(list.size()), stream_) != 0 ||
+                hipStreamSynchronize(stream_) != hipSuccess)
+                rc = fail(FC_ERR_DEVICE, "flag edit fixup failed");
+            hipRelease(d);
+            if (rc != FC_OK) return rc;
+        }
+        rampActive_ = false;
+        emaSeeded_ = false;
+        forcePending_ = false;
+        stepsAtLastForceFold_ = steps_;
+        gateOpenAtSteps_ = steps_ + static_cast<long long>(kWarmGateFlowThroughs * flowThroughSteps());
+        lastEditCount_ = static_cast<long long>(list.size());
+        return FC_OK;
+    }
+    long long lastEditCount() const { return lastEditCount_; }
+
+    // Compact-snapshot restore (lbm_solver.cpp:1773-1803): the macro fields
+    // go into the solver's own arrays, f = feq(rho, u) in layout L. A fine
+    // patch (and the level nested in it) is re-seeded from the restored
+    // coarse field.
+    int restoreFromMacroscopic(const float* r, const float* u, const float* v, const float* w) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!r || !u || !v || !w) return fail(FC_ERR_INVALID, "null macroscopic field");
+        const std::size_t bytes = static_cast<std::size_t>(ncells_) * sizeof(float);
+        const std::pair<const float*, float*> ups[] = {{r, rho_}, {u, u_}, {v, v_}, {w, w_}};
+        for (const auto& up : ups)
+            if (hipMemcpyAsync(up.second, up.first, bytes, hipMemcpyHostToDevice, stream_) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "macroscopic upload failed");
+        if (launchInitFromMacro(view(), stream_) != 0) return fail(FC_ERR_DEVICE, "equilibrium re-init failed");
+        layout_ = kLayoutL;
+        if (fine_.active) {
+            if (int e = seedFine(); e != FC_OK) return e;
+        }
+        if (finer_.active) {
+            if (int e = seedFiner(); e != FC_OK) return e;
+        }
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "equilibrium re-init failed");
+        notifySnapshotRestored(false);
+        return FC_OK;
+    }
+
+    // Exact state of the populations: the device regions an exact snapshot
+    // copies (coarse f, outlet feed planes, then the fine and finer levels'
+    // f when active) and the layout parity of each level.
+    void stateRegions(std::vector<std::pair<void*, std::size_t>>& out, int layouts[3]) const {
+        const std::size_t sb = storageBytes(storage_);
+        const std::size_t plane = static_cast<std::size_t>(dims_.ny) * dims_.nz;
+        out.clear();
+        out.emplace_back(f_, static_cast<std::size_t>(kQ) * ncells_ * sb);
+        out.emplace_back(feed_, 2 * kQ * plane * sb);
+        layouts[0] = layout_;
+        layouts[1] = layouts[2] = -1;
+        if (fine_.active) {
+            out.emplace_back(fine_.f, static_cast<std::size_t>(kQ) * fine_.ncells * sb);
+            layouts[1] = fine_.layout;
+        }
+        if (finer_.active) {
+            out.emplace_back(finer_.f, static_cast<std::size_t>(kQ) * finer_.ncells * sb);
+            layouts[2] = finer_.layout;
+        }
+    }
+    // After an exact snapshot's copies landed on this solver's stream.
+    int stateRestored(const int layouts[3]) {
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "snapshot restore failed");
+        layout_ = layouts[0];
+        if (fine_.active) fine_.layout = layouts[1];
+        if (finer_.active) finer_.layout = layouts[2];
+        notifySnapshotRestored(true);
+        return FC_OK;
+    }
+
     int nanDetected() {
         pollAsync();
         return nanLatched_ ? 1 : 0;
@@ -1776,6 +1897,7 @@ private:
     bool nanLatched_ = false, nanPending_ = false;
     int nanKind_ = 0;
     int stepsSinceWatchdog_ = 0;
+    long long lastEditCount_ = 0;
     bool timingPending_ = false;
     bool groupTimed_ = false;
     int timingBatchN_ = 0;
@@ -1801,6 +1923,15 @@ void copyErr(const std::string& msg, char* err, int errlen) {
 struct fc_solver {
     fc::Solver impl;
 };
+
+namespace fc {
+bool solverReady(fc_solver* s) { return s && s->impl.initialized(); }
+void solverStateRegions(fc_solver* s, std::vector<std::pair<void*, std::size_t>>& out, int layouts[3]) {
+    s->impl.stateRegions(out, layouts);
+}
+int solverStateRestored(fc_solver* s, const int layouts[3]) { return s->impl.stateRestored(layouts); }
+void* solverStream(fc_solver* s) { return s->impl.stream(); }
+}  // namespace fc
 
 using fc::Solver;
 
@@ -2075,6 +2206,17 @@ int fc_notify_snapshot_restored(fc_solver* s, int full_state) {
     if (!s) return FC_ERR_INVALID;
     s->impl.notifySnapshotRestored(full_state != 0);
     return FC_OK;
+}
+
+int fc_apply_edited_flags(fc_solver* s, const uint8_t* flags) {
+    return s ? s->impl.applyEditedFlags(flags) : FC_ERR_INVALID;
+}
+
+long long fc_last_edit_count(fc_solver* s) { return s ? s->impl.lastEditCount() : 0; }
+
+int fc_restore_from_macroscopic(fc_solver* s, const float* rho, const float* u, const float* v,
+                                const float* w) {
+    return s ? s->impl.restoreFromMacroscopic(rho, u, v, w) : FC_ERR_INVALID;
 }
 
 int fc_nan_detected(fc_solver* s) { return s ? s->impl.nanDetected() : 0; }

@@ -286,6 +286,28 @@ class LBMSolver:
         f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
         check(self._lib.fc_set_flags(self._h, _u8(f)), "fc_set_flags")
 
+    def apply_edited_flags(self, flags) -> int:
+        """LBMSolver::applyEditedFlags (warm-restart flag edit); returns the
+        number of cells fixed."""
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        if f.size != self.ncells:
+            raise ValueError("flag field size does not match grid dimensions")
+        check(self._lib.fc_apply_edited_flags(self._h, _u8(f)), "fc_apply_edited_flags", self.last_error())
+        return int(self._lib.fc_last_edit_count(self._h))
+
+    def restore_from_macroscopic(self, rho, u, v, w) -> None:
+        """LBMSolver::restoreFromMacroscopic: f = feq(rho, u), compact restore."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (rho, u, v, w)]
+        if any(a.size != self.ncells for a in arrs):
+            raise ValueError("macroscopic field size does not match grid dimensions")
+        check(self._lib.fc_restore_from_macroscopic(self._h, *[_f32(a) for a in arrs]),
+              "fc_restore_from_macroscopic", self.last_error())
+
+    def last_error(self) -> str:
+        buf = C.create_string_buffer(512)
+        self._lib.fc_last_error(self._h, buf, 512)
+        return buf.value.decode(errors="replace")
+
     def set_startup_ramp_enabled(self, on: bool):
         check(self._lib.fc_set_startup_ramp(self._h, 1 if on else 0), "fc_set_startup_ramp")
 
@@ -384,6 +406,12 @@ class LBMSolver:
         check(self._lib.fc_synchronize(self._h), "fc_synchronize")
 
     # readouts ---------------------------------------------------------------
+    def read_flags(self) -> np.ndarray:
+        """Host copy of the cell flags (LBMSolver::hostFlags)."""
+        out = np.empty(self.ncells, dtype=np.uint8)
+        check(self._lib.fc_read_flags(self._h, _u8(out)), "fc_read_flags")
+        return out
+
     def forces(self) -> ForceReadout:
         r = FcForceReadout()
         check(self._lib.fc_forces(self._h, C.byref(r)), "fc_forces")

@@ -29,6 +29,7 @@ extern "C" {
 #define FC_ERR_ALLOC 2     /* device / pinned allocation failed */
 #define FC_ERR_DEVICE 3    /* a HIP runtime call or launch failed */
 #define FC_ERR_STATE 4     /* call not valid in the current state (e.g. before init) */
+#define FC_ERR_IO 5        /* snapshot file could not be written / read or was rejected */
 
 /* ---- population storage formats --------------------------------------- */
 #define FC_STORAGE_F32 0   /* FP32 shifted deviations f - w */
@@ -235,7 +236,21 @@ int fc_download_f(fc_solver* s, void* host, int as_f64);
 /* Replace the state (tests' uploadF, tests/lbm_test_util.h:116-147) and run
  * the exact-restore bookkeeping (notifySnapshotRestored(true), :465). */
 int fc_upload_f(fc_solver* s, const void* host, int is_f64);
-int fc_notify_snapshot_restored(fc_solver* s, int full_state);
+int fc_notify_snapshot_restored(fc_solver* s, int full_state);  /* :1750-1771 */
+/* applyEditedFlags (lbm_solver.h:174, lbm_solver.cpp:1086-1126; fixup kernel
+ * lbm_core.cu:644-717): new flags on a developed field. Cells whose solid
+ * state flips are fixed in place (newly solid: f = 0; newly fluid: f =
+ * feq(mean rho, mean u) over the neighbours that are Fluid and unedited,
+ * rest state when none); all other populations are kept. No viscosity ramp;
+ * the force gate reopens after half a flow-through. */
+int fc_apply_edited_flags(fc_solver* s, const uint8_t* flags);
+long long fc_last_edit_count(fc_solver* s);  /* cells fixed by the last call */
+/* restoreFromMacroscopic (lbm_solver.h:478, lbm_solver.cpp:1773-1803): the
+ * four fields (dims cells each) become the solver's macro arrays and f =
+ * feq(rho, u) (solid cells at rest), then notifySnapshotRestored(false). An
+ * active fine patch (and the level nested in it) is re-seeded from it. */
+int fc_restore_from_macroscopic(fc_solver* s, const float* rho, const float* u, const float* v,
+                                const float* w);
 int fc_nan_detected(fc_solver* s);          /* nanDetected, :411 (1 = latched) */
 int fc_nan_kind(fc_solver* s);              /* watchdog verdict of the trip */
 int fc_get_perf_stats(fc_solver* s, fc_perf_stats* out);   /* perfStats, :419 */
@@ -457,6 +472,90 @@ int fc_particles_write(fc_solver* s, const float* pos4);
 int fc_particles_count(fc_solver* s);
 /* ParticleAdvectParams ageRate / inletSpawnFraction (defaults 0.002 / 0.3). */
 int fc_particles_set_params(fc_solver* s, float age_rate, float inlet_fraction);
+
+/* ---- warm-start snapshots (src/sim/snapshot.h, snapshot.cpp) ------------- */
+#define FC_SOLVER_VERSION 1  /* kSolverVersion, snapshot.h:22 */
+/* SnapshotKey (snapshot.h:33-64). geometry_id: "naca:2412", "dat:<file>",
+ * "stl:<16-hex hash>" (NUL-terminated, at most 4096 bytes for files). */
+typedef struct fc_snapshot_key {
+    const char* geometry_id;
+    int aoa_bucket;       /* fc_snapshot_aoa_bucket(aoa_deg) = lround(2 aoa) */
+    float u_lat;
+    fc_dims dims;
+    int solver_version;   /* FC_SOLVER_VERSION */
+} fc_snapshot_key;
+int fc_snapshot_aoa_bucket(float aoa_deg);                        /* bucketForAoA, :45 */
+uint64_t fc_snapshot_key_hash(const fc_snapshot_key* key);        /* hash, snapshot.cpp:89 */
+int fc_snapshot_key_string(const fc_snapshot_key* key, char* out, int cap);  /* toString, :101 */
+/* "<16 hex digits of the hash>.fcs" (cacheFileName, snapshot.cpp:76-81) */
+int fc_snapshot_key_file_name(const fc_snapshot_key* key, char* out, int cap);
+int fc_snapshot_key_equal(const fc_snapshot_key* a, const fc_snapshot_key* b);  /* operator==, :52 */
+uint64_t fc_hash_flags(const uint8_t* flags, uint64_t count);     /* hashFlags, FNV-1a 64 */
+
+/* VramSnapshot (snapshot.h:77-122): a device-resident exact copy of the
+ * populations (coarse f, outlet feed planes, and the fine / finer levels'
+ * f when active) with their layout parity. Capture and restore are
+ * device-to-device copies on the solver's stream; restore needs the same
+ * dims, storage and level shapes and runs notifySnapshotRestored(true).
+ * Each object is its own slot. */
+typedef struct fc_vram_snapshot fc_vram_snapshot;
+fc_vram_snapshot* fc_vram_snapshot_create(void);
+void fc_vram_snapshot_destroy(fc_vram_snapshot* snap);
+int fc_vram_snapshot_capture(fc_vram_snapshot* snap, fc_solver* s, const fc_snapshot_key* key,
+                             char* err, int errlen);
+int fc_vram_snapshot_restore(fc_vram_snapshot* snap, fc_solver* s);
+int fc_vram_snapshot_has_data(const fc_vram_snapshot* snap);
+/* The stored key; geometry_id points into id_buf (id_cap bytes). */
+int fc_vram_snapshot_get_key(const fc_vram_snapshot* snap, fc_snapshot_key* out, char* id_buf, int id_cap);
+uint64_t fc_vram_snapshot_flags_hash(const fc_vram_snapshot* snap);
+long long fc_vram_snapshot_bytes(const fc_vram_snapshot* snap);
+void fc_vram_snapshot_release(fc_vram_snapshot* snap);
+
+/* CompactSnapshot (snapshot.h:124-180): host copy of rho, u, v, w (the macro
+ * fields of the last stepped batch), key, flags hash and scaling; restore =
+ * fc_restore_from_macroscopic. Files are the FCS1 container (snapshot.cpp:
+ * 287-424): u32 magic 0x31534346, u32 version 1, u32 id length + id bytes,
+ * i32 aoa bucket, f32 u_lat, i32 nx ny nz, i32 solver version, u64 flags
+ * hash, scaling (f32 chord_m airspeed_ms nu_m2s, i32 chord_cells, f32 u_lat
+ * dx dt nu_lat_target nu_lat tau, u8 tau_clamped, f32 re_target
+ * re_effective), u32 compression 0, u64 cells, then the four FP32 arrays;
+ * little-endian, unpadded. */
+typedef struct fc_compact_snapshot fc_compact_snapshot;
+fc_compact_snapshot* fc_compact_snapshot_create(void);
+void fc_compact_snapshot_destroy(fc_compact_snapshot* snap);
+int fc_compact_snapshot_capture(fc_compact_snapshot* snap, fc_solver* s, const fc_snapshot_key* key,
+                                char* err, int errlen);
+/* Fill from host arrays (a field computed elsewhere; no solver needed). */
+int fc_compact_snapshot_set(fc_compact_snapshot* snap, const fc_snapshot_key* key, uint64_t flags_hash,
+                            const fc_scaling* scaling, const float* rho, const float* u, const float* v,
+                            const float* w, long long ncells);
+int fc_compact_snapshot_restore(fc_compact_snapshot* snap, fc_solver* s);
+int fc_compact_snapshot_save(const fc_compact_snapshot* snap, const char* path, char* err, int errlen);
+int fc_compact_snapshot_load(fc_compact_snapshot* snap, const char* path, char* err, int errlen);
+int fc_compact_snapshot_has_data(const fc_compact_snapshot* snap);
+int fc_compact_snapshot_get_key(const fc_compact_snapshot* snap, fc_snapshot_key* out, char* id_buf,
+                                int id_cap);
+uint64_t fc_compact_snapshot_flags_hash(const fc_compact_snapshot* snap);
+int fc_compact_snapshot_scaling(const fc_compact_snapshot* snap, fc_scaling* out);
+/* Copies of the held fields (cells floats each; any pointer may be NULL). */
+int fc_compact_snapshot_fields(const fc_compact_snapshot* snap, float* rho, float* u, float* v, float* w);
+
+/* DiskSnapshotCache (snapshot.h:182-219): a directory of FCS1 files named by
+ * key hash, evicted oldest-modification-first past a byte budget (the file
+ * just stored is never evicted); a hit refreshes the file's time. */
+typedef struct fc_snapshot_cache fc_snapshot_cache;
+fc_snapshot_cache* fc_snapshot_cache_open(const char* dir, uint64_t budget_bytes);
+void fc_snapshot_cache_close(fc_snapshot_cache* c);  /* queued writes finish first */
+/* 1 = hit (path written to path_out when non-NULL), 0 = miss, -1 = bad argument. */
+int fc_snapshot_cache_find(fc_snapshot_cache* c, const fc_snapshot_key* key, char* path_out, int cap);
+int fc_snapshot_cache_store(fc_snapshot_cache* c, const fc_compact_snapshot* snap, char* err, int errlen);
+/* storeAsync: written by the cache's worker thread; the snapshot's data is
+ * shared with the queue, so the handle may be reused or destroyed at once. */
+int fc_snapshot_cache_store_async(fc_snapshot_cache* c, const fc_compact_snapshot* snap);
+/* Waits until the queue is empty; returns the count of failed async stores. */
+long long fc_snapshot_cache_flush(fc_snapshot_cache* c);
+int fc_snapshot_cache_set_budget(fc_snapshot_cache* c, uint64_t budget_bytes);
+uint64_t fc_snapshot_cache_used_bytes(const fc_snapshot_cache* c);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Per-launch HIP-event timing of the stream-collide kernel (on the solver's

@@ -505,3 +505,155 @@ def particles_advect(pos, u, v, w, flags, dims, dt, seed, age_rate=0.002, inlet_
     arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w)]
     lib().orc_particles_advect(_p(pos, _fp), len(pos), *[_p(a, _fp) for a in arrs],
                                _p(fl, _u8p), *dims, dt, age_rate, inlet_fraction, seed)
+
+
+# ---- warm restart (src/sim/snapshot.cpp, lbm_core.cu:162-198, :644-717) ------
+# Restated in numpy / pure Python; the layout and the hash chain follow the
+# reference's writer and SnapshotKey::hash field by field.
+FNV_OFFSET = 0xCBF29CE484222325
+FNV_PRIME = 0x100000001B3
+FCS_MAGIC = 0x31534346
+FCS_FORMAT = 1
+SOLVER_VERSION = 1
+SOLID_FLAG = 1
+FLUID_FLAG = 0
+
+
+def fnv1a64(data: bytes, h: int = FNV_OFFSET) -> int:
+    """FNV-1a 64 (snapshot.cpp:33-45)."""
+    for b in data:
+        h ^= b
+        h = (h * FNV_PRIME) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def snapshot_key_hash(geometry_id: str, aoa_bucket: int, u_lat: float, dims, version=SOLVER_VERSION) -> int:
+    """SnapshotKey::hash (snapshot.cpp:89-99): id bytes, then the native
+    little-endian bytes of aoaBucket, u_lat, nx, ny, nz, solverVersion."""
+    import struct
+    h = fnv1a64(geometry_id.encode())
+    for fmt, v in (("<i", aoa_bucket), ("<f", u_lat), ("<i", dims[0]), ("<i", dims[1]), ("<i", dims[2]),
+                   ("<i", version)):
+        h = fnv1a64(struct.pack(fmt, v), h)
+    return h
+
+
+def snapshot_key_string(geometry_id: str, aoa_bucket: int, u_lat: float, dims, version=SOLVER_VERSION) -> str:
+    """SnapshotKey::toString (snapshot.cpp:101-112)."""
+    u32 = float(np.float32(u_lat))
+    s = "%s_a%.1f_u%.4f_%dx%dx%d_v%d" % (geometry_id, float(np.float32(aoa_bucket) * np.float32(0.5)), u32,
+                                         dims[0], dims[1], dims[2], version)
+    return s.replace(":", "-").replace("\\", "-").replace("/", "-")
+
+
+# FCS1 scaling block, in file order (snapshot.cpp:316-329).
+FCS_SCALING = [("chord_m", "<f"), ("airspeed_ms", "<f"), ("nu_m2s", "<f"), ("chord_cells", "<i"),
+               ("u_lat", "<f"), ("dx", "<f"), ("dt", "<f"), ("nu_lat_target", "<f"), ("nu_lat", "<f"),
+               ("tau", "<f"), ("tau_clamped", "<B"), ("re_target", "<f"), ("re_effective", "<f")]
+
+
+def fcs1_write(path, geometry_id, aoa_bucket, u_lat, dims, flags_hash, scaling: dict, rho, u, v, w,
+               version=SOLVER_VERSION, magic=FCS_MAGIC, fmt=FCS_FORMAT, compression=0, ncells=None):
+    """CompactSnapshot::saveToFile (snapshot.cpp:287-347); the keyword
+    overrides write deliberately malformed files for the loader's rejections."""
+    import struct
+    gid = geometry_id.encode()
+    out = bytearray()
+    out += struct.pack("<III", magic, fmt, len(gid)) + gid
+    out += struct.pack("<ifiiii", aoa_bucket, u_lat, dims[0], dims[1], dims[2], version)
+    out += struct.pack("<Q", flags_hash)
+    for name, f in FCS_SCALING:
+        out += struct.pack(f, scaling[name])
+    n = len(rho) if ncells is None else ncells
+    out += struct.pack("<IQ", compression, n)
+    for a in (rho, u, v, w):
+        out += np.ascontiguousarray(a, dtype="<f4").tobytes()
+    with open(path, "wb") as fh:
+        fh.write(bytes(out))
+
+
+def fcs1_read(path):
+    """Parse an FCS1 file into a dict (layout of snapshot.cpp:287-347)."""
+    import struct
+    b = open(path, "rb").read()
+    o = 0
+
+    def take(fmt):
+        nonlocal o
+        v = struct.unpack_from(fmt, b, o)
+        o += struct.calcsize(fmt)
+        return v if len(v) > 1 else v[0]
+
+    d = {}
+    d["magic"], d["format"], idlen = take("<III")
+    d["geometry_id"] = b[o:o + idlen].decode()
+    o += idlen
+    d["aoa_bucket"], d["u_lat"], nx, ny, nz, d["solver_version"] = take("<ifiiii")
+    d["dims"] = (nx, ny, nz)
+    d["flags_hash"] = take("<Q")
+    d["scaling"] = {name: take(f) for name, f in FCS_SCALING}
+    d["compression"], n = take("<IQ")
+    d["ncells"] = n
+    for name in ("rho", "u", "v", "w"):
+        d[name] = np.frombuffer(b, dtype="<f4", count=n, offset=o).copy()
+        o += 4 * n
+    d["trailing_bytes"] = len(b) - o
+    return d
+
+
+def equilibrium_np(rho, ux, uy, uz):
+    """Second-order D3Q19 equilibrium, (Q, n) FP64 (lbm_core.cu:65-73)."""
+    rho, ux, uy, uz = (np.asarray(a, np.float64) for a in (rho, ux, uy, uz))
+    usq = 1.5 * (ux * ux + uy * uy + uz * uz)
+    cu = 3.0 * (CX[:, None] * ux + CY[:, None] * uy + CZ[:, None] * uz)
+    return W[:, None] * rho * (1.0 + cu + 0.5 * cu * cu - usq)
+
+
+def init_from_macro(flags, rho, u, v, w):
+    """Compact-snapshot restore (initFromMacroKernel, lbm_core.cu:162-198):
+    canonical f (Q, n) = feq(rho, u), solid cells at rest."""
+    fl = np.asarray(flags).reshape(-1)
+    solid = fl == SOLID_FLAG
+    r = np.where(solid, 1.0, np.asarray(rho, np.float64))
+    a, b, c = (np.where(solid, 0.0, np.asarray(x, np.float64)) for x in (u, v, w))
+    return equilibrium_np(r, a, b, c)
+
+
+def apply_flag_edits(f, dims, old_flags, new_flags):
+    """Flag-edit fixup on canonical f (Q, n), returned as a new array
+    (applyEditedFlags + applyFlagEditsKernel, lbm_solver.cpp:1086-1126,
+    lbm_core.cu:644-717): solid-state flips only; newly solid -> f = 0;
+    newly fluid -> feq at the mean rho and mean u = j / rho over the 18
+    neighbours (x / y not wrapped, z wrapped) that are Fluid under the new
+    flags and not edited; rest state when none."""
+    nx, ny, nz = dims
+    f = np.asarray(f, np.float64).reshape(Q, -1)
+    old = np.asarray(old_flags).reshape(-1)
+    new = np.asarray(new_flags).reshape(-1)
+    was, now = old == SOLID_FLAG, new == SOLID_FLAG
+    edit = np.zeros(old.size, np.uint8)
+    edit[~was & now] = 1
+    edit[was & ~now] = 2
+    out = f.copy()
+    for c in np.nonzero(edit)[0]:
+        if edit[c] == 1:
+            out[:, c] = 0.0
+            continue
+        z, rem = divmod(int(c), nx * ny)
+        y, x = divmod(rem, nx)
+        acc = np.zeros(4)
+        found = 0
+        for q in range(1, Q):
+            xn, yn, zn = x + CX[q], y + CY[q], (z + CZ[q]) % nz
+            if xn < 0 or xn >= nx or yn < 0 or yn >= ny:
+                continue
+            n = xn + nx * (yn + ny * zn)
+            if new[n] != FLUID_FLAG or edit[n] != 0:
+                continue
+            fn = f[:, n]
+            r = fn.sum()
+            acc += (r, (CX * fn).sum() / r, (CY * fn).sum() / r, (CZ * fn).sum() / r)
+            found += 1
+        rho, ux, uy, uz = acc / found if found else (1.0, 0.0, 0.0, 0.0)
+        out[:, c] = equilibrium_np([rho], [ux], [uy], [uz])[:, 0]
+    return out
