@@ -117,6 +117,24 @@ class FcSnapshotKey(C.Structure):
                 ("dims", FcDims), ("solver_version", C.c_int)]
 
 
+class FcDelta99Sample(C.Structure):
+    _fields_ = [("x_c", C.c_float), ("delta99_c", C.c_float), ("ue_edge", C.c_float), ("valid", C.c_int)]
+
+
+class FcPlaneCirculation(C.Structure):
+    _fields_ = [("pos", C.c_float), ("neg", C.c_float)]
+
+
+class FcBandCirculation(C.Structure):
+    _fields_ = [("zone", FcPlaneCirculation), ("ambient", FcPlaneCirculation), ("zone_cols", C.c_int),
+                ("ambient_cols", C.c_int)]
+
+
+class FcVGAuditReadout(C.Structure):
+    _fields_ = [("valid", C.c_int), ("gamma_measured", C.c_float), ("gamma_predicted", C.c_float),
+                ("ratio", C.c_float), ("plane_xc", C.c_float)]
+
+
 class FcInitOptions(C.Structure):
     _fields_ = [("storage", C.c_int), ("periodic_x", C.c_int), ("periodic_y", C.c_int)]
 
@@ -289,6 +307,20 @@ SIGNATURES = {
     "fc_snapshot_cache_flush": (C.c_longlong, [_vp]),
     "fc_snapshot_cache_set_budget": (C.c_int, [_vp, C.c_uint64]),
     "fc_snapshot_cache_used_bytes": (C.c_uint64, [_vp]),
+    "fc_extract_suction_delta99": (C.c_int, [_vp, _fp, C.c_int, C.POINTER(FcDelta99Sample)]),
+    "fc_separation_onset_xc": (C.c_int, [_vp, _fp]),
+    "fc_download_crossflow_plane": (C.c_int, [_vp, C.c_int, _fp, _fp]),
+    "fc_suction_surface_y": (C.c_int, [_vp, C.c_int]),
+    "fc_lattice_x_for_chord_station": (C.c_int, [_vp, C.c_float]),
+    "fc_band_circulation_live": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(FcBandCirculation)]),
+    "fc_wendt_circulation_lattice": (C.c_float, [C.c_float] * 5),
+    "fc_integrate_streamwise_circulation": (C.c_int, [_fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                      C.POINTER(FcPlaneCirculation)]),
+    "fc_integrate_band_circulation": (C.c_int, [_fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.POINTER(FcBandCirculation)]),
+    "fc_audit_vg_vortex_strength": (C.c_int, [_vp, C.POINTER(FcVGParams), C.c_int, C.c_float, C.c_float,
+                                              C.POINTER(FcVGAuditReadout)]),
     "fc_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),

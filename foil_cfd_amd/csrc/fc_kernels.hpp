@@ -89,6 +89,36 @@ struct FlagEdit {
 };
 int launchFlagEdits(const LatticeView& lat, int layout, const FlagEdit* edits, int count, void* stream);
 
+// ---- boundary-layer readouts on the macro fields ---------------------------
+// One delta99 probe: column ix of the mid-span plane, from the first fluid
+// row above the surface row ys up to yTop (inclusive).
+struct BlStation {
+    int ix, ys, yTop, pad;
+};
+// ue = the ray's maximum in-plane speed |(u, v)|; firstY = first row whose
+// speed reaches 0.99 ue (-1 when none); uNear = u at row ys + 1.
+struct BlProbe {
+    float ue;
+    int firstY;
+    float uNear;
+    int pad;
+};
+int launchDelta99(const float* u, const float* v, int nx, long long planeOff, const BlStation* stations,
+                  int count, BlProbe* out, void* stream);
+// Smallest ix in [ixStart, ixEnd - 2] whose columns ix .. ix + 2 all have a
+// surface row (topY >= 0, topY + 2 < ny) and u < uRev two rows above it;
+// INT_MAX in *outMin when none.
+int launchSeparationOnset(const float* u, const int* topY, int nx, int ny, long long planeOff, int ixStart,
+                          int ixEnd, float uRev, int* outMin, void* stream);
+// (v, w) at column ix packed y + ny * z.
+int launchCrossflowPlane(const float* v, const float* w, const Dims& d, int ix, float* outV, float* outW,
+                         void* stream);
+// Signed streamwise-vorticity sums over rows [yMin, yMax] of column ix:
+// out4 = zone +, zone -, ambient +, ambient - (zone = z in [z0, z1]).
+int bandPartialSlots(const Dims& d);
+int launchBandCirculation(const float* v, const float* w, const Dims& d, int ix, int yMin, int yMax, int z0, int z1,
+                          double* partials, int slots, double* out4, void* stream);
+
 // ---- lattice groups (one batched step launch for several lattices) --------
 inline constexpr int kGroupMaxMembers = 64;
 // Size of one device record of a (step, member) pair; built on the host.

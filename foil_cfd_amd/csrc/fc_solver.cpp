@@ -11,6 +11,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -230,6 +231,11 @@ public:
         hipRelease(dForce_);
         hipRelease(dMass_);
         hipRelease(dMassPartials_);
+        hipRelease(dTopY_);
+        hipRelease(dBlScratch_);
+        hipRelease(dBandPartials_);
+        bandSlots_ = 0;
+        surfDirty_ = true;
         hipRelease(dNan_);
         hipRelease(dForceCells_);
         hipRelease(dForceMasks_);
@@ -620,6 +626,7 @@ public:
         } else {
             surfaceRef_.assign(clean, clean + ncells_);
         }
+        surfDirty_ = true;
         return rebuildWallModel();
     }
 
@@ -987,6 +994,181 @@ public:
         return FC_OK;
     }
 
+    // ---- boundary-layer readouts (lbm_solver.cpp:700-748, :1511-1655) -----
+    // Suction-surface description of the mid-span plane: the top solid row
+    // of every column and the solid x extent, from the clean-foil reference
+    // flags when they match the grid (VG voxels never move the surface),
+    // else the live flags. Rebuilt lazily after a flag change.
+    int surface() {
+        if (!surfDirty_) return FC_OK;
+        const std::vector<std::uint8_t>& src = surfaceRef_.size() == hostFlags_.size() ? surfaceRef_ : hostFlags_;
+        topY_.assign(static_cast<std::size_t>(dims_.nx), -1);
+        xLE_ = dims_.nx;
+        xTE_ = -1;
+        const int zmid = dims_.nz / 2;
+        for (int y = 0; y < dims_.ny; ++y)
+            for (int x = 0; x < dims_.nx; ++x)
+                if (src[at(x, y, zmid)] == kSolid) {
+                    topY_[static_cast<std::size_t>(x)] = y;
+                    xLE_ = std::min(xLE_, x);
+                    xTE_ = std::max(xTE_, x);
+                }
+        surfValid_ = xTE_ > xLE_;
+        if (!dTopY_ && hipAlloc(&dTopY_, topY_.size() * sizeof(int)) != FC_OK)
+            return fail(FC_ERR_ALLOC, "surface row allocation failed");
+        if (!dBlScratch_ && hipAlloc(&dBlScratch_, kBlScratchBytes) != FC_OK)
+            return fail(FC_ERR_ALLOC, "readout scratch allocation failed");
+        if (hipMemcpyAsync(dTopY_, topY_.data(), topY_.size() * sizeof(int), hipMemcpyHostToDevice, stream_) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "surface row upload failed");
+        surfDirty_ = false;
+        return FC_OK;
+    }
+    long long midPlaneOffset() const { return static_cast<long long>(dims_.nx) * dims_.ny * (dims_.nz / 2); }
+
+    // extractSuctionDelta99 (lbm_solver.cpp:1511-1584): stations mapped onto
+    // the projected solid extent, one wavefront per station on the device,
+    // the slope correction and validity on the host.
+    int delta99(const float* xc, int n, fc_delta99_sample* out) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (n < 0 || (n > 0 && (!xc || !out))) return fail(FC_ERR_INVALID, "bad station arrays");
+        for (int i = 0; i < n; ++i) out[i] = fc_delta99_sample{xc[i], 0.0f, 0.0f, 0};
+        if (int e = surface(); e != FC_OK) return e;
+        if (!surfValid_ || n == 0) return FC_OK;
+        const int nx = dims_.nx, ny = dims_.ny;
+        const float chordSpan = static_cast<float>(xTE_ - xLE_);
+        std::vector<BlStation> st;
+        std::vector<int> which;
+        for (int i = 0; i < n; ++i) {
+            const int ix = std::clamp(static_cast<int>(std::lround(static_cast<float>(xLE_) + xc[i] * chordSpan)), 0,
+                                      nx - 1);
+            const int ys = topY_[static_cast<std::size_t>(ix)];
+            if (ys < 0 || ys >= ny - 3) continue;
+            st.push_back(BlStation{ix, ys, std::min(ny - 2, ys + scaling_.chordCells / 2), 0});
+            which.push_back(i);
+        }
+        const int m = static_cast<int>(st.size());
+        if (m == 0) return FC_OK;
+        if (static_cast<std::size_t>(m) * (sizeof(BlStation) + sizeof(BlProbe)) > kBlScratchBytes)
+            return fail(FC_ERR_INVALID, "too many delta99 stations in one call");
+        auto* dSt = reinterpret_cast<BlStation*>(dBlScratch_);
+        auto* dPr = reinterpret_cast<BlProbe*>(dBlScratch_ + m * sizeof(BlStation));
+        std::vector<BlProbe> pr(static_cast<std::size_t>(m));
+        if (hipMemcpyAsync(dSt, st.data(), m * sizeof(BlStation), hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            launchDelta99(u_, v_, nx, midPlaneOffset(), dSt, m, dPr, stream_) != 0 ||
+            hipMemcpyAsync(pr.data(), dPr, m * sizeof(BlProbe), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "delta99 probe failed");
+        const float uLat = scaling_.u_lat;
+        for (int k = 0; k < m; ++k) {
+            fc_delta99_sample& o = out[which[static_cast<std::size_t>(k)]];
+            const BlStation& s = st[static_cast<std::size_t>(k)];
+            const BlProbe& p = pr[static_cast<std::size_t>(k)];
+            o.ue_edge = p.ue;
+            if (p.ue < 0.05f * uLat) continue;  // dead air: separated or stalled
+            const float d99Cells = p.firstY >= 0 ? static_cast<float>(p.firstY - s.ys) - 0.5f : -1.0f;
+            if (d99Cells <= 0.0f) continue;
+            // vertical -> wall-normal: slope of the stair-step crest over +-2 columns
+            float cosTheta = 1.0f;
+            if (s.ix >= 2 && s.ix + 2 < nx) {
+                const int yl = topY_[static_cast<std::size_t>(s.ix - 2)];
+                const int yr = topY_[static_cast<std::size_t>(s.ix + 2)];
+                if (yl >= 0 && yr >= 0) {
+                    const float slope = static_cast<float>(yr - yl) / 4.0f;
+                    cosTheta = 1.0f / std::sqrt(1.0f + slope * slope);
+                }
+            }
+            o.delta99_c = d99Cells * cosTheta / static_cast<float>(scaling_.chordCells);
+            o.valid = p.uNear > -0.02f * uLat ? 1 : 0;  // reverse near-wall flow: separated
+        }
+        return FC_OK;
+    }
+
+    // separationOnsetXc (lbm_solver.cpp:1587-1617): -1 when attached.
+    int separationOnset(float* out) {
+        if (!out) return fail(FC_ERR_INVALID, "null output");
+        *out = -1.0f;
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (int e = surface(); e != FC_OK) return e;
+        if (!surfValid_) return FC_OK;
+        const float chordSpan = static_cast<float>(xTE_ - xLE_);
+        const int ixStart = xLE_ + std::max(2, static_cast<int>(0.03f * chordSpan));
+        const int ixEnd = std::min(xTE_, dims_.nx - 1);
+        int* dMin = reinterpret_cast<int*>(dBlScratch_);
+        int minIx = 0;
+        if (launchSeparationOnset(u_, dTopY_, dims_.nx, dims_.ny, midPlaneOffset(), ixStart, ixEnd,
+                                  -0.02f * scaling_.u_lat, dMin, stream_) != 0 ||
+            hipMemcpyAsync(&minIx, dMin, sizeof(int), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "separation search failed");
+        if (minIx != INT_MAX) *out = (static_cast<float>(minIx) - static_cast<float>(xLE_)) / chordSpan;
+        return FC_OK;
+    }
+
+    // downloadCrossflowPlane (lbm_solver.cpp:1619-1640): (v, w) at column ix,
+    // gathered on the device, one contiguous copy.
+    int crossflowPlane(int ix, float* v, float* w) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (ix < 0 || ix >= dims_.nx || !v || !w) return fail(FC_ERR_INVALID, "bad crossflow column or output");
+        const std::size_t count = static_cast<std::size_t>(dims_.ny) * dims_.nz;
+        float* d = nullptr;
+        if (hipAlloc(&d, 2 * count * sizeof(float)) != FC_OK) return fail(FC_ERR_ALLOC, "crossflow scratch");
+        int rc = FC_OK;
+        if (launchCrossflowPlane(v_, w_, dims_, ix, d, d + count, stream_) != 0 ||
+            hipMemcpyAsync(v, d, count * sizeof(float), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipMemcpyAsync(w, d + count, count * sizeof(float), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            rc = fail(FC_ERR_DEVICE, "crossflow download failed");
+        hipRelease(d);
+        return rc;
+    }
+
+    int suctionSurfaceY(int ix) {
+        if (!initialized_ || surface() != FC_OK || !surfValid_ || ix < 0 || ix >= dims_.nx) return -1;
+        return topY_[static_cast<std::size_t>(ix)];
+    }
+    int latticeXForChordStation(float xc) {
+        if (!initialized_ || surface() != FC_OK || !surfValid_) return -1;
+        const float span = static_cast<float>(xTE_ - xLE_);
+        return std::clamp(static_cast<int>(std::lround(static_cast<float>(xLE_) + xc * span)), 0, dims_.nx - 1);
+    }
+
+    // Streamwise-vorticity band integral of the live field at column ix
+    // (integrateBandCirculation, vg_audit.cpp:62-101, with the plane read in
+    // place and the sums in FP64). Same clamps and empty-band result.
+    int bandCirculation(int ix, int yMin, int yMax, int z0, int z1, fc_band_circulation* out) {
+        if (!out) return fail(FC_ERR_INVALID, "null output");
+        *out = fc_band_circulation{};
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (ix < 0 || ix >= dims_.nx) return fail(FC_ERR_INVALID, "bad crossflow column");
+        const int ny = dims_.ny, nz = dims_.nz;
+        if (nz < 3) return FC_OK;
+        yMin = std::max(yMin, 0);
+        yMax = std::min(yMax, ny - 1);
+        z0 = std::max(z0, 0);
+        z1 = std::min(z1, nz - 1);
+        if (yMax <= yMin || z1 < z0) return FC_OK;
+        if (!dBandPartials_) {
+            bandSlots_ = bandPartialSlots(dims_);
+            if (hipAlloc(&dBandPartials_, static_cast<std::size_t>(bandSlots_ + 1) * 4 * sizeof(double)) != FC_OK)
+                return fail(FC_ERR_ALLOC, "band partials");
+        }
+        double* dOut = dBandPartials_ + 4 * bandSlots_;
+        double h[4] = {0, 0, 0, 0};
+        if (launchBandCirculation(v_, w_, dims_, ix, yMin, yMax, z0, z1, dBandPartials_, bandSlots_, dOut, stream_) !=
+                0 ||
+            hipMemcpyAsync(h, dOut, sizeof h, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "band circulation failed");
+        out->zone = fc_plane_circulation{static_cast<float>(h[0]), static_cast<float>(h[1])};
+        out->ambient = fc_plane_circulation{static_cast<float>(h[2]), static_cast<float>(h[3])};
+        out->zone_cols = z1 - z0 + 1;
+        out->ambient_cols = nz - out->zone_cols;
+        return FC_OK;
+    }
+    int chordCells() const { return scaling_.chordCells; }
+
     int nanDetected() {
         pollAsync();
         return nanLatched_ ? 1 : 0;
@@ -1329,6 +1511,7 @@ private:
     }
 
     int uploadFlags() {
+        surfDirty_ = true;
         KindBuild kb;
         buildKinds(hostFlags_, dims_, periodicX_, periodicY_, kb);
         hostKind_ = kb.kind;
@@ -1898,6 +2081,16 @@ private:
     int nanKind_ = 0;
     int stepsSinceWatchdog_ = 0;
     long long lastEditCount_ = 0;
+    // boundary-layer readouts: surface rows of the mid-span plane (host and
+    // device copies) and small device scratch
+    static constexpr std::size_t kBlScratchBytes = 64 * 1024;
+    std::vector<int> topY_;
+    int xLE_ = 0, xTE_ = -1;
+    bool surfValid_ = false, surfDirty_ = true;
+    int* dTopY_ = nullptr;
+    char* dBlScratch_ = nullptr;
+    double* dBandPartials_ = nullptr;
+    int bandSlots_ = 0;
     bool timingPending_ = false;
     bool groupTimed_ = false;
     int timingBatchN_ = 0;
@@ -2213,6 +2406,84 @@ int fc_apply_edited_flags(fc_solver* s, const uint8_t* flags) {
 }
 
 long long fc_last_edit_count(fc_solver* s) { return s ? s->impl.lastEditCount() : 0; }
+
+int fc_extract_suction_delta99(fc_solver* s, const float* stations_xc, int count, fc_delta99_sample* out) {
+    return s ? s->impl.delta99(stations_xc, count, out) : FC_ERR_INVALID;
+}
+
+int fc_separation_onset_xc(fc_solver* s, float* out) {
+    return s ? s->impl.separationOnset(out) : FC_ERR_INVALID;
+}
+
+int fc_download_crossflow_plane(fc_solver* s, int ix, float* v, float* w) {
+    return s ? s->impl.crossflowPlane(ix, v, w) : FC_ERR_INVALID;
+}
+
+int fc_suction_surface_y(fc_solver* s, int ix) { return s ? s->impl.suctionSurfaceY(ix) : -1; }
+
+int fc_lattice_x_for_chord_station(fc_solver* s, float xc) { return s ? s->impl.latticeXForChordStation(xc) : -1; }
+
+int fc_band_circulation_live(fc_solver* s, int ix, int y_min, int y_max, int z0, int z1, fc_band_circulation* out) {
+    return s ? s->impl.bandCirculation(ix, y_min, y_max, z0, z1, out) : FC_ERR_INVALID;
+}
+
+// auditVGVortexStrength (vg_audit.cpp:103-187): Wendt prediction from the
+// live edge velocity and delta99, measured circulation from the crossflow
+// band a rollup distance behind the vanes, minus the area-scaled ambient
+// floor of the same band outside the vane-row zone.
+int fc_audit_vg_vortex_strength(fc_solver* s, const fc_vg_params* vg, int chord_cells, float ue_lat,
+                                float delta99_cells, fc_vg_audit_readout* out) {
+    if (!s || !vg || !out) return FC_ERR_INVALID;
+    fc_vg_audit_readout r{0, 0.0f, 0.0f, 0.0f, 0.0f};
+    *out = r;
+    if (chord_cells <= 0 || ue_lat <= 0.0f || delta99_cells <= 0.0f || vg->count <= 0) return FC_OK;
+    const float h = vg->height_c * static_cast<float>(chord_cells);  // vgHeightCells, vg.h:181
+    if (h <= 0.0f) return FC_OK;
+    const float vaneChord = vg->length_h * h;
+    const float betaRad = vg->beta_deg * 3.14159265f / 180.0f;
+    r.gamma_predicted = fc_wendt_circulation_lattice(betaRad, ue_lat, h, vaneChord, delta99_cells);
+    if (r.gamma_predicted <= 0.0f) {
+        *out = r;
+        return FC_OK;
+    }
+    constexpr float kPlaneHeightsDownstream = 3.0f;  // vg_audit.h:35
+    constexpr float kBandHeights = 4.0f;             // vg_audit.h:41
+    r.plane_xc = std::clamp(vg->x_c + vg->height_c * (vg->length_h + kPlaneHeightsDownstream), 0.02f, 0.97f);
+    *out = r;
+    const int ix = s->impl.latticeXForChordStation(r.plane_xc);
+    if (ix < 0) return FC_OK;
+    const int ys = s->impl.suctionSurfaceY(ix);
+    if (ys < 0) return FC_OK;
+    const fc::Dims d = s->impl.dims();
+    const int band = std::max(2, static_cast<int>(std::ceil(kBandHeights * h)));
+    // vane-row zone: every unit of the centred array plus 3h of meander margin
+    const float pitchCells = vg->pitch_c * static_cast<float>(chord_cells);
+    const float rowHalf =
+        0.5f * (static_cast<float>(vg->count - 1) * pitchCells + vg->gap_h * h) + 3.0f * h;
+    const int mid = d.nz / 2;
+    const int z0 = mid - static_cast<int>(std::ceil(rowHalf));
+    const int z1 = mid + static_cast<int>(std::ceil(rowHalf));
+    fc_band_circulation b{};
+    if (int e = s->impl.bandCirculation(ix, ys + 1, std::min(d.ny - 2, ys + band), z0, z1, &b); e != FC_OK) return e;
+    if (b.zone_cols <= 0) return FC_OK;
+    float floorPos = 0.0f, floorNeg = 0.0f;
+    if (b.ambient_cols >= 4) {
+        const float scale = static_cast<float>(b.zone_cols) / static_cast<float>(b.ambient_cols);
+        floorPos = b.ambient.pos * scale;
+        floorNeg = -b.ambient.neg * scale;
+    }
+    const float posNet = std::max(0.0f, b.zone.pos - floorPos);
+    const float negNet = std::max(0.0f, -b.zone.neg - floorNeg);
+    const float n = static_cast<float>(vg->count);
+    // counter-rotating pair (1) and ramp (3) shed one vortex of each sign per
+    // unit; single vane (0) and co-rotating array (2) take the dominant sign
+    const float gamma = (vg->type == 1 || vg->type == 3) ? (posNet + negNet) / (2.0f * n) : std::max(posNet, negNet) / n;
+    r.gamma_measured = gamma;
+    r.ratio = gamma / r.gamma_predicted;
+    r.valid = 1;
+    *out = r;
+    return FC_OK;
+}
 
 int fc_restore_from_macroscopic(fc_solver* s, const float* rho, const float* u, const float* v,
                                 const float* w) {

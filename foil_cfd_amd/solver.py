@@ -406,6 +406,46 @@ class LBMSolver:
         check(self._lib.fc_synchronize(self._h), "fc_synchronize")
 
     # readouts ---------------------------------------------------------------
+    # boundary-layer readouts -----------------------------------------------
+    def extract_suction_delta99(self, stations_xc):
+        """LBMSolver::extractSuctionDelta99: list of dicts (x_c, delta99_c,
+        ue_edge, valid), one per station."""
+        from ._abi import FcDelta99Sample
+        xs = np.ascontiguousarray(stations_xc, dtype=np.float32).reshape(-1)
+        out = (FcDelta99Sample * max(1, xs.size))()
+        check(self._lib.fc_extract_suction_delta99(self._h, _f32(xs), xs.size, out),
+              "fc_extract_suction_delta99", self.last_error())
+        return [{"x_c": o.x_c, "delta99_c": o.delta99_c, "ue_edge": o.ue_edge, "valid": bool(o.valid)}
+                for o in out[:xs.size]]
+
+    def separation_onset_xc(self) -> float:
+        r = C.c_float(0.0)
+        check(self._lib.fc_separation_onset_xc(self._h, C.byref(r)), "fc_separation_onset_xc", self.last_error())
+        return float(r.value)
+
+    def download_crossflow_plane(self, ix: int):
+        """(v, w) at column ix as (nz, ny) arrays (flat index y + ny*z)."""
+        _, ny, nz = self.dims
+        v = np.empty(ny * nz, dtype=np.float32)
+        w = np.empty(ny * nz, dtype=np.float32)
+        check(self._lib.fc_download_crossflow_plane(self._h, int(ix), _f32(v), _f32(w)),
+              "fc_download_crossflow_plane", self.last_error())
+        return v, w
+
+    def suction_surface_y(self, ix: int) -> int:
+        return int(self._lib.fc_suction_surface_y(self._h, int(ix)))
+
+    def lattice_x_for_chord_station(self, xc: float) -> int:
+        return int(self._lib.fc_lattice_x_for_chord_station(self._h, float(xc)))
+
+    def band_circulation_live(self, ix, y_min, y_max, z0, z1) -> dict:
+        from ._abi import FcBandCirculation
+        b = FcBandCirculation()
+        check(self._lib.fc_band_circulation_live(self._h, int(ix), int(y_min), int(y_max), int(z0), int(z1),
+                                                 C.byref(b)), "fc_band_circulation_live", self.last_error())
+        return {"zone": (b.zone.pos, b.zone.neg), "ambient": (b.ambient.pos, b.ambient.neg),
+                "zone_cols": b.zone_cols, "ambient_cols": b.ambient_cols}
+
     def read_flags(self) -> np.ndarray:
         """Host copy of the cell flags (LBMSolver::hostFlags)."""
         out = np.empty(self.ncells, dtype=np.uint8)

@@ -557,6 +557,47 @@ long long fc_snapshot_cache_flush(fc_snapshot_cache* c);
 int fc_snapshot_cache_set_budget(fc_snapshot_cache* c, uint64_t budget_bytes);
 uint64_t fc_snapshot_cache_used_bytes(const fc_snapshot_cache* c);
 
+/* ---- boundary-layer readouts and the VG vortex audit ---------------------- */
+/* (lbm_solver.h:58-65, :379-412; src/geom/vg_audit.h). The readouts work on
+ * the macro fields of the last fc_step call at mid-span (z = nz / 2); the
+ * surface is the clean-foil reference when set (fc_set_surface_reference),
+ * else the live flags. */
+typedef struct fc_delta99_sample {   /* Delta99Sample */
+    float x_c, delta99_c, ue_edge;
+    int valid;
+} fc_delta99_sample;
+typedef struct fc_plane_circulation { float pos, neg; } fc_plane_circulation;  /* PlaneCirculation */
+typedef struct fc_band_circulation {  /* BandCirculation */
+    fc_plane_circulation zone, ambient;
+    int zone_cols, ambient_cols;
+} fc_band_circulation;
+typedef struct fc_vg_audit_readout {  /* VGAuditReadout */
+    int valid;
+    float gamma_measured, gamma_predicted, ratio, plane_xc;
+} fc_vg_audit_readout;
+/* extractSuctionDelta99 (lbm_solver.cpp:1511-1584): one sample per station. */
+int fc_extract_suction_delta99(fc_solver* s, const float* stations_xc, int count, fc_delta99_sample* out);
+/* separationOnsetXc (:1587-1617): onset x/c, or -1 when attached. */
+int fc_separation_onset_xc(fc_solver* s, float* out);
+/* downloadCrossflowPlane (:1619-1640): v, w at column ix, ny*nz each (y + ny*z). */
+int fc_download_crossflow_plane(fc_solver* s, int ix, float* v, float* w);
+int fc_suction_surface_y(fc_solver* s, int ix);                 /* suctionSurfaceY, :1642; -1 if none */
+int fc_lattice_x_for_chord_station(fc_solver* s, float xc);    /* latticeXForChordStation, :1648 */
+/* integrateBandCirculation of the live field's column ix computed on the
+ * device (FP64 sums); same clamps and empty-band result as the host form. */
+int fc_band_circulation_live(fc_solver* s, int ix, int y_min, int y_max, int z0, int z1,
+                             fc_band_circulation* out);
+/* Host math of vg_audit.h:44-105 on host planes (index y + ny*z). */
+float fc_wendt_circulation_lattice(float beta_rad, float ue_lat, float h_cells, float vane_chord_cells,
+                                   float delta99_cells);
+int fc_integrate_streamwise_circulation(const float* v, const float* w, int ny, int nz, int y_min, int y_max,
+                                        fc_plane_circulation* out);
+int fc_integrate_band_circulation(const float* v, const float* w, int ny, int nz, int y_min, int y_max, int z0,
+                                  int z1, fc_band_circulation* out);
+/* auditVGVortexStrength (vg_audit.cpp:103-187), band integral on the device. */
+int fc_audit_vg_vortex_strength(fc_solver* s, const fc_vg_params* vg, int chord_cells, float ue_lat,
+                                float delta99_cells, fc_vg_audit_readout* out);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Per-launch HIP-event timing of the stream-collide kernel (on the solver's
  * stream); fc_kernel_timing returns launches and summed milliseconds since

@@ -657,3 +657,188 @@ def apply_flag_edits(f, dims, old_flags, new_flags):
         rho, ux, uy, uz = acc / found if found else (1.0, 0.0, 0.0, 0.0)
         out[:, c] = equilibrium_np([rho], [ux], [uy], [uz])[:, 0]
     return out
+
+
+# ---- boundary-layer readouts and the VG audit (lbm_solver.cpp:700-748,
+# :1511-1655; vg_audit.cpp:12-187), restated with numpy on host arrays ---------
+def surface_cache(flags, dims):
+    """Top solid row per column of the mid-span plane and the solid x extent
+    (rebuildSurfaceCache, lbm_solver.cpp:700-728)."""
+    nx, ny, nz = dims
+    plane = np.asarray(flags).reshape(nz, ny, nx)[nz // 2] == SOLID_FLAG
+    top = np.full(nx, -1, np.int64)
+    for x in range(nx):
+        ys = np.nonzero(plane[:, x])[0]
+        if ys.size:
+            top[x] = ys.max()
+    cols = np.nonzero(top >= 0)[0]
+    if cols.size == 0:
+        return top, nx, -1, False
+    xle, xte = int(cols.min()), int(cols.max())
+    return top, xle, xte, xte > xle
+
+
+def speed32(a, b):
+    """FP32 |(a, b)| from the exact FP64 sum of squares (glibc hypotf)."""
+    a = np.asarray(a, np.float32).astype(np.float64)
+    b = np.asarray(b, np.float32).astype(np.float64)
+    return np.sqrt(a * a + b * b).astype(np.float32)
+
+
+def _f32(x):
+    return np.float32(x)
+
+
+def suction_delta99(u, v, flags, dims, stations, chord_cells, u_lat):
+    """extractSuctionDelta99 (lbm_solver.cpp:1511-1584) on macro fields
+    (cells each); FP32 arithmetic as the reference's host code."""
+    nx, ny, nz = dims
+    top, xle, xte, ok = surface_cache(flags, dims)
+    out = [{"x_c": float(_f32(s)), "delta99_c": 0.0, "ue_edge": 0.0, "valid": False} for s in stations]
+    if not ok:
+        return out
+    mu = np.asarray(u, np.float32).reshape(nz, ny, nx)[nz // 2]
+    mv = np.asarray(v, np.float32).reshape(nz, ny, nx)[nz // 2]
+    span = _f32(xte - xle)
+    ul = _f32(u_lat)
+    for o in out:
+        pos = _f32(xle) + _f32(o["x_c"]) * span
+        ix = int(min(max(int(np.floor(abs(float(pos)) + 0.5)) * (1 if pos >= 0 else -1), 0), nx - 1))
+        ys = int(top[ix])
+        if ys < 0 or ys >= ny - 3:
+            continue
+        ytop = min(ny - 2, ys + chord_cells // 2)
+        sp = speed32(mu[ys + 1:ytop + 1, ix], mv[ys + 1:ytop + 1, ix])
+        ue = _f32(sp.max()) if sp.size else _f32(0)
+        o["ue_edge"] = float(ue)
+        if ue < _f32(0.05) * ul:
+            continue
+        hit = np.nonzero(sp >= _f32(0.99) * ue)[0]
+        if hit.size == 0:
+            continue
+        d99 = _f32(hit[0] + 1) - _f32(0.5)
+        cos = _f32(1)
+        if ix >= 2 and ix + 2 < nx and top[ix - 2] >= 0 and top[ix + 2] >= 0:
+            slope = _f32(top[ix + 2] - top[ix - 2]) / _f32(4)
+            cos = _f32(1) / np.sqrt(_f32(1) + slope * slope, dtype=np.float32)
+        o["delta99_c"] = float(d99 * cos / _f32(chord_cells))
+        o["valid"] = bool(mu[ys + 1, ix] > _f32(-0.02) * ul)
+    return out
+
+
+def separation_onset_xc(u, flags, dims, u_lat):
+    """separationOnsetXc (lbm_solver.cpp:1587-1617), sequential scan."""
+    nx, ny, nz = dims
+    top, xle, xte, ok = surface_cache(flags, dims)
+    if not ok:
+        return -1.0
+    mu = np.asarray(u, np.float32).reshape(nz, ny, nx)[nz // 2]
+    span = _f32(xte - xle)
+    urev = _f32(-0.02) * _f32(u_lat)
+    run = 0
+    start = xle + max(2, int(_f32(0.03) * span))
+    ix = start
+    while ix <= xte and ix < nx:
+        ys = int(top[ix])
+        if ys < 0 or ys + 2 >= ny:
+            run = 0
+        elif mu[ys + 2, ix] < urev:
+            run += 1
+            if run >= 3:
+                return float((_f32(ix - run + 1) - _f32(xle)) / span)
+        else:
+            run = 0
+        ix += 1
+    return -1.0
+
+
+def wendt_circulation(beta_rad, ue, h, c, d99):
+    """Wendt correlation in lattice units (vg_audit.cpp:12-25), FP32."""
+    if ue <= 0 or h <= 0 or c <= 0 or d99 <= 0:
+        return 0.0
+    f = np.float32
+    ar = f(8) * f(h) / (f(3.14159265) * f(c))
+    return float(f(1.61) * abs(f(beta_rad)) * f(ue) * f(c) * np.tanh(f(1.41) * f(h) / f(d99), dtype=f)
+                 / (f(1) + f(0.48) / ar))
+
+
+def band_circulation(v, w, ny, nz, y0, y1, z0, z1, f64=False):
+    """integrateBandCirculation (vg_audit.cpp:62-101) on a plane indexed
+    y + ny*z: omega_x in FP32; sums in FP32 in the reference's z-then-y order,
+    or in FP64."""
+    v = np.asarray(v, np.float32).reshape(nz, ny)
+    w = np.asarray(w, np.float32).reshape(nz, ny)
+    out = {"zone": [0.0, 0.0], "ambient": [0.0, 0.0], "zone_cols": 0, "ambient_cols": 0}
+    if nz < 3:
+        return out
+    y0, y1 = max(y0, 0), min(y1, ny - 1)
+    z0, z1 = max(z0, 0), min(z1, nz - 1)
+    if y1 <= y0 or z1 < z0:
+        return out
+    acc = np.float64 if f64 else np.float32
+    sums = {"zone": [acc(0), acc(0)], "ambient": [acc(0), acc(0)]}
+    h = np.float32(0.5)
+    for z in range(nz):
+        key = "zone" if z0 <= z <= z1 else "ambient"
+        out[key + "_cols"] += 1
+        zm, zp = (z - 1) % nz, (z + 1) % nz
+        for y in range(y0, y1 + 1):
+            if y == y0:
+                dwdy = w[z, y + 1] - w[z, y]
+            elif y == y1:
+                dwdy = w[z, y] - w[z, y - 1]
+            else:
+                dwdy = h * (w[z, y + 1] - w[z, y - 1])
+            om = dwdy - h * (v[zp, y] - v[zm, y])
+            sums[key][0 if om > 0 else 1] += acc(om)
+    out["zone"] = [float(x) for x in sums["zone"]]
+    out["ambient"] = [float(x) for x in sums["ambient"]]
+    return out
+
+
+def audit_vg(v, w, flags, dims, vg, chord_cells, ue, d99):
+    """auditVGVortexStrength (vg_audit.cpp:103-187) on host fields (cells
+    each), band sums in FP64; vg: dict with type, x_c, height_c, length_h,
+    beta_deg, pitch_c, gap_h, count."""
+    f = np.float32
+    out = {"valid": False, "gamma_measured": 0.0, "gamma_predicted": 0.0, "ratio": 0.0, "plane_xc": 0.0}
+    if chord_cells <= 0 or ue <= 0 or d99 <= 0 or vg["count"] <= 0:
+        return out
+    h = f(vg["height_c"]) * f(chord_cells)
+    if h <= 0:
+        return out
+    beta = f(vg["beta_deg"]) * f(3.14159265) / f(180)
+    gp = wendt_circulation(beta, ue, h, f(vg["length_h"]) * h, d99)
+    out["gamma_predicted"] = gp
+    if gp <= 0:
+        return out
+    pxc = float(min(max(f(vg["x_c"]) + f(vg["height_c"]) * (f(vg["length_h"]) + f(3)), f(0.02)), f(0.97)))
+    out["plane_xc"] = pxc
+    nx, ny, nz = dims
+    top, xle, xte, ok = surface_cache(flags, dims)
+    if not ok:
+        return out
+    pos = f(xle) + f(pxc) * f(xte - xle)
+    ix = int(min(max(int(np.floor(abs(float(pos)) + 0.5)) * (1 if pos >= 0 else -1), 0), nx - 1))
+    ys = int(top[ix])
+    if ys < 0:
+        return out
+    band = max(2, int(np.ceil(f(4) * h)))
+    pitch = f(vg["pitch_c"]) * f(chord_cells)
+    half = f(0.5) * (f(vg["count"] - 1) * pitch + f(vg["gap_h"]) * h) + f(3) * h
+    mid = nz // 2
+    z0, z1 = mid - int(np.ceil(half)), mid + int(np.ceil(half))
+    pv = np.asarray(v, np.float32).reshape(nz, ny, nx)[:, :, ix].reshape(-1)
+    pw = np.asarray(w, np.float32).reshape(nz, ny, nx)[:, :, ix].reshape(-1)
+    b = band_circulation(pv, pw, ny, nz, ys + 1, min(ny - 2, ys + band), z0, z1, f64=True)
+    if b["zone_cols"] <= 0:
+        return out
+    fp = fn = 0.0
+    if b["ambient_cols"] >= 4:
+        sc = b["zone_cols"] / b["ambient_cols"]
+        fp, fn = b["ambient"][0] * sc, -b["ambient"][1] * sc
+    pn, nn = max(0.0, b["zone"][0] - fp), max(0.0, -b["zone"][1] - fn)
+    n = vg["count"]
+    g = (pn + nn) / (2 * n) if vg["type"] in (1, 3) else max(pn, nn) / n
+    out.update(valid=True, gamma_measured=g, ratio=g / gp)
+    return out
