@@ -307,6 +307,9 @@ SIGNATURES = {
     "fc_snapshot_cache_flush": (C.c_longlong, [_vp]),
     "fc_snapshot_cache_set_budget": (C.c_int, [_vp, C.c_uint64]),
     "fc_snapshot_cache_used_bytes": (C.c_uint64, [_vp]),
+    "fc_seed_from_coarse": (C.c_int, [_vp, _vp]),
+    "fc_preconverge": (C.c_int, [_vp, FcDims, C.POINTER(FcScaling), _u8p, C.POINTER(FcInitOptions),
+                                 C.POINTER(C.c_longlong), C.c_char_p, C.c_int]),
     "fc_extract_suction_delta99": (C.c_int, [_vp, _fp, C.c_int, C.POINTER(FcDelta99Sample)]),
     "fc_separation_onset_xc": (C.c_int, [_vp, _fp]),
     "fc_download_crossflow_plane": (C.c_int, [_vp, C.c_int, _fp, _fp]),

@@ -123,6 +123,22 @@ def finer_patch(case: Case, box, factor: int, finer_box, finer_factor: int = 2, 
     return S.build_fine_patch_flags(pts, aoa, fine_layout, S.FcPatchBox(*finer_box), finer_factor)
 
 
+def presolve_case(case: Case, chord: int | None = None, aoa: float = 8.0, digits: str = "2412") -> Case:
+    """The mesh-sequencing companion of a tunnel case (src/main.cpp:756-790):
+    the same section and margins at `chord` cells per chord (default
+    max(48, chord / 4)), grid scaled in proportion, same u_lat and physics."""
+    nx, ny, nz = case.dims
+    c0 = case.scaling.chord_cells
+    nc = max(48, c0 // 4) if chord is None else int(chord)
+    dims = tuple(max(3, int(round(n * nc / c0))) for n in (nx, ny, nz))
+    pts, _ = S.generate_naca4(digits, 200)
+    layout = S.make_layout(*dims, nc)
+    flags = S.build_clean_foil_flags(pts, aoa, layout)
+    sc = case.scaling
+    pre = S.compute_scaling(sc.chord_m, sc.airspeed_ms, sc.nu_m2s, nc, sc.u_lat)  # same physical inputs
+    return Case(f"{case.name}_pre{nc}", dims, flags, pre, cs=case.cs, ramp=True)
+
+
 def make_solver(case: Case, storage: str = "f32") -> S.LBMSolver:
     s = S.LBMSolver()
     s.set_startup_ramp_enabled(case.ramp)

@@ -89,6 +89,11 @@ struct FlagEdit {
 };
 int launchFlagEdits(const LatticeView& lat, int layout, const FlagEdit* edits, int count, void* stream);
 
+// Trilinear upsample of macro fields (rho, u, v, w; cells of s each) onto
+// grid d (cell centres mapped by the ratio, x / y clamped, z periodic).
+int launchUpsampleMacro(const float* sRho, const float* sU, const float* sV, const float* sW, const Dims& s,
+                        float* dRho, float* dU, float* dV, float* dW, const Dims& d, void* stream);
+
 // ---- boundary-layer readouts on the macro fields ---------------------------
 // One delta99 probe: column ix of the mid-span plane, from the first fluid
 // row above the surface row ys up to yTop (inclusive).

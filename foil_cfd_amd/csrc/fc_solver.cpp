@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -1168,6 +1169,34 @@ public:
         return FC_OK;
     }
     int chordCells() const { return scaling_.chordCells; }
+
+    // ---- mesh sequencing (seedFromCoarse, lbm_solver.cpp:2062-2102) ---------
+    // The presolver's macro fields, trilinearly upsampled on the device into
+    // this solver's macro arrays, become f = feq(rho, u) (the compact-restore
+    // path); the startup viscosity ramp is re-armed over the seeded field and
+    // an active patch re-seeds from it (child after parent).
+    int seedFromCoarse(Solver& pre) {
+        if (!initialized_ || !pre.initialized_) return fail(FC_ERR_STATE, "both solvers must be initialized");
+        if (&pre == this) return fail(FC_ERR_INVALID, "a solver cannot seed itself");
+        if (hipStreamSynchronize(pre.stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "presolver stream failed");
+        if (launchUpsampleMacro(pre.rho_, pre.u_, pre.v_, pre.w_, pre.dims_, rho_, u_, v_, w_, dims_, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "macroscopic upsample failed");
+        if (launchInitFromMacro(view(), stream_) != 0) return fail(FC_ERR_DEVICE, "equilibrium re-init failed");
+        layout_ = kLayoutL;
+        notifySnapshotRestored(false);
+        rampActive_ = true;
+        rampStartStep_ = steps_;
+        if (fine_.active) {
+            if (int e = seedFine(); e != FC_OK) return e;
+        }
+        if (finer_.active) {
+            if (int e = seedFiner(); e != FC_OK) return e;
+        }
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "seeding failed");
+        return FC_OK;
+    }
+    float smagorinskyCs() const { return cs_; }
+    const Scaling& baseScaling() const { return scaling_; }
 
     int nanDetected() {
         pollAsync();
@@ -2406,6 +2435,55 @@ int fc_apply_edited_flags(fc_solver* s, const uint8_t* flags) {
 }
 
 long long fc_last_edit_count(fc_solver* s) { return s ? s->impl.lastEditCount() : 0; }
+
+int fc_seed_from_coarse(fc_solver* s, fc_solver* presolver) {
+    if (!s || !presolver) return FC_ERR_INVALID;
+    return s->impl.seedFromCoarse(presolver->impl);
+}
+
+// runPreconverge (src/main.cpp:756-817): the caller voxelizes the same
+// geometry at the presolve resolution; the presolver runs two flow-throughs
+// (the force-gate criterion) in batches of 1024 steps with a divergence check
+// after each, then seeds the solver. The presolver takes the solver's
+// Smagorinsky constant and keeps its startup ramp.
+int fc_preconverge(fc_solver* s, fc_dims pre_dims, const fc_scaling* pre_scaling, const uint8_t* pre_flags,
+                   const fc_init_options* options, long long* steps_run, char* err, int errlen) {
+    if (!s || !pre_scaling || !pre_flags) return FC_ERR_INVALID;
+    if (steps_run) *steps_run = 0;
+    if (!s->impl.initialized()) {
+        fc::copyErr("solver not initialized", err, errlen);
+        return FC_ERR_STATE;
+    }
+    auto pre = std::make_unique<fc_solver>();
+    if (int rc = pre->impl.init(pre_dims, *pre_scaling, pre_flags, options); rc != FC_OK) {
+        fc::copyErr("presolve init failed: " + pre->impl.error, err, errlen);
+        return rc;
+    }
+    pre->impl.setCs(s->impl.smagorinskyCs());
+    const long long total =
+        static_cast<long long>(fc::kForceGateFlowThroughs * fc::fromC(*pre_scaling).flowThroughSteps(pre_dims.nx));
+    long long done = 0;
+    while (done < total) {
+        const int n = static_cast<int>(std::min<long long>(1024, total - done));
+        int rc = pre->impl.stepN(n);
+        if (rc == FC_OK) rc = pre->impl.synchronize();
+        if (rc != FC_OK) {
+            fc::copyErr("presolve step failed: " + pre->impl.error, err, errlen);
+            return rc;
+        }
+        if (pre->impl.nanDetected()) {
+            fc::copyErr("coarse companion diverged", err, errlen);
+            return FC_ERR_STATE;
+        }
+        done += n;
+    }
+    if (steps_run) *steps_run = done;
+    if (int rc = s->impl.seedFromCoarse(pre->impl); rc != FC_OK) {
+        fc::copyErr(s->impl.error, err, errlen);
+        return rc;
+    }
+    return FC_OK;
+}
 
 int fc_extract_suction_delta99(fc_solver* s, const float* stations_xc, int count, fc_delta99_sample* out) {
     return s ? s->impl.delta99(stations_xc, count, out) : FC_ERR_INVALID;

@@ -406,6 +406,24 @@ class LBMSolver:
         check(self._lib.fc_synchronize(self._h), "fc_synchronize")
 
     # readouts ---------------------------------------------------------------
+    # mesh sequencing ---------------------------------------------------------
+    def seed_from_coarse(self, presolver: "LBMSolver") -> None:
+        """LBMSolver::seedFromCoarse: upsampled presolve field as the state."""
+        check(self._lib.fc_seed_from_coarse(self._h, presolver._h), "fc_seed_from_coarse", self.last_error())
+
+    def preconverge(self, pre_dims, pre_scaling: FcScaling, pre_flags, storage="f32") -> int:
+        """runPreconverge (src/main.cpp:756-817) on a presolve grid built by
+        the caller; returns the presolver's step count."""
+        st = _abi.STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
+        fl = np.ascontiguousarray(pre_flags, dtype=np.uint8).reshape(-1)
+        opt = FcInitOptions(st, 0, 0)
+        steps = C.c_longlong(0)
+        err = C.create_string_buffer(512)
+        rc = self._lib.fc_preconverge(self._h, FcDims(*pre_dims), C.byref(pre_scaling), _u8(fl), C.byref(opt),
+                                      C.byref(steps), err, 512)
+        check(rc, "fc_preconverge", err.value.decode(errors="replace"))
+        return int(steps.value)
+
     # boundary-layer readouts -----------------------------------------------
     def extract_suction_delta99(self, stations_xc):
         """LBMSolver::extractSuctionDelta99: list of dicts (x_c, delta99_c,

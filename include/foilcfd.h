@@ -557,6 +557,18 @@ long long fc_snapshot_cache_flush(fc_snapshot_cache* c);
 int fc_snapshot_cache_set_budget(fc_snapshot_cache* c, uint64_t budget_bytes);
 uint64_t fc_snapshot_cache_used_bytes(const fc_snapshot_cache* c);
 
+/* ---- mesh sequencing (lbm_solver.h:258, src/main.cpp:756-817) ------------- */
+/* seedFromCoarse (lbm_solver.cpp:2062-2102): trilinear upsample of the
+ * presolver's macro fields (lbm_refine.cu:437-501) into s, f = feq(rho, u),
+ * compact-restore bookkeeping, startup ramp re-armed, patch levels re-seeded. */
+int fc_seed_from_coarse(fc_solver* s, fc_solver* presolver);
+/* runPreconverge: a presolver on (pre_dims, pre_scaling, pre_flags) - the
+ * same geometry voxelized at max(48, chord / 4) cells per chord - runs two
+ * flow-throughs in batches of 1024 steps (stopping with FC_ERR_STATE if it
+ * diverges), then seeds s. steps_run (may be NULL) receives its step count. */
+int fc_preconverge(fc_solver* s, fc_dims pre_dims, const fc_scaling* pre_scaling, const uint8_t* pre_flags,
+                   const fc_init_options* options, long long* steps_run, char* err, int errlen);
+
 /* ---- boundary-layer readouts and the VG vortex audit ---------------------- */
 /* (lbm_solver.h:58-65, :379-412; src/geom/vg_audit.h). The readouts work on
  * the macro fields of the last fc_step call at mid-span (z = nz / 2); the

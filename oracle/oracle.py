@@ -842,3 +842,50 @@ def audit_vg(v, w, flags, dims, vg, chord_cells, ue, d99):
     g = (pn + nn) / (2 * n) if vg["type"] in (1, 3) else max(pn, nn) / n
     out.update(valid=True, gamma_measured=g, ratio=g / gp)
     return out
+
+
+def upsample_macro(fields, sdims, ddims):
+    """Trilinear upsample of macro fields (upsampleMacroKernel,
+    lbm_refine.cu:437-501) in FP32, the expression order as written; fields
+    = (rho, u, v, w), cells of sdims each. Returns the four fields on ddims."""
+    snx, sny, snz = sdims
+    dnx, dny, dnz = ddims
+    f = np.float32
+    z, y, x = np.meshgrid(np.arange(dnz), np.arange(dny), np.arange(dnx), indexing="ij")
+    x, y, z = (a.reshape(-1) for a in (x, y, z))
+
+    def centre(i, s, d):
+        return (i.astype(f) + f(0.5)) * f(s) / f(d) - f(0.5)
+
+    gx, gy, gz = centre(x, snx, dnx), centre(y, sny, dny), centre(z, snz, dnz)
+    x0 = np.floor(gx).astype(np.int64)
+    y0 = np.floor(gy).astype(np.int64)
+    zf = np.floor(gz).astype(np.int64)
+    tx = (gx - x0.astype(f)).astype(f)
+    ty = (gy - y0.astype(f)).astype(f)
+    tz = (gz - zf.astype(f)).astype(f)
+    lo = x0 < 0
+    x0[lo], tx[lo] = 0, 0
+    lo = y0 < 0
+    y0[lo], ty[lo] = 0, 0
+    x1, y1 = x0 + 1, y0 + 1
+    hi = x1 > snx - 1
+    x1[hi], tx[hi] = snx - 1, 0
+    hi = y1 > sny - 1
+    y1[hi], ty[hi] = sny - 1, 0
+    z0 = ((zf % snz) + snz) % snz
+    z1 = (z0 + 1) % snz
+    ux, uy, uz = f(1) - tx, f(1) - ty, f(1) - tz
+    wts = [ux * uy * uz, tx * uy * uz, ux * ty * uz, tx * ty * uz, ux * uy * tz, tx * uy * tz, ux * ty * tz,
+           tx * ty * tz]
+    idx = [(x0, y0, z0), (x1, y0, z0), (x0, y1, z0), (x1, y1, z0), (x0, y0, z1), (x1, y0, z1), (x0, y1, z1),
+           (x1, y1, z1)]
+    out = []
+    for a in fields:
+        a = np.asarray(a, f).reshape(-1)
+        acc = None
+        for wk, (xi, yi, zi) in zip(wts, idx):
+            term = (wk * a[xi + snx * (yi + sny * zi)]).astype(f)
+            acc = term if acc is None else (acc + term).astype(f)
+        out.append(acc)
+    return out
