@@ -80,7 +80,8 @@ class FcPerfStats(C.Structure):
 class FcWallModelReadout(C.Structure):
     _fields_ = [("enabled", C.c_int), ("mode", C.c_int), ("listed", C.c_int), ("cells", C.c_int),
                 ("mean_yplus", C.c_float), ("max_yplus", C.c_float), ("clamped_frac", C.c_float),
-                ("excluded_vg", C.c_int), ("degenerate", C.c_int), ("mlp_cells", C.c_int)]
+                ("excluded_vg", C.c_int), ("degenerate", C.c_int), ("mlp_cells", C.c_int),
+                ("from_fine", C.c_int)]
 
 
 class FcMlpParams(C.Structure):
@@ -307,6 +308,12 @@ SIGNATURES = {
     "fc_snapshot_cache_flush": (C.c_longlong, [_vp]),
     "fc_snapshot_cache_set_budget": (C.c_int, [_vp, C.c_uint64]),
     "fc_snapshot_cache_used_bytes": (C.c_uint64, [_vp]),
+    "fc_wall_cell_count_level": (C.c_int, [_vp, C.c_int]),
+    "fc_read_wall_slip_level": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16),
+                                          C.POINTER(C.c_uint16), _fp]),
+    "fc_set_refined_surface_reference": (C.c_int, [_vp, _u8p]),
+    "fc_set_refined_finer_surface_reference": (C.c_int, [_vp, _u8p]),
+    "fc_adaptive_steps_for_budget": (C.c_int, [_vp, C.c_double]),
     "fc_seed_from_coarse": (C.c_int, [_vp, _vp]),
     "fc_preconverge": (C.c_int, [_vp, FcDims, C.POINTER(FcScaling), _u8p, C.POINTER(FcInitOptions),
                                  C.POINTER(C.c_longlong), C.c_char_p, C.c_int]),

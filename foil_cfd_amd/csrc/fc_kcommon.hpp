@@ -244,6 +244,17 @@ struct PopPtr {
     }
 };
 
+// Cache-policy bits of the population buffer accesses (gfx950 CPol: bit 0
+// sc0, bit 1 nt, bit 4 sc1); 0 = the default policy.
+#ifndef FC_POP_LOAD_POLICY
+#define FC_POP_LOAD_POLICY 0
+#endif
+#ifndef FC_POP_STORE_POLICY
+#define FC_POP_STORE_POLICY 0
+#endif
+inline constexpr int kPopLoadPolicy = FC_POP_LOAD_POLICY;
+inline constexpr int kPopStorePolicy = FC_POP_STORE_POLICY;
+
 template <class S>
 struct PopBuf {
     using T = typename S::T;
@@ -271,9 +282,9 @@ struct PopBuf {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0)));
+            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, kPopLoadPolicy)));
         else
-            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0)));
+            return S::fromBits(static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy)));
     }
     // Raw storage bits of a slot (decoded later with S::fromBits), so a load
     // can stay in flight across other work.
@@ -281,17 +292,27 @@ struct PopBuf {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0));
+            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, kPopLoadPolicy));
         else
-            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0));
+            return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy));
+    }
+    // Direct-to-LDS load of one slot per lane: the value lands zero-extended
+    // in the dword at lds + 4 * lane (the wave's LDS base must be uniform).
+    __device__ void ldLds(int q, int ex, int ey, int ez, __attribute__((address_space(3))) void* lds) const {
+        const int vo = static_cast<int>(vx[ex + 1]);
+        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
+        if constexpr (sizeof(Bits) == 2)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, lds, 2, vo, so, 0, kPopLoadPolicy);
+        else
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, lds, 4, vo, so, 0, kPopLoadPolicy);
     }
     __device__ void st(int q, int ex, int ey, int ez, C v) const {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            __builtin_amdgcn_raw_buffer_store_b16(S::toBits(v), rs, vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(S::toBits(v), rs, vo, so, kPopStorePolicy);
         else
-            __builtin_amdgcn_raw_buffer_store_b32(S::toBits(v), rs, vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(S::toBits(v), rs, vo, so, kPopStorePolicy);
     }
 
     // Two consecutive slots (x+ex, x+ex+1) of one direction and row in one
@@ -314,17 +335,17 @@ struct PopBuf {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0);
+            return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy);
         else
-            return __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0);
+            return __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, kPopLoadPolicy);
     }
     __device__ void st2(int q, int ex, int ey, int ez, Pair v) const {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            __builtin_amdgcn_raw_buffer_store_b32(v, rs, vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(v, rs, vo, so, kPopStorePolicy);
         else
-            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, so, kPopStorePolicy);
     }
 };
 

@@ -20,7 +20,9 @@ namespace fc {
 // where the grid allows it (even nx, non-periodic x / y, array under 2 GiB),
 // kStepPipe runs the software-pipelined persistent kernel (one cell per lane,
 // next task's loads in flight during the collision; arrays under 2 GiB).
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3 };
+// kStepStage runs the LDS-staged persistent kernel (next task's loads staged
+// through direct-to-LDS buffer loads; arrays under 2 GiB).
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3, kStepStage = 4 };
 
 struct StepArgs {
     float tau0 = 0.6f;

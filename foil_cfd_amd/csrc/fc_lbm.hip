@@ -405,6 +405,81 @@ __global__ void __launch_bounds__(kBlock) stepKernelPipe(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged persistent step. Same task order as the pipelined kernel, but
+// the next task's 19 pull loads and kind bytes travel by direct-to-LDS buffer
+// loads into the wave's own 5 KiB LDS slot, so the prefetch holds no VGPRs
+// and eight waves still fit per SIMD (20 KiB per workgroup, 160 KiB per CU).
+// Per task: wait for the staged loads, copy them into registers, issue the
+// next task's loads into the same slot, then collide and store.
+// ---------------------------------------------------------------------------
+inline constexpr int kStageRows = kQ + 1;  // 19 populations + the kind bytes
+using LdsPtr = __attribute__((address_space(3))) void*;
+
+// s_waitcnt immediates of the gfx9 encoding: vmcnt(0) with the other
+// counters left open, and lgkmcnt(0) with vmcnt / expcnt left open.
+inline constexpr int kWaitVm0 = 0x0F70;
+inline constexpr int kWaitLgkm0 = 0xC07F;
+
+template <class S, bool FROM_L, bool PX, bool PY>
+__device__ inline void stageIssue(const KArgs& a, int t, unsigned* slot) {
+    RowPos q;
+    pipeLoadPos<PX, PY>(a, t, q);
+    const PopBuf<S> pop(a, q);
+    staticFor<kQ>([&](auto qc) {
+        using L = PullSlot<FROM_L, decltype(qc)::value>;
+        pop.ldLds(L::dir, L::ex, L::ey, L::ez, (LdsPtr)(slot + decltype(qc)::value * kWave));
+    });
+    // kind bytes through the same path: an untyped byte buffer over the array
+    const __amdgpu_buffer_rsrc_t kr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<std::uint8_t*>(a.kind), 0, 0x7FFFFFFF, 0x00020000);
+    const unsigned rowOff = static_cast<unsigned>(q.R[1][1]);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(kr, (LdsPtr)(slot + kQ * kWave), 1, static_cast<int>(q.x),
+                                             static_cast<int>(rowOff), 0, 0);
+}
+
+template <class S, bool FROM_L, bool WM, bool PX, bool PY>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+stepKernelStage(KArgs a) {
+    using C = typename S::C;
+    using Bits = typename S::Bits;
+    __shared__ unsigned stage[kWavesPerBlock][kStageRows * kWave];
+    const int hb = blockIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int W = a.pipeWaves;
+    const int i = (hb >> 3) * kWavesPerBlock + wv;
+    const int t0 = (hb & (kXcds - 1)) * a.pipeTasks + i;
+    const int tEnd = min((hb & (kXcds - 1)) * a.pipeTasks + a.pipeTasks, a.nwaves);
+    if (i >= W || t0 >= tEnd) return;
+    const int n = (tEnd - t0 + W - 1) / W;
+    unsigned* slot = stage[wv];
+    stageIssue<S, FROM_L, PX, PY>(a, t0, slot);
+    for (int j = 0; j < n; ++j) {
+        const int t = t0 + j * W;
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        Bits B[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) B[q] = static_cast<Bits>(slot[q * kWave + lane]);
+        const std::uint8_t k = static_cast<std::uint8_t>(slot[kQ * kWave + lane]);
+        // the copies must have left LDS before the next loads overwrite it
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        if (j + 1 < n) stageIssue<S, FROM_L, PX, PY>(a, t + W, slot);
+        RowPos p;
+        if (!taskPos<PX, PY>(a, t, p)) continue;
+        RowPos q = p;
+        if (!PX) {
+            q.xm = max(q.x - 1, 0);
+            q.xp = min(q.x + 1, a.nx - 1);
+        }
+        const PopBuf<S> pop(a, q);
+        C H[kQ];
+#pragma unroll
+        for (int r = 0; r < kQ; ++r) H[r] = S::fromBits(B[r]);
+        cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, p, pop, k, H);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched step of a group of independent lattices (e.g. the cases of a polar
 // sweep, each with its own grid size, layout parity, relaxation time and
 // inlet speed) in one launch. The row-segment tasks of all members are
@@ -1004,6 +1079,35 @@ struct PipeStep {
     };
 };
 
+template <class S>
+struct StageStep {
+    template <bool FROM_L, bool WM, bool PX, bool PY>
+    struct K {
+        static void launch(dim3, dim3 b, hipStream_t st, KArgs a) {
+            static int occ = 0;
+            if (occ == 0) {
+                int o = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &o, reinterpret_cast<const void*>(&stepKernelStage<S, FROM_L, WM, PX, PY>), kBlock, 0) !=
+                    hipSuccess)
+                    (void)hipGetLastError();
+                occ = o > 0 ? o : 1;
+            }
+            int dev = 0, cus = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+                (void)hipGetLastError();
+                cus = 256;
+            }
+            const int perXcd = (cus + kXcds - 1) / kXcds;
+            a.pipeTasks = (a.nwaves + kXcds - 1) / kXcds;
+            const int blocksPerXcd = min((a.pipeTasks + kWavesPerBlock - 1) / kWavesPerBlock, perXcd * occ);
+            a.pipeWaves = blocksPerXcd * kWavesPerBlock;
+            stepKernelStage<S, FROM_L, WM, PX, PY><<<dim3(blocksPerXcd * kXcds), b, 0, st>>>(a);
+        }
+    };
+};
+
 // Buffer addressing whenever every population byte offset fits in 31 bits
 // (FP16 / BF16 up to 56 M cells, FP32 up to 28 M); FP64 and larger arrays
 // take the flat-pointer accessor. The two-cells-per-lane kernel needs an even
@@ -1029,6 +1133,7 @@ int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, int variant
                 return lastError();
             }
             if (variant == kStepPipe) return dispatchStepK<PipeStep<S>::template K>(a, lat, layout, st);
+            if (variant == kStepStage) return dispatchStepK<StageStep<S>::template K>(a, lat, layout, st);
             return dispatchStepK<BufStep<S>::template K>(a, lat, layout, st);
         }
     }

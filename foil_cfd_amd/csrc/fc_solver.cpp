@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kWatchdogPeriodSteps = 200;     // lbm_solver.cpp:25
 constexpr float kForceGateFlowThroughs = 2.0f;  // lbm_solver.h:25
+constexpr int kMaxStepsPerBatch = 4096;         // lbm_solver.cpp:38
 constexpr float kWarmGateFlowThroughs = 0.5f;   // lbm_solver.cpp:30
 constexpr long long kCompactSettleSteps = 1000; // lbm_solver.cpp:34
 constexpr double kTimingEmaAlpha = 0.25;        // lbm_solver.cpp:41
@@ -135,6 +136,8 @@ Airfoil airfoilFrom(const float* xy, int npts) {
 // ---------------------------------------------------------------------------
 class Solver {
 public:
+    struct WMLevel;    // one level's wall-cell list and slip field
+    struct FineLevel;  // a refinement patch level
     ~Solver() { freeAll(); }
 
     std::string error;
@@ -303,13 +306,16 @@ public:
         if (finer_.active) {
             if (int fe = seedFiner(); fe != FC_OK) return fe;
         }
-        // Wall-model EMA state and slip field restart with the flow.
-        if (wm_.active) {
-            const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
-            (void)hipMemsetAsync(wm_.dUTau, 0, static_cast<std::size_t>(wm_.count) * sizeof(float), stream_);
-            (void)hipMemsetAsync(wm_.dUwx, 0, cb, stream_);
-            (void)hipMemsetAsync(wm_.dUwy, 0, cb, stream_);
-            (void)hipMemsetAsync(wm_.dUwz, 0, cb, stream_);
+        // Wall-model EMA state and slip field restart with the flow on every
+        // level (resetWallModelState, lbm_solver.cpp:456-468).
+        for (int level = 0; level < 3; ++level) {
+            const WMLevel* w = wallLevel(level);
+            if (!w || !w->active) continue;
+            const std::size_t cb = static_cast<std::size_t>(levelCells(level)) * sizeof(std::uint16_t);
+            (void)hipMemsetAsync(w->dUTau, 0, static_cast<std::size_t>(w->count) * sizeof(float), stream_);
+            (void)hipMemsetAsync(w->dUwx, 0, cb, stream_);
+            (void)hipMemsetAsync(w->dUwy, 0, cb, stream_);
+            (void)hipMemsetAsync(w->dUwz, 0, cb, stream_);
         }
         return FC_OK;
     }
@@ -329,7 +335,7 @@ public:
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
     int setStepVariant(int v) {
-        if (v != kStepAuto && v != kStepSingle && v != kStepPair && v != kStepPipe)
+        if (v != kStepAuto && v != kStepSingle && v != kStepPair && v != kStepPipe && v != kStepStage)
             return fail(FC_ERR_INVALID, "unknown step variant");
         stepVariant_ = v;
         return FC_OK;
@@ -401,8 +407,8 @@ public:
             }
         }
         // One wall-model update per batch on the state the batch produced;
-        // the new slip field takes effect next batch (lbm_solver.cpp:1356-1367).
-        if (wm_.active) {
+        // the new slip field takes effect next batch (lbm_solver.cpp:1356-1390).
+        if (anyWall()) {
             if (int e = wallUpdate(); e != FC_OK) return e;
         }
         return FC_OK;
@@ -628,7 +634,7 @@ public:
             surfaceRef_.assign(clean, clean + ncells_);
         }
         surfDirty_ = true;
-        return rebuildWallModel();
+        return rebuildCoarseWall();
     }
 
     int setWallHarvest(bool on) {
@@ -672,12 +678,17 @@ public:
         *r = fc_wall_model_readout{};
         r->mode = wmMode_;
         r->enabled = wmMode_ != kWallOff ? 1 : 0;
-        r->excluded_vg = wm_.stats.excludedVG;
-        r->degenerate = wm_.stats.degenerate;
-        r->listed = wm_.count;
-        if (!initialized_ || !wm_.active) return FC_OK;
+        // the level with the finest wall physics (lbm_solver.cpp:994-1010)
+        const WMLevel& wm = (finer_.active && finer_.wm.active) ? finer_.wm
+                            : (fine_.active && fine_.wm.active) ? fine_.wm
+                                                                : wm_;
+        r->from_fine = &wm != &wm_ ? 1 : 0;
+        r->excluded_vg = wm.stats.excludedVG;
+        r->degenerate = wm.stats.degenerate;
+        r->listed = wm.count;
+        if (!initialized_ || !wm.active) return FC_OK;
         WallStats st{};
-        if (hipMemcpyAsync(&st, wm_.dStats, sizeof st, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        if (hipMemcpyAsync(&st, wm.dStats, sizeof st, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
             hipStreamSynchronize(stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "wall stats readback failed");
         r->cells = st.activeCells;
@@ -711,11 +722,51 @@ public:
     // once per fc_step call).
     int wallUpdateNow() {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
-        if (!wm_.active) return fail(FC_ERR_STATE, "wall model inactive");
+        if (!anyWall()) return fail(FC_ERR_STATE, "wall model inactive");
         return wallUpdate();
     }
 
     int wallCount() const { return wm_.count; }
+    // Level 0 coarse, 1 fine, 2 finer (nullptr when that level is absent).
+    const WMLevel* wallLevel(int level) const {
+        if (level == 0) return &wm_;
+        if (level == 1 && fine_.active) return &fine_.wm;
+        if (level == 2 && finer_.active) return &finer_.wm;
+        return nullptr;
+    }
+    long long levelCells(int level) const {
+        return level == 0 ? ncells_ : level == 1 ? fine_.ncells : finer_.ncells;
+    }
+    int wallCountLevel(int level) const {
+        const WMLevel* w = wallLevel(level);
+        return w ? w->count : -1;
+    }
+    int readWallSlipLevel(int level, std::uint16_t* ux, std::uint16_t* uy, std::uint16_t* uz, float* utau) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        const WMLevel* w = wallLevel(level);
+        if (!w || !w->active) return fail(FC_ERR_STATE, "wall model inactive on this level");
+        const std::size_t b = static_cast<std::size_t>(levelCells(level)) * sizeof(std::uint16_t);
+        const std::pair<std::uint16_t*, std::uint16_t*> pairs[] = {{ux, w->dUwx}, {uy, w->dUwy}, {uz, w->dUwz}};
+        for (const auto& pr : pairs)
+            if (pr.first && hipMemcpyAsync(pr.first, pr.second, b, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+                return fail(FC_ERR_DEVICE, "slip readback failed");
+        if (utau && hipMemcpyAsync(utau, w->dUTau, static_cast<std::size_t>(w->count) * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream_) != hipSuccess)
+            return fail(FC_ERR_DEVICE, "u_tau readback failed");
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "slip readback failed");
+        return FC_OK;
+    }
+    // setRefinedSurfaceReference / setRefinedFinerSurfaceReference
+    // (lbm_solver.cpp:980-990, :2022-2030): clean flags of a patch level.
+    int setLevelSurfaceReference(int level, const std::uint8_t* clean) {
+        FineLevel* lv = level == 1 ? &fine_ : level == 2 ? &finer_ : nullptr;
+        if (!initialized_ || !lv || !lv->active) return fail(FC_ERR_STATE, "no such refinement level");
+        if (clean)
+            lv->surfaceRef.assign(clean, clean + lv->ncells);
+        else
+            lv->surfaceRef.clear();
+        return rebuildLevelWall(*lv);
+    }
 
     int synchronize() {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
@@ -1198,6 +1249,24 @@ public:
     float smagorinskyCs() const { return cs_; }
     const Scaling& baseScaling() const { return scaling_; }
 
+    // adaptiveStepsForBudget (lbm_solver.cpp:1392-1408): steps per frame
+    // from the measured EMA step time; 1 while unmeasured, growth capped at
+    // 2x per call, shrinking immediate, 0 once the watchdog has latched.
+    int adaptiveStepsForBudget(double budgetMs) {
+        if (!initialized_) return 0;
+        pollAsync();
+        if (nanLatched_) return 0;
+        int n = 1;
+        if (emaStepMs_ > 0.0) {
+            // in double before the cast (a huge budget must not overflow int)
+            n = static_cast<int>(std::clamp(budgetMs / emaStepMs_, 0.0, static_cast<double>(kMaxStepsPerBatch)));
+            if (lastChosenN_ > 0) n = std::min(n, 2 * lastChosenN_);
+            n = std::clamp(n, 1, kMaxStepsPerBatch);
+        }
+        lastChosenN_ = n;
+        return n;
+    }
+
     int nanDetected() {
         pollAsync();
         return nanLatched_ ? 1 : 0;
@@ -1211,7 +1280,7 @@ public:
         if (!p) return fail(FC_ERR_INVALID, "null output");
         pollAsync();
         *p = fc_perf_stats{};
-        p->last_steps_per_frame = timingBatchN_;
+        p->last_steps_per_frame = lastChosenN_;  // the pacer's last choice
         p->last_step_ms = lastStepMs_;
         // Cell updates per coarse step: every coarse cell once, every fine cell
         // m times (lbm_solver.cpp:1705-1713).
@@ -1323,126 +1392,130 @@ private:
 
     // Device mirror of one level's wall-cell list + slip field
     // (reference WMLevel, lbm_solver.cpp:206-243).
-    struct WMLevel {
-        bool active = false;
-        int count = 0;
-        WallListStats stats;
-        long long *dCell = nullptr, *dSample = nullptr;
-        float *dNx = nullptr, *dNy = nullptr, *dNz = nullptr, *dYs = nullptr, *dUTau = nullptr;
-        std::uint32_t* dMask = nullptr;
-        std::uint16_t *dUwx = nullptr, *dUwy = nullptr, *dUwz = nullptr;
-        WallStats* dStats = nullptr;
-        float* dHarvest = nullptr;
-        std::vector<long long> hostCells;
-        WallListView view() const {
-            WallListView v;
-            v.cellIdx = dCell;
-            v.sampleIdx = dSample;
-            v.nx = dNx;
-            v.ny = dNy;
-            v.nz = dNz;
-            v.ys = dYs;
-            v.mask = dMask;
-            v.uTau = dUTau;
-            v.count = count;
-            return v;
-        }
-        void free() {
-            hipRelease(dCell);
-            hipRelease(dSample);
-            hipRelease(dNx);
-            hipRelease(dNy);
-            hipRelease(dNz);
-            hipRelease(dYs);
-            hipRelease(dUTau);
-            hipRelease(dMask);
-            hipRelease(dUwx);
-            hipRelease(dUwy);
-            hipRelease(dUwz);
-            hipRelease(dStats);
-            hipRelease(dHarvest);
-            active = false;
-            count = 0;
-            stats = WallListStats{};
-            hostCells.clear();
-        }
-    };
 
-    // (Re)build the wall list from the live flags and the clean surface
-    // reference; listed cells get the kind wall bit (lbm_solver.cpp:295-367).
-    int rebuildWallModel() {
-        // clear old wall bits
-        for (long long c : wm_.hostCells) hostKind_[static_cast<std::size_t>(c)] &= ~kKindWall;
-        wm_.free();
+    // (Re)build one level's wall list from its live flags and its clean
+    // surface reference; listed cells get the kind wall bit (buildWMLevel and
+    // rebuildWallModelCoarse / Fine / Finer, lbm_solver.cpp:295-455).
+    int buildWallLevel(WMLevel& wm, const Dims& d, const std::vector<std::uint8_t>& flags,
+                       const std::vector<std::uint8_t>& ref, int sampleCells, std::vector<std::uint8_t>& hostKind,
+                       std::uint8_t* devKind, bool harvest) {
+        for (long long c : wm.hostCells) hostKind[static_cast<std::size_t>(c)] &= ~kKindWall;
+        wm.free();
+        const long long ncells = d.cells();
         int rc = FC_OK;
-        if (wmMode_ != kWallOff) {
-            const std::vector<std::uint8_t>& clean = surfaceRef_.size() == hostFlags_.size() ? surfaceRef_ : hostFlags_;
-            WallList list = buildWallCellList(dims_, hostFlags_, clean, kWallSampleCellsCoarse, &wm_.stats);
+        if (wmMode_ != kWallOff && static_cast<long long>(hostKind.size()) == ncells &&
+            static_cast<long long>(flags.size()) == ncells) {
+            const std::vector<std::uint8_t>& clean = ref.size() == flags.size() ? ref : flags;
+            WallList list = buildWallCellList(d, flags, clean, sampleCells, &wm.stats);
             const std::size_t n = list.size();
             if (n > 0) {
-                const std::size_t cb = static_cast<std::size_t>(ncells_) * sizeof(std::uint16_t);
-                rc |= hipAlloc(&wm_.dCell, n * sizeof(long long));
-                rc |= hipAlloc(&wm_.dSample, n * sizeof(long long));
-                rc |= hipAlloc(&wm_.dNx, n * sizeof(float));
-                rc |= hipAlloc(&wm_.dNy, n * sizeof(float));
-                rc |= hipAlloc(&wm_.dNz, n * sizeof(float));
-                rc |= hipAlloc(&wm_.dYs, n * sizeof(float));
-                rc |= hipAlloc(&wm_.dUTau, n * sizeof(float));
-                rc |= hipAlloc(&wm_.dMask, n * sizeof(std::uint32_t));
-                rc |= hipAlloc(&wm_.dUwx, cb);
-                rc |= hipAlloc(&wm_.dUwy, cb);
-                rc |= hipAlloc(&wm_.dUwz, cb);
-                rc |= hipAlloc(&wm_.dStats, sizeof(WallStats));
+                const std::size_t cb = static_cast<std::size_t>(ncells) * sizeof(std::uint16_t);
+                rc |= hipAlloc(&wm.dCell, n * sizeof(long long));
+                rc |= hipAlloc(&wm.dSample, n * sizeof(long long));
+                rc |= hipAlloc(&wm.dNx, n * sizeof(float));
+                rc |= hipAlloc(&wm.dNy, n * sizeof(float));
+                rc |= hipAlloc(&wm.dNz, n * sizeof(float));
+                rc |= hipAlloc(&wm.dYs, n * sizeof(float));
+                rc |= hipAlloc(&wm.dUTau, n * sizeof(float));
+                rc |= hipAlloc(&wm.dMask, n * sizeof(std::uint32_t));
+                rc |= hipAlloc(&wm.dUwx, cb);
+                rc |= hipAlloc(&wm.dUwy, cb);
+                rc |= hipAlloc(&wm.dUwz, cb);
+                rc |= hipAlloc(&wm.dStats, sizeof(WallStats));
                 if (rc != FC_OK) {
-                    wm_.free();
+                    wm.free();
                     return fail(FC_ERR_ALLOC, "wall-model allocation failed");
                 }
                 const std::pair<void*, std::pair<const void*, std::size_t>> ups[] = {
-                    {wm_.dCell, {list.cellIdx.data(), n * sizeof(long long)}},
-                    {wm_.dSample, {list.sampleIdx.data(), n * sizeof(long long)}},
-                    {wm_.dNx, {list.nx.data(), n * sizeof(float)}},
-                    {wm_.dNy, {list.ny.data(), n * sizeof(float)}},
-                    {wm_.dNz, {list.nz.data(), n * sizeof(float)}},
-                    {wm_.dYs, {list.sampleDist.data(), n * sizeof(float)}},
-                    {wm_.dMask, {list.linkMask.data(), n * sizeof(std::uint32_t)}},
+                    {wm.dCell, {list.cellIdx.data(), n * sizeof(long long)}},
+                    {wm.dSample, {list.sampleIdx.data(), n * sizeof(long long)}},
+                    {wm.dNx, {list.nx.data(), n * sizeof(float)}},
+                    {wm.dNy, {list.ny.data(), n * sizeof(float)}},
+                    {wm.dNz, {list.nz.data(), n * sizeof(float)}},
+                    {wm.dYs, {list.sampleDist.data(), n * sizeof(float)}},
+                    {wm.dMask, {list.linkMask.data(), n * sizeof(std::uint32_t)}},
                 };
                 for (const auto& u : ups)
                     if (hipMemcpyAsync(u.first, u.second.first, u.second.second, hipMemcpyHostToDevice, stream_) !=
                         hipSuccess)
                         rc = FC_ERR_DEVICE;
-                (void)hipMemsetAsync(wm_.dUTau, 0, n * sizeof(float), stream_);
-                (void)hipMemsetAsync(wm_.dUwx, 0, cb, stream_);
-                (void)hipMemsetAsync(wm_.dUwy, 0, cb, stream_);
-                (void)hipMemsetAsync(wm_.dUwz, 0, cb, stream_);
-                (void)hipMemsetAsync(wm_.dStats, 0, sizeof(WallStats), stream_);
-                wm_.count = static_cast<int>(n);
-                wm_.hostCells = list.cellIdx;
-                for (long long c : wm_.hostCells) hostKind_[static_cast<std::size_t>(c)] |= kKindWall;
-                wm_.active = true;
-                if (harvestOn_) {
-                    if (hipAlloc(&wm_.dHarvest, n * 4 * sizeof(float)) != FC_OK) rc = FC_ERR_ALLOC;
+                (void)hipMemsetAsync(wm.dUTau, 0, n * sizeof(float), stream_);
+                (void)hipMemsetAsync(wm.dUwx, 0, cb, stream_);
+                (void)hipMemsetAsync(wm.dUwy, 0, cb, stream_);
+                (void)hipMemsetAsync(wm.dUwz, 0, cb, stream_);
+                (void)hipMemsetAsync(wm.dStats, 0, sizeof(WallStats), stream_);
+                wm.count = static_cast<int>(n);
+                wm.hostCells = list.cellIdx;
+                for (long long c : wm.hostCells) hostKind[static_cast<std::size_t>(c)] |= kKindWall;
+                wm.active = true;
+                if (harvest) {
+                    if (hipAlloc(&wm.dHarvest, n * 4 * sizeof(float)) != FC_OK) rc = FC_ERR_ALLOC;
                 }
             }
         }
-        if (hipMemcpyAsync(kind_, hostKind_.data(), hostKind_.size(), hipMemcpyHostToDevice, stream_) != hipSuccess)
+        if (devKind && !hostKind.empty() &&
+            hipMemcpyAsync(devKind, hostKind.data(), hostKind.size(), hipMemcpyHostToDevice, stream_) != hipSuccess)
             rc = FC_ERR_DEVICE;
         if (hipStreamSynchronize(stream_) != hipSuccess) rc = FC_ERR_DEVICE;
         if (rc != FC_OK) {
-            wm_.free();
+            wm.free();
             return fail(rc, "wall-model upload failed");
         }
         return FC_OK;
     }
+    int rebuildCoarseWall() {
+        return buildWallLevel(wm_, dims_, hostFlags_, surfaceRef_, kWallSampleCellsCoarse, hostKind_, kind_,
+                              harvestOn_);
+    }
+    int rebuildLevelWall(FineLevel& lv) {
+        return buildWallLevel(lv.wm, lv.dims, lv.hostFlags, lv.surfaceRef, kWallSampleCellsFine, lv.hostKind, lv.kind,
+                              false);
+    }
+    // Every level (setWallModelEnabled: coarse, and the patch levels when active).
+    int rebuildWallModel() {
+        if (int e = rebuildCoarseWall(); e != FC_OK) return e;
+        if (fine_.active) {
+            if (int e = rebuildLevelWall(fine_); e != FC_OK) return e;
+        }
+        if (finer_.active) {
+            if (int e = rebuildLevelWall(finer_); e != FC_OK) return e;
+        }
+        return FC_OK;
+    }
+    bool anyWall() const { return wm_.active || (fine_.active && fine_.wm.active) || (finer_.active && finer_.wm.active); }
 
+    // One update of every level that carries a wall list, each at its own
+    // viscosity (the fine level's acoustically scaled tau, the finer level's
+    // composed one), cutoff from the coarse u_lat (lbm_solver.cpp:1356-1390).
     int wallUpdate() {
         WallParams p;
-        p.nuLat = (effectiveTau() - 0.5f) / 3.0f;
         p.utCutoff = 1e-3f * scaling_.u_lat;
         p.mode = wmMode_;
         p.mlp = dMlp_;
-        if (launchWallUpdate(view(), layout_, wm_.view(), wm_.dUwx, wm_.dUwy, wm_.dUwz, p, wm_.dStats,
-                             wm_.dHarvest, stream_) != 0)
-            return fail(FC_ERR_DEVICE, "wall-model update failed");
+        const float tauC = effectiveTau();
+        if (wm_.active) {
+            p.nuLat = (tauC - 0.5f) / 3.0f;
+            if (launchWallUpdate(view(), layout_, wm_.view(), wm_.dUwx, wm_.dUwy, wm_.dUwz, p, wm_.dStats,
+                                 wm_.dHarvest, stream_) != 0)
+                return fail(FC_ERR_DEVICE, "wall-model update failed");
+        }
+        if (fine_.active) {
+            const float tauF = fineTauFor(tauC, fine_.factor);
+            if (fine_.wm.active) {
+                p.nuLat = (tauF - 0.5f) / 3.0f;
+                WMLevel& w = fine_.wm;
+                if (launchWallUpdate(viewOf(fine_), fine_.layout, w.view(), w.dUwx, w.dUwy, w.dUwz, p, w.dStats,
+                                     nullptr, stream_) != 0)
+                    return fail(FC_ERR_DEVICE, "fine wall-model update failed");
+            }
+            if (finer_.active && finer_.wm.active) {
+                p.nuLat = (fineTauFor(tauF, finer_.factor) - 0.5f) / 3.0f;
+                WMLevel& w = finer_.wm;
+                if (launchWallUpdate(viewOf(finer_), finer_.layout, w.view(), w.dUwx, w.dUwy, w.dUwz, p, w.dStats,
+                                     nullptr, stream_) != 0)
+                    return fail(FC_ERR_DEVICE, "finer wall-model update failed");
+            }
+        }
         return FC_OK;
     }
 
@@ -1570,7 +1643,7 @@ private:
         // The host vectors die at return: finish the copies first.
         if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "flag upload failed");
         // Geometry changed: the wall-cell list is keyed to the flags.
-        if (wmMode_ != kWallOff) return rebuildWallModel();
+        if (wmMode_ != kWallOff) return rebuildCoarseWall();
         return FC_OK;
     }
 
@@ -1592,6 +1665,51 @@ private:
 public:
     // ---- two-level refinement (reference initRefinement / stepN coupling,
     // lbm_solver.cpp:1816-1895, :1172-1290) ----------------------------------
+    struct WMLevel {
+        bool active = false;
+        int count = 0;
+        WallListStats stats;
+        long long *dCell = nullptr, *dSample = nullptr;
+        float *dNx = nullptr, *dNy = nullptr, *dNz = nullptr, *dYs = nullptr, *dUTau = nullptr;
+        std::uint32_t* dMask = nullptr;
+        std::uint16_t *dUwx = nullptr, *dUwy = nullptr, *dUwz = nullptr;
+        WallStats* dStats = nullptr;
+        float* dHarvest = nullptr;
+        std::vector<long long> hostCells;
+        WallListView view() const {
+            WallListView v;
+            v.cellIdx = dCell;
+            v.sampleIdx = dSample;
+            v.nx = dNx;
+            v.ny = dNy;
+            v.nz = dNz;
+            v.ys = dYs;
+            v.mask = dMask;
+            v.uTau = dUTau;
+            v.count = count;
+            return v;
+        }
+        void free() {
+            hipRelease(dCell);
+            hipRelease(dSample);
+            hipRelease(dNx);
+            hipRelease(dNy);
+            hipRelease(dNz);
+            hipRelease(dYs);
+            hipRelease(dUTau);
+            hipRelease(dMask);
+            hipRelease(dUwx);
+            hipRelease(dUwy);
+            hipRelease(dUwz);
+            hipRelease(dStats);
+            hipRelease(dHarvest);
+            active = false;
+            count = 0;
+            stats = WallListStats{};
+            hostCells.clear();
+        }
+    };
+
     struct FineLevel {
         bool active = false;
         int factor = 0;
@@ -1613,6 +1731,9 @@ public:
         float* dBand = nullptr;
         int nband = 0;
         long long bytes = 0;
+        WMLevel wm;                            // the level's wall-cell list and slip field
+        std::vector<std::uint8_t> hostKind;    // host copy of the level's kind bytes
+        std::vector<std::uint8_t> surfaceRef;  // clean (VG-free) flags for wall normals
     };
 
     static RefineGeom geomOf(const Dims& parent, const FineLevel& l) {
@@ -1631,6 +1752,11 @@ public:
         l.storage = storage_;
         l.kind = lv.kind;
         l.dims = lv.dims;
+        if (lv.wm.active) {
+            l.uwx = lv.wm.dUwx;
+            l.uwy = lv.wm.dUwy;
+            l.uwz = lv.wm.dUwz;
+        }
         return l;
     }
     LatticeView fineView() const { return viewOf(fine_); }
@@ -1644,6 +1770,7 @@ public:
         hipRelease(l.dBandXY);
         hipRelease(l.dBandIndex);
         hipRelease(l.dBand);
+        l.wm.free();
         l = FineLevel{};
     }
     // The finer level nests in the fine one: both go together.
@@ -1676,6 +1803,8 @@ public:
                              hipMemcpyHostToDevice, stream_) != hipSuccess)) ||
             hipStreamSynchronize(stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "fine flag upload failed");
+        fine_.hostKind = kb.kind;
+        if (int e = rebuildLevelWall(fine_); e != FC_OK) return e;
         const int x0 = fine_.box.x0 + 2, x1 = fine_.box.x1 - 2, y0 = fine_.box.y0 + 2, y1 = fine_.box.y1 - 2;
         fine_.forcesFromFine = true;
         for (int z = 0; z < dims_.nz && fine_.forcesFromFine; ++z)
@@ -1868,6 +1997,11 @@ public:
             hipSuccess) {
             freeLevel(finer_);
             return fail(FC_ERR_DEVICE, "finer flag upload failed");
+        }
+        finer_.hostKind = kb.kind;
+        if (int e = rebuildLevelWall(finer_); e != FC_OK) {
+            freeLevel(finer_);
+            return e;
         }
         finer_.active = true;
         if (int e = seedFiner(); e != FC_OK) {
@@ -2124,6 +2258,7 @@ private:
     bool groupTimed_ = false;
     int timingBatchN_ = 0;
     double emaStepMs_ = 0.0, lastStepMs_ = 0.0;
+    int lastChosenN_ = 0;
 
     bool kernelTiming_ = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> kEvents_;
@@ -2436,6 +2571,10 @@ int fc_apply_edited_flags(fc_solver* s, const uint8_t* flags) {
 
 long long fc_last_edit_count(fc_solver* s) { return s ? s->impl.lastEditCount() : 0; }
 
+int fc_adaptive_steps_for_budget(fc_solver* s, double budget_ms) {
+    return s ? s->impl.adaptiveStepsForBudget(budget_ms) : 0;
+}
+
 int fc_seed_from_coarse(fc_solver* s, fc_solver* presolver) {
     if (!s || !presolver) return FC_ERR_INVALID;
     return s->impl.seedFromCoarse(presolver->impl);
@@ -2662,6 +2801,16 @@ int fc_read_wall_slip(fc_solver* s, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz,
 
 int fc_wall_update_now(fc_solver* s) { return s ? s->impl.wallUpdateNow() : FC_ERR_INVALID; }
 int fc_wall_cell_count(fc_solver* s) { return s ? s->impl.wallCount() : 0; }
+int fc_wall_cell_count_level(fc_solver* s, int level) { return s ? s->impl.wallCountLevel(level) : -1; }
+int fc_read_wall_slip_level(fc_solver* s, int level, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz, float* utau) {
+    return s ? s->impl.readWallSlipLevel(level, uwx, uwy, uwz, utau) : FC_ERR_INVALID;
+}
+int fc_set_refined_surface_reference(fc_solver* s, const uint8_t* clean_fine_flags) {
+    return s ? s->impl.setLevelSurfaceReference(1, clean_fine_flags) : FC_ERR_INVALID;
+}
+int fc_set_refined_finer_surface_reference(fc_solver* s, const uint8_t* clean_finer_flags) {
+    return s ? s->impl.setLevelSurfaceReference(2, clean_finer_flags) : FC_ERR_INVALID;
+}
 int fc_set_wall_harvest(fc_solver* s, int enabled) { return s ? s->impl.setWallHarvest(enabled != 0) : FC_ERR_INVALID; }
 int fc_wall_harvest(fc_solver* s, float* out, int cap) { return s ? s->impl.wallHarvest(out, cap) : -FC_ERR_INVALID; }
 

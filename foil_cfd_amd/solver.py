@@ -392,7 +392,7 @@ class LBMSolver:
         """"auto" (library default), "single" (one cell per lane), "pair"
         (two x-adjacent cells per lane where the grid allows it) or "pipe"
         (persistent, software-pipelined one-cell-per-lane kernel)."""
-        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3}[variant]
+        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3, "stage": 4}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):
@@ -406,6 +406,10 @@ class LBMSolver:
         check(self._lib.fc_synchronize(self._h), "fc_synchronize")
 
     # readouts ---------------------------------------------------------------
+    def adaptive_steps_for_budget(self, budget_ms: float = 10.0) -> int:
+        """LBMSolver::adaptiveStepsForBudget (frame pacer)."""
+        return int(self._lib.fc_adaptive_steps_for_budget(self._h, float(budget_ms)))
+
     # mesh sequencing ---------------------------------------------------------
     def seed_from_coarse(self, presolver: "LBMSolver") -> None:
         """LBMSolver::seedFromCoarse: upsampled presolve field as the state."""
@@ -561,6 +565,30 @@ class LBMSolver:
         check(self._lib.fc_read_wall_slip(self._h, ux.ctypes.data_as(u16), uy.ctypes.data_as(u16),
                                           uz.ctypes.data_as(u16), _f32(utau)), "fc_read_wall_slip")
         return ux, uy, uz, utau[: self.wall_cell_count()]
+
+    def wall_cell_count_level(self, level: int) -> int:
+        return int(self._lib.fc_wall_cell_count_level(self._h, int(level)))
+
+    def read_wall_slip_level(self, level: int):
+        """(uwx, uwy, uwz half bits, u_tau) of level 0 coarse / 1 fine / 2 finer."""
+        n = {0: self.ncells}.get(level)
+        if n is None:
+            info = self.refinement_info() if level == 1 else self.finer_refinement_info()
+            fd = info["fine_dims"] if level == 1 else info["finer_dims"]
+            n = fd[0] * fd[1] * fd[2]
+        cnt = self.wall_cell_count_level(level)
+        u16 = C.POINTER(C.c_uint16)
+        ux, uy, uz = (np.empty(n, np.uint16) for _ in range(3))
+        ut = np.empty(max(cnt, 1), np.float32)
+        check(self._lib.fc_read_wall_slip_level(self._h, int(level), ux.ctypes.data_as(u16), uy.ctypes.data_as(u16),
+                                                uz.ctypes.data_as(u16), _f32(ut)),
+              "fc_read_wall_slip_level", self.last_error())
+        return ux, uy, uz, ut[:cnt]
+
+    def set_refined_surface_reference(self, clean_fine, finer: bool = False):
+        c = None if clean_fine is None else np.ascontiguousarray(clean_fine, dtype=np.uint8).reshape(-1)
+        fn = self._lib.fc_set_refined_finer_surface_reference if finer else self._lib.fc_set_refined_surface_reference
+        check(fn(self._h, None if c is None else _u8(c)), "fc_set_refined_surface_reference", self.last_error())
 
     def wall_update_now(self):
         check(self._lib.fc_wall_update_now(self._h), "fc_wall_update_now")

@@ -320,6 +320,7 @@ typedef struct fc_wall_model_readout {
     float mean_yplus, max_yplus, clamped_frac;
     int excluded_vg, degenerate;
     int mlp_cells;
+    int from_fine;   /* the readout is of a patch level (the finest with a wall list) */
 } fc_wall_model_readout;
 
 /* Learned u_tau network: 3 -> 32 -> 32 -> 1 tanh MLP on normalised
@@ -358,6 +359,17 @@ int fc_read_wall_slip(fc_solver* s, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz,
 /* Run one wall-model update on the current state now (fc_step runs one per call). */
 int fc_wall_update_now(fc_solver* s);
 int fc_wall_cell_count(fc_solver* s);
+/* Per level (0 coarse, 1 fine patch, 2 nested finer patch): wall-list size
+ * (-1 when the level is absent) and the level's slip field (its cells) and
+ * u_tau. Patch levels list their wall cells at a sample distance of 3 cells
+ * and update at their own viscosity (lbm_solver.cpp:425-455, :1368-1389). */
+int fc_wall_cell_count_level(fc_solver* s, int level);
+int fc_read_wall_slip_level(fc_solver* s, int level, uint16_t* uwx, uint16_t* uwy, uint16_t* uwz, float* utau);
+/* setRefinedSurfaceReference (lbm_solver.h:302) and its finer analog
+ * (lbm_solver.cpp:2022): clean VG-free flags of the patch level (NULL =
+ * the level's live flags). */
+int fc_set_refined_surface_reference(fc_solver* s, const uint8_t* clean_fine_flags);
+int fc_set_refined_finer_surface_reference(fc_solver* s, const uint8_t* clean_finer_flags);
 /* Harvest training samples (y_s, |u_t|, nu, Newton u_tau; u_tau < 0 = none)
  * from every update while enabled; fc_wall_harvest copies the last update's
  * samples (4 floats each) and returns the count (out NULL: count only). */
@@ -556,6 +568,12 @@ int fc_snapshot_cache_store_async(fc_snapshot_cache* c, const fc_compact_snapsho
 long long fc_snapshot_cache_flush(fc_snapshot_cache* c);
 int fc_snapshot_cache_set_budget(fc_snapshot_cache* c, uint64_t budget_bytes);
 uint64_t fc_snapshot_cache_used_bytes(const fc_snapshot_cache* c);
+
+/* adaptiveStepsForBudget (lbm_solver.h:363, lbm_solver.cpp:1392-1408): steps
+ * for the next frame within budget_ms (reference default 10 ms) from the
+ * measured per-step time: 1 until measured, at most 2x the last choice,
+ * at most 4096, 0 while the NaN watchdog is latched. */
+int fc_adaptive_steps_for_budget(fc_solver* s, double budget_ms);
 
 /* ---- mesh sequencing (lbm_solver.h:258, src/main.cpp:756-817) ------------- */
 /* seedFromCoarse (lbm_solver.cpp:2062-2102): trilinear upsample of the

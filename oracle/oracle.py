@@ -93,6 +93,9 @@ def lib():
         L.orc_run_refined_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                           _u8p] + [C.c_int] * 6 + [_dp, _dp, C.c_double,
                                                                    C.c_double] + [_dp] * 4
+        L.orc_run_refined_slip_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                               _u8p] + [C.c_int] * 6 + [_dp, _dp, C.c_double,
+                                                                        C.c_double] + [_dp] * 4 + [_u16p] * 6
         L.orc_run_refined3_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                            _u8p] + [C.c_int] * 5 + [_dp, _dp, _u8p] + [C.c_int] * 5 + \
             [C.c_int, _dp, _dp, C.c_double, C.c_double] + [_dp] * 4
@@ -424,13 +427,32 @@ def seed_fine(cf, cflags, cdims, fflags, box, m, tau_c, f64=True):
     return ff
 
 
-def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.0, cs=0.1733):
+def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.0, cs=0.1733,
+                coarse_slip=None, fine_slip=None):
     """len(taus) refined steps (coarse + m fine sub-steps + restriction,
     lbm_solver.cpp:1144-1290); cf / ff are advanced in place. Returns the
-    coarse (rho, u, v, w) of the last step."""
+    coarse (rho, u, v, w) of the last step. coarse_slip / fine_slip: optional
+    wall-model slip fields (3 half-bit arrays per level, FP64 runs only)."""
     nx, ny, nz = cdims
     n = nx * ny * nz
     f64 = cf.dtype == np.float64
+    if coarse_slip is not None or fine_slip is not None:
+        assert f64 and ff.dtype == np.float64
+        ctmp, ftmp = np.empty_like(cf), np.empty_like(ff)
+        taus = np.ascontiguousarray(taus, dtype=np.float64)
+        uins = np.ascontiguousarray(uins, dtype=np.float64)
+        m4 = [np.zeros(n, dtype=np.float64) for _ in range(4)]
+        sl = [None if a is None else [np.ascontiguousarray(x, np.uint16) for x in a]
+              for a in (coarse_slip, fine_slip)]
+        ptrs = []
+        for a in sl:
+            ptrs += [None] * 3 if a is None else [_p(x, _u16p) for x in a]
+        lib().orc_run_refined_slip_f64(_p(cf, _dp), _p(ctmp, _dp), _p(np.ascontiguousarray(cflags, np.uint8), _u8p),
+                                       nx, ny, nz, _p(ff, _dp), _p(ftmp, _dp),
+                                       _p(np.ascontiguousarray(fflags, np.uint8), _u8p), *box, m, len(taus),
+                                       _p(taus, _dp), _p(uins, _dp), lam, float(smag_prefactor(cs)),
+                                       *[_p(a, _dp) for a in m4], *ptrs)
+        return tuple(m4)
     dt, pt = (np.float64, _dp) if f64 else (np.float32, _fp)
     assert ff.dtype == cf.dtype
     ctmp, ftmp = np.empty_like(cf), np.empty_like(ff)

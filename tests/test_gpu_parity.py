@@ -305,22 +305,24 @@ def test_pair_kernel_matches_single_cell_kernel(gpu, storage):
         assert rel_l2(a, b) < 1e-5
 
 
+@pytest.mark.parametrize("variant", ["pipe", "stage"])
 @pytest.mark.parametrize("storage", ["f16c", "f16", "bf16", "f32"])
 @pytest.mark.parametrize("flow", ["tunnel", "tunnel+wall", "periodic"])
-def test_pipelined_kernel_matches_single_cell_kernel(gpu, storage, flow):
-    """The persistent software-pipelined step runs the same cell update as the
+def test_pipelined_kernel_matches_single_cell_kernel(gpu, storage, flow, variant):
+    """The persistent software-pipelined step (register prefetch, "pipe") and
+    the LDS-staged persistent step ("stage") run the same cell update as the
     one-cell-per-lane step: populations and macro fields bit-identical after
     an odd number of steps (both AA layouts), on a tunnel with boundary
     waves, with the wall model's slip term, and on the periodic box (the step
     kernels place every fused multiply-add explicitly, so variants round
     identically)."""
-    say(f"case: pipe vs single kernel, {flow} {storage} 61 steps")
+    say(f"case: {variant} vs single kernel, {flow} {storage} 61 steps")
     if flow == "periodic":
         case = cases.taylor_green(48)
     else:
         case = cases.naca_tunnel(320, 64, 16, 64, aoa=8.0, cs=0.1, u_lat=0.05, tau=0.56)
     out = {}
-    for var in ("single", "pipe"):
+    for var in ("single", variant):
         s = cases.make_solver(case, storage)
         s.set_step_variant(var)
         if flow == "tunnel+wall":
@@ -332,7 +334,7 @@ def test_pipelined_kernel_matches_single_cell_kernel(gpu, storage, flow):
         s.synchronize()
         out[var] = (s.download_f(f64=True), s.read_macro())
         s.close()
-    fp, mp = out["pipe"]
+    fp, mp = out[variant]
     fs, ms = out["single"]
     same_nan = np.isnan(fp) & np.isnan(fs)
     diff = int(np.count_nonzero((fp != fs) & ~same_nan))

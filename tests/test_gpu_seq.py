@@ -116,3 +116,33 @@ def test_seed_with_fine_patch(gpu):
     assert not s.nan_detected()
     for x in (pre, s):
         x.close()
+
+
+def test_adaptive_steps_pacer(gpu):
+    """adaptiveStepsForBudget (lbm_solver.cpp:1392-1408): 1 until a batch has
+    been timed, then budget / (EMA ms per step) with growth capped at 2x per
+    call and the 4096 ceiling; 0 once the watchdog has latched."""
+    say("case: frame pacer on a 96x48x8 tunnel, f32")
+    case = cases.naca_tunnel(96, 48, 8, 32, aoa=8.0, cs=0.1, u_lat=0.05, tau=0.56, ramp=False)
+    s = cases.make_solver(case, "f32")
+    assert s.adaptive_steps_for_budget(10.0) == 1
+    s.step_n(64)
+    s.synchronize()
+    s.step_n(64)
+    s.synchronize()
+    assert s.perf_stats().last_step_ms > 0
+    seq = [s.adaptive_steps_for_budget(10.0) for _ in range(16)]
+    say(f"  choices {seq}")
+    assert seq[0] <= 2 and all(b <= 2 * a for a, b in zip(seq, seq[1:]))
+    assert seq[-1] == seq[-2] and 1 <= seq[-1] <= 4096  # settled at budget / EMA
+    assert s.perf_stats().last_steps_per_frame == seq[-1]
+    assert s.adaptive_steps_for_budget(1e9) == min(4096, 2 * seq[-1])
+    assert s.adaptive_steps_for_budget(1e-9) == 1  # shrinking is immediate
+    f = s.download_f(f64=False).reshape(Q, case.dims[2], case.dims[1], case.dims[0])
+    f[:, 3, 1:-1, 1:-1] = np.nan
+    s.upload_f(f.reshape(-1))
+    s.step_n(400)
+    s.synchronize()
+    assert s.nan_detected()
+    assert s.adaptive_steps_for_budget(10.0) == 0
+    s.close()

@@ -134,3 +134,61 @@ def test_wall_model_fp16_runs(gpu):
     assert r.cells > 0 and np.isfinite(r.mean_yplus)
     assert not s.nan_detected()
     s.close()
+
+
+def test_fine_level_wall_model(gpu):
+    """Wall model on the refinement patch (rebuildWallModelFine and the fine
+    update, lbm_solver.cpp:425-438, :1368-1377): the fine list at sample
+    distance 3, its update at the fine viscosity against the oracle, the fine
+    readout, and one coupled step with both levels' slip fields against the
+    oracle's refined run fed the same fields."""
+    say("case: fine-level wall model, refined tunnel 160x64x8, f64")
+    case = cases.naca_tunnel(160, 64, 8, 32, aoa=8.0, cs=0.1, u_lat=0.05, tau=0.56, ramp=False)
+    box, fine = cases.refinement_patch(case)
+    s = cases.make_solver(case, "f64")
+    s.init_refinement(box, 2, fine)
+    s.step_n(120)
+    s.synchronize()
+    s.set_wall_model("newton")
+    fd = orc.fine_dims(box, case.dims, 2)
+    nf = s.wall_cell_count_level(1)
+    wl, _ = build_wall_cell_list(fd, fine, sample_cells=3)
+    assert nf == len(wl["cell"]) and nf > 100
+    assert s.wall_cell_count_level(2) == -1
+    ff = s.download_fine_f(f64=True)
+    s.wall_update_now()
+    ux, uy, uz, ut = s.read_wall_slip_level(1)
+    tau_f = orc.fine_tau_for(case.scaling.tau, 2)
+    nu = (tau_f - 0.5) / 3.0
+    ut_o = np.zeros(nf, np.float32)
+    ox, oy, oz, ost = orc.wall_update(ff.astype(np.float32), fd, wl, ut_o, nu, 1e-3 * case.scaling.u_lat)
+    live = ut_o > 0
+    say(f"  fine wall cells {nf}, live {int(live.sum())}")
+    assert live.sum() > 0.5 * nf
+    assert np.allclose(ut[live], ut_o[live], rtol=2e-4, atol=1e-9)
+    for a, b in ((ux, ox), (uy, oy), (uz, oz)):
+        assert np.allclose(half(a), half(b), rtol=2e-3, atol=2e-6)
+    r = s.wall_model_readout()
+    assert r.from_fine == 1 and r.listed == nf and r.cells == int(ost[2])
+    # one coupled step with the current slip fields of both levels
+    cux, cuy, cuz, _ = s.read_wall_slip_level(0)
+    cf = s.download_f(f64=True)
+    ff = s.download_fine_f(f64=True)
+    s.step_n(1)
+    s.synchronize()
+    got_c = s.download_f(f64=True).reshape(Q, -1)
+    got_f = s.download_fine_f(f64=True).reshape(Q, -1)
+    taus = np.full(1, case.scaling.tau)
+    uins = np.full(1, case.scaling.u_lat)
+    orc.run_refined(cf, ff, case.flags, fine, case.dims, box, 2, taus, uins, cs=case.cs,
+                    coarse_slip=(cux, cuy, cuz), fine_slip=(ux, uy, uz))
+    cfl, ffl = case.flags == fc.FLUID, fine == fc.FLUID
+    ec = np.linalg.norm(got_c[:, cfl] - cf.reshape(Q, -1)[:, cfl]) / np.linalg.norm(cf.reshape(Q, -1)[:, cfl])
+    ef = np.linalg.norm(got_f[:, ffl] - ff.reshape(Q, -1)[:, ffl]) / np.linalg.norm(ff.reshape(Q, -1)[:, ffl])
+    say(f"  coupled step with slip: coarse {ec:.2e}, fine {ef:.2e}")
+    assert ec < 1e-6 and ef < 1e-6
+    # the fine force carries the fine slip (forcesFromFine)
+    if s.refinement_info()["forces_from_fine"]:
+        raw, _ = s.force_raw()
+        assert np.all(np.isfinite(raw))
+    s.close()

@@ -221,3 +221,52 @@ def test_uniform_freestream_stays_uniform_through_three_levels():
     for arr, fl in ((ff, fflags), (gf, gflags)):
         g = arr.reshape(Q, -1)[:, fl == fc.FLUID]
         assert g.shape[1] > 0 and np.abs(g - feq[:, None]).max() < 1e-13
+
+
+def test_refined_run_slip_wiring():
+    """The slip-aware refined runner: zero slip fields equal the plain run bit
+    for bit; a coarse slip field changes the coarse step outside the patch
+    exactly as the single-level step with that field does."""
+    import numpy as np
+    from oracle import oracle as O
+    dims, box, m = (48, 32, 4), (14, 10, 30, 22), 2
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    import foil_cfd_amd as fc
+    flags = fc.build_boundary_flags(*dims).reshape(nz, ny, nx)
+    flags[:, 4:6, 4:8] = 1  # a small solid block outside the patch
+    flags = flags.reshape(-1)
+    fd = O.fine_dims(box, dims, m)
+    fine = np.zeros(fd[0] * fd[1] * fd[2], np.uint8)
+    fc.stamp_interface_shell(*fd, fine)
+    rng = np.random.default_rng(0)
+    f0 = O.init_equilibrium(flags, dims, 0.05, f64=True)
+    f0 += 1e-4 * rng.standard_normal(f0.size)
+    g0 = O.init_equilibrium(fine, fd, 0.05, f64=True)
+    taus, uins = np.full(2, 0.56), np.full(2, 0.05)
+    z3 = [np.zeros(n, np.uint16)] * 3
+    zf = [np.zeros(fd[0] * fd[1] * fd[2], np.uint16)] * 3
+    a_c, a_f = f0.copy(), g0.copy()
+    O.run_refined(a_c, a_f, flags, fine, dims, box, m, taus, uins)
+    b_c, b_f = f0.copy(), g0.copy()
+    O.run_refined(b_c, b_f, flags, fine, dims, box, m, taus, uins, coarse_slip=z3, fine_slip=zf)
+    assert np.array_equal(a_c, b_c) and np.array_equal(a_f, b_f)
+    # coarse slip on the cells around the block
+    sl = np.zeros(n, np.float16)
+    sl.reshape(nz, ny, nx)[:, 3:7, 3:9] = np.float16(0.01)
+    slip = [sl.view(np.uint16), np.zeros(n, np.uint16), np.zeros(n, np.uint16)]
+    c_c, c_f = f0.copy(), g0.copy()
+    O.run_refined(c_c, c_f, flags, fine, dims, box, m, taus[:1], uins[:1], coarse_slip=slip)
+    d = f0.copy()
+    O.run(d, flags, dims, taus[:1].astype(np.float64), uins[:1].astype(np.float64), slip=tuple(slip), cs=0.1733)
+    x, y = np.meshgrid(np.arange(nx), np.arange(ny))
+    outside = ((x < box[0]) | (x >= box[2]) | (y < box[1]) | (y >= box[3])).reshape(-1)
+    outside = np.tile(outside, nz)
+    fl = flags == 0
+    sel = outside & fl
+    cc = c_c.reshape(19, -1)[:, sel]
+    dd = d.reshape(19, -1)[:, sel]
+    assert np.array_equal(cc, dd)
+    e_c, _ = f0.copy(), g0.copy()
+    O.run_refined(e_c, g0.copy(), flags, fine, dims, box, m, taus[:1], uins[:1])
+    assert not np.array_equal(e_c.reshape(19, -1)[:, sel], cc)
