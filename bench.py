@@ -1,13 +1,20 @@
 """FoilCFD MI355X benchmark: GLUPS and HBM roofline fraction of the fused
 D3Q19 TRT + Smagorinsky stream-collide on the 23.6 M-cell NACA 2412 wing
 (512 x 192 x 240, chord 256, AoA 8 deg), 16-bit shifted storage (FP16C,
-s1e4m11) vs FP32.
+s1e4m11) vs FP32, in the full default-tunnel configuration: a factor-2
+refinement patch hugging the airfoil (the reference app's patch margins
+0.20 / 0.50 / 0.10 / 0.20 chords), the iMEM wall model on the wall cells of
+both levels, and 1 M tracer particles advected once per call.
 
-One "step" is one lattice time step of the whole grid (the reference's
-LBMSolver::stepN unit). Steps run in calls of 100 (one macro store and one
-momentum-exchange sample per call, like the reference's selftest frame).
+One "step" is one coarse time step with its two fine sub-steps (the
+reference's LBMSolver::stepN unit); GLUPS counts coarse + 2 x fine cell
+updates per step, the reference's perfStats rule. Steps run in calls of 100
+(one macro store, one momentum-exchange sample, one wall-model update per
+level and one particle advection per call, like the reference's frame).
 
 Prints ONE JSON line. value = FP16C-storage GLUPS (whole job, one GPU).
+--coarse-only measures the bare coarse wing instead (no patch, wall model or
+particles).
 """
 import argparse
 import json
@@ -78,6 +85,93 @@ def run_storage(case, storage, steps, warmup, per_call, kernel_steps, variant="a
     return res
 
 
+PATCH_MARGINS_C = (0.20, 0.50, 0.10, 0.20)  # upstream, wake, below, above (src/app/ui.h:159-162)
+PARTICLES = 1_000_000
+
+
+def full_setup(case, storage, aoa, wall="newton"):
+    """The default-tunnel configuration on top of the coarse wing: factor-2
+    patch derived from the solid bbox with the app's margins, clean surface
+    references, wall model, particles."""
+    from foil_cfd_amd import cases
+    t0 = time.perf_counter()
+    up, wake, below, above = PATCH_MARGINS_C
+    box, fine = cases.refinement_patch(case, 2, upstream_c=up, wake_c=wake, below_c=below, above_c=above,
+                                       aoa=aoa)
+    s = cases.make_solver(case, storage)
+    s.set_surface_reference(case.flags)
+    s.init_refinement(box, 2, fine)
+    s.set_refined_surface_reference(fine)
+    s.set_wall_model(wall)
+    s.particles_seed(PARTICLES, 0xF01DCAFE)
+    info = s.refinement_info()
+    log(f"[bench] {storage}: patch {box} fine grid {info['fine_dims']} "
+        f"({info['device_bytes'] / 1e9:.2f} GB), wall cells coarse {s.wall_cell_count_level(0)} "
+        f"fine {s.wall_cell_count_level(1)}, setup {time.perf_counter() - t0:.1f} s")
+    return s, box, info
+
+
+def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant="auto"):
+    n_c = case.dims[0] * case.dims[1] * case.dims[2]
+    s, box, info = full_setup(case, storage, aoa)
+    s.set_step_variant(variant)
+    fd = info["fine_dims"]
+    n_f = fd[0] * fd[1] * fd[2]
+    updates = n_c + 2 * n_f
+    seed = [1]
+
+    def run(k_total):
+        done = 0
+        while done < k_total:
+            k = min(per_call, k_total - done)
+            s.step_n(k)
+            s.particles_advect(float(k), seed[0])
+            seed[0] += 1
+            done += k
+
+    run(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    s.set_kernel_timing(True)
+    run(kernel_steps)
+    lv = [s.kernel_timing_level(i) for i in (0, 1)]
+    s.set_kernel_timing(False)
+    nan = s.nan_detected()
+    fr = s.forces()
+    wr = s.wall_model_readout()
+    s.close()
+    bpc = BYTES_PER_CELL[storage]
+    per_level = {}
+    for name, (launches, ms), cells in (("coarse", lv[0], n_c), ("fine", lv[1], n_f)):
+        avg = ms / launches if launches else float("nan")
+        per_level[name] = {"cells": cells, "launches": launches, "kernel_ms": avg,
+                           "kernel_gbs": bpc * cells / (avg * 1e-3) / 1e9}
+    tot_ms = lv[0][1] + lv[1][1]
+    res = {
+        "glups": updates * steps / wall / 1e9,
+        "ms_per_step": wall * 1e3 / steps,
+        "cell_updates_per_step": updates,
+        "kernel_ms": per_level["fine"]["kernel_ms"],
+        "kernel_gbs": per_level["fine"]["kernel_gbs"],
+        "all_step_kernels_gbs": bpc * (n_c * lv[0][0] + n_f * lv[1][0]) / (tot_ms * 1e-3) / 1e9,
+        "step_kernel_share_of_wall": tot_ms / (kernel_steps / steps * wall * 1e3) if wall > 0 else None,
+        "per_level": per_level,
+        "nan": bool(nan),
+        "cl_ema": fr.cl,
+        "cd_ema": fr.cd,
+        "wall_cells_active_fine": wr.cells,
+        "patch_box": list(box),
+        "fine_dims": list(fd),
+    }
+    log(f"[bench] {storage} full: {res['glups']:.2f} GLUPS wall ({res['ms_per_step']:.2f} ms/step), fine step "
+        f"kernel {res['kernel_ms']:.3f} ms ({res['kernel_gbs']:.0f} GB/s), coarse step kernel "
+        f"{per_level['coarse']['kernel_ms']:.3f} ms ({per_level['coarse']['kernel_gbs']:.0f} GB/s)")
+    return res
+
+
 def cpu_baseline(case, seconds=12.0):
     """The oracle's FP32 stream-collide (reference semantics, pull ping-pong)
     on the host cores, on the same grid, for a bounded number of steps."""
@@ -107,13 +201,15 @@ def cpu_baseline(case, seconds=12.0):
                       f"({dt:.1f} s wall)", "seconds": dt}
 
 
-def measured_traffic(storage):
-    """HBM bytes per step-kernel launch measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE on this workload (copy-kernel calibrated), from the committed
-    profile summary; None when absent."""
+def measured_traffic(storage, level="coarse"):
+    """HBM bytes per step-kernel launch of a level measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE on this workload (copy-kernel calibrated), from the
+    committed profile summary; None when absent."""
     path = os.path.join(ROOT, "profiles", "stream_collide_wing_traffic.json")
     key = {"f16c": "StoreF16C", "f16": "StoreF16", "bf16": "StoreBF16", "f32": "StoreF32",
            "f64": "StoreF64"}[storage]
+    if level != "coarse":
+        key = f"{key}:{level}"
     try:
         with open(path) as fh:
             rec = json.load(fh).get(key)
@@ -243,8 +339,10 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16c,f32")
-    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe"],
-                    help="step kernel: auto (library default), single or pair")
+    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe", "stage"],
+                    help="step kernel: auto (library default), single, pair, pipe or stage")
+    ap.add_argument("--coarse-only", action="store_true",
+                    help="bare coarse wing (no refinement patch, wall model or particles)")
     ap.add_argument("--polar", action="store_true",
                     help="batched 16-case polar sweep instead of the wing (separate JSON line)")
     ap.add_argument("--polar-storage", default="f16c")
@@ -277,13 +375,41 @@ def main():
     storages = [s.strip() for s in args.storages.split(",") if s.strip()]
     results = {}
     for st in storages:
-        results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call,
-                                  args.kernel_steps, args.variant)
+        if args.coarse_only:
+            results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call, args.kernel_steps,
+                                      args.variant)
+        else:
+            results[st] = run_full(case, st, args.steps, args.warmup, args.per_call, args.kernel_steps, args.aoa,
+                                   args.variant)
     head = next((h for h in ("f16c", "f16") if h in results), storages[0])
     r = results[head]
     cpu = None if args.no_cpu else cpu_baseline(case)
 
     ncells = case.dims[0] * case.dims[1] * case.dims[2]
+    store = {"f16c": "fp16c_s1e4m11_shifted", "f16": "fp16_shifted", "bf16": "bf16_shifted", "f32": "fp32",
+             "f64": "fp64"}[head]
+    if args.coarse_only:
+        workload = f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_{head}store_coarse_only"
+        config = {"workload": workload, "cells": ncells, "storage": store,
+                  "not_included": "refinement patch, wall model, particles (coarse grid only)"}
+        kernel = "stepKernel (fused stream-collide), coarse wing"
+        traffic = measured_traffic(head)
+        bytes_per_launch = BYTES_PER_CELL[head] * ncells
+    else:
+        fd = r["fine_dims"]
+        workload = (f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_imem_patch2x_"
+                    f"particles1M_{head}store")
+        config = {"workload": workload, "cells": ncells, "fine_cells": fd[0] * fd[1] * fd[2],
+                  "fine_grid": fd, "patch_box_coarse": r["patch_box"], "refine_factor": 2,
+                  "patch_margins_chords": dict(zip(("upstream", "wake", "below", "above"), PATCH_MARGINS_C)),
+                  "cell_updates_per_step": r["cell_updates_per_step"], "storage": store,
+                  "wall_model": "iMEM Reichardt (Newton), coarse + fine wall cells",
+                  "particles": PARTICLES}
+        kernel = "stepKernel (fused stream-collide), fine-level launch (dominant)"
+        traffic = measured_traffic(head, "fine")
+        bytes_per_launch = BYTES_PER_CELL[head] * fd[0] * fd[1] * fd[2]
+    config.update({"collision": "TRT(3/16) + Smagorinsky(0.1733)", "streaming": "in-place AA",
+                   "step_variant": args.variant, "steps_per_call": args.per_call, "parallelism": "single_gpu"})
     out = {
         "metric": METRIC,
         "value": round(r["glups"], 4),
@@ -297,28 +423,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_{head}store",
-            "cells": ncells,
-            "storage": {"f16c": "fp16c_s1e4m11_shifted", "f16": "fp16_shifted",
-                        "bf16": "bf16_shifted", "f32": "fp32", "f64": "fp64"}[head],
-            "collision": "TRT(3/16) + Smagorinsky(0.1733)",
-            "streaming": "in-place AA",
-            "step_variant": args.variant,
-            "steps_per_call": args.per_call,
-            "not_included": "refinement patch, wall model, particles (coarse grid only)",
-            "parallelism": "single_gpu",
-        },
+        "config": config,
         "roofline": {
             "bound": "hbm",
             "achieved": round(r["kernel_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(r["kernel_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(head),
+            "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
-            "kernel": "stepKernel (fused stream-collide)",
+            "kernel": kernel,
             "kernel_ms": round(r["kernel_ms"], 5),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
             "algorithmic_bytes_per_cell": BYTES_PER_CELL[head],
         },
         "cpu_baseline": cpu,

@@ -333,6 +333,7 @@ SIGNATURES = {
                                               C.POINTER(FcVGAuditReadout)]),
     "fc_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
+    "fc_kernel_timing_level": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
 }
 

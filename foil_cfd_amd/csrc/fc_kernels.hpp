@@ -259,10 +259,14 @@ int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, co
                    const LatticeView& fine, int fLayout, const RefineGeom& g, float timeWeight, float neqScale,
                    int fullVolume, void* stream);
 // Fine-to-coarse restriction into the coarse array in layout cLayout;
-// optional coarse macro fields (all four or none).
+// optional coarse macro fields (all four or none). With `scratch`
+// (restrictScratchFloats(g) floats of device memory) the child averages run
+// as a separate coalesced pass first; with nullptr the restriction computes
+// them itself per tile halo.
+long long restrictScratchFloats(const RefineGeom& g);
 int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
                        const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
-                       void* stream);
+                       float* scratch, void* stream);
 
 int launchQLibbPrepass(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* packed,
                        int count, double* scratch, void* stream);

@@ -304,11 +304,83 @@ constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;  // 648 cells, 19 floats each = 49 KB
 
+// Child-average pass of the staged restriction: one wavefront per row
+// segment of 64 / m coarse cells of the restricted box, m lanes per coarse
+// cell, lane l on fine column m (xc - x0) + l % m, so every population load
+// of the wave is one contiguous run of fine x. Each lane sums its m x m
+// children in (y, z), the m lanes of a cell are combined by cross-lane reads,
+// and the cell's first lane stores fneq = h_avg - heq(h_avg) (19 floats),
+// the moments (4) and a valid mark (1) into the scratch, SoA over the box:
+// scratch[k * nR + (xc - rx0) + rw ((yc - ry0) + rh zc)], k = 0..23.
 template <class S>
+__global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
+                                                         RArgs r, float* scratch) {
+    const int rx0 = r.x0 + kRestrictBand, ry0 = r.y0 + kRestrictBand;
+    const int rw = r.x1 - kRestrictBand - rx0, rh = r.y1 - kRestrictBand - ry0;
+    const int m = r.m;
+    const int cpw = kWave / m;  // coarse cells per wave
+    const int nseg = (rw + cpw - 1) / cpw;
+    const long long wave = static_cast<long long>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wave >= static_cast<long long>(nseg) * rh * r.cnz) return;  // wave-uniform
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int seg = static_cast<int>(wave % nseg);
+    const long long row = wave / nseg;
+    const int yc = ry0 + static_cast<int>(row % rh);
+    const int zc = static_cast<int>(row / rh);
+    const int k = lane / m, lx = lane - k * m;
+    const int xc = rx0 + seg * cpw + k;
+    const bool live = k < cpw && xc < rx0 + rw;
+    float h[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) h[q] = 0.0f;
+    int nf = 0;
+    if (live) {
+        const int fx = m * (xc - r.x0) + lx;
+        for (int kk = 0; kk < m; ++kk)
+            for (int j = 0; j < m; ++j) {
+                const int fy = m * (yc - r.y0) + j, fz = m * zc + kk;
+                if ((fkind[cellOf(fx, fy, fz, r.fnx, r.fny)] & kKindFlagMask) == kSolid) continue;
+                ++nf;
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) h[q] += readH<S>(ff, r.fn, fLayout, q, fx, fy, fz, r.fnx, r.fny, r.fnz);
+            }
+    }
+    // the m lanes of a coarse cell: the first gathers the others' sums
+    for (int i = 1; i < m; ++i) {
+        const int src = min(lane + i, kWave - 1);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const float o = __shfl(h[q], src, kWave);
+            if (lx == 0) h[q] += o;
+        }
+        const int on = __shfl(nf, src, kWave);
+        if (lx == 0) nf += on;
+    }
+    if (!live || lx != 0) return;
+    const long long nR = static_cast<long long>(rw) * rh * r.cnz;
+    const long long ri = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
+    if (nf == 0) {
+        scratch[23 * nR + ri] = 0.0f;
+        return;
+    }
+    const float invN = 1.0f / static_cast<float>(nf);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) h[q] *= invN;
+    const Mom mo = momentsOf(h);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) scratch[q * nR + ri] = h[q] - heq(q, mo);
+    scratch[19 * nR + ri] = mo.rho;
+    scratch[20 * nR + ri] = mo.ux;
+    scratch[21 * nR + ri] = mo.uy;
+    scratch[22 * nR + ri] = mo.uz;
+    scratch[23 * nR + ri] = 1.0f;
+}
+
+template <class S, bool STAGED>
 __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
                                                         void* cf, const std::uint8_t* ckind, int cLayout, RArgs r,
                                                         float neqScaleInv, float* mRho, float* mU, float* mV,
-                                                        float* mW) {
+                                                        float* mW, const float* scratch) {
     __shared__ float sFneq[kQ][kHalo];
     __shared__ unsigned char sValid[kHalo];
     __shared__ float sMom[4][kTX * kTY * kTZ];
@@ -319,14 +391,30 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
     const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
     const int ox = rx0 + bx * kTX, oy = ry0 + by * kTY, oz = bz * kTZ;
 
-    // Phase 1: child averages of the halo tile.
+    // Phase 1: child averages of the halo tile (from the scratch of the
+    // child-average pass, or computed here from the fine array).
     for (int i = threadIdx.x; i < kHalo; i += kBlock) {
         const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
         const int xc = ox + hx - 1, yc = oy + hy - 1;
         int zc = oz + hz - 1;
         zc = (zc % r.cnz + r.cnz) % r.cnz;
         unsigned char ok = 0;
-        if (xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
+        if (STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
+            const int rw = rx1 - rx0, rh = ry1 - ry0;
+            const long long nR = static_cast<long long>(rw) * rh * r.cnz;
+            const long long ri =
+                (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
+            if (scratch[23 * nR + ri] != 0.0f) {
+                ok = 1;
+#pragma unroll
+                for (int q = 0; q < kQ; ++q) sFneq[q][i] = scratch[q * nR + ri];
+                if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY && hz >= 1 && hz <= kTZ) {
+                    const int o = (hx - 1) + kTX * ((hy - 1) + kTY * (hz - 1));
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) sMom[c][o] = scratch[(19 + c) * nR + ri];
+                }
+            }
+        } else if (!STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
             float h[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) h[q] = 0.0f;
@@ -434,14 +522,25 @@ struct RefineOps {
         return static_cast<int>(hipGetLastError());
     }
     static int restrictTo(const LatticeView& f, int fLayout, const LatticeView& c, int cLayout, const RArgs& r,
-                          float neqScaleInv, float* rho, float* u, float* v, float* w, hipStream_t st) {
+                          float neqScaleInv, float* rho, float* u, float* v, float* w, float* scratch,
+                          hipStream_t st) {
         const int rw = r.x1 - r.x0 - 2 * kRestrictBand, rh = r.y1 - r.y0 - 2 * kRestrictBand;
         if (rw <= 0 || rh <= 0) return static_cast<int>(hipErrorInvalidValue);
         const unsigned blocks = static_cast<unsigned>(((rw + kTX - 1) / kTX) * ((rh + kTY - 1) / kTY) *
                                                       ((r.cnz + kTZ - 1) / kTZ));
         const bool macro = rho && u && v && w;
-        restrictKernel<S><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r, neqScaleInv,
-                                                     macro ? rho : nullptr, u, v, w);
+        if (scratch) {
+            const int cpw = kWave / r.m;
+            const long long waves = static_cast<long long>((rw + cpw - 1) / cpw) * rh * r.cnz;
+            childFneqKernel<S><<<static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0,
+                                 st>>>(f.f, f.kind, fLayout, r, scratch);
+            restrictKernel<S, true><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r,
+                                                               neqScaleInv, macro ? rho : nullptr, u, v, w, scratch);
+        } else {
+            restrictKernel<S, false><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r,
+                                                                neqScaleInv, macro ? rho : nullptr, u, v, w,
+                                                                nullptr);
+        }
         return static_cast<int>(hipGetLastError());
     }
 };
@@ -498,25 +597,30 @@ int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, co
     }
 }
 
+long long restrictScratchFloats(const RefineGeom& g) {
+    const long long rw = g.box.x1 - g.box.x0 - 2 * kRestrictBand, rh = g.box.y1 - g.box.y0 - 2 * kRestrictBand;
+    return rw > 0 && rh > 0 ? 24LL * rw * rh * g.coarse.nz : 0;
+}
+
 int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
                        const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
-                       void* stream) {
+                       float* scratch, void* stream) {
     if (!geomOk(g) || coarse.storage != fine.storage) return static_cast<int>(hipErrorInvalidValue);
     const RArgs r = makeRArgs(g);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     switch (coarse.storage) {
         case kStoreF32:
-            return RefineOps<StoreF32>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+            return RefineOps<StoreF32>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
         case kStoreF16:
-            return RefineOps<StoreF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+            return RefineOps<StoreF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
         case kStoreBF16:
             return RefineOps<StoreBF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
-                                                    st);
+                                                    scratch, st);
         case kStoreF64:
-            return RefineOps<StoreF64>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, st);
+            return RefineOps<StoreF64>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
         case kStoreF16C:
             return RefineOps<StoreF16C>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
-                                                    st);
+                                                    scratch, st);
         default: return static_cast<int>(hipErrorInvalidValue);
     }
 }

@@ -357,11 +357,6 @@ public:
             a.uInlet = effectiveU();
             a.writeMacro = (i == n - 1) ? 1 : 0;
             a.variant = stepVariant_;
-            std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-            if (kernelTiming_) {
-                ev = nextKernelEvents();
-                if (ev) (void)hipEventRecord(ev->first, stream_);
-            }
             if (qCount_ > 0 && qlibbEnabled_ &&
                 launchQLibbPrepass(lat, layout_, dQCells_, dQPacked_, qCount_, dQScratch_, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "q-LIBB pre-pass failed");
@@ -370,8 +365,9 @@ public:
             if (fine_.active &&
                 launchBandSnapshot(lat, layout_, fineGeom(), fine_.dBandXY, fine_.nband, fine_.dBand, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "refinement band snapshot failed");
+            std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(0);
             const int e = launchStep(lat, layout_, a, stream_);
-            if (ev) (void)hipEventRecord(ev->second, stream_);
+            timedEnd(ev);
             if (e != 0) return fail(FC_ERR_DEVICE, std::string("step launch failed: ") +
                                                        hipGetErrorString(static_cast<hipError_t>(e)));
             if (fine_.active) {
@@ -1317,6 +1313,20 @@ public:
         kEventsUsed_ = 0;
         kTimedMs_ = 0.0;
         kTimedLaunches_ = 0;
+        for (int i = 0; i < 3; ++i) {
+            kLevelMs_[i] = 0.0;
+            kLevelLaunches_[i] = 0;
+        }
+        return FC_OK;
+    }
+
+    int kernelTimingLevel(int level, int* launches, double* totalMs) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (level < 0 || level > 2) return fail(FC_ERR_INVALID, "level must be 0, 1 or 2");
+        if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "stream error");
+        harvestKernelEvents();
+        if (launches) *launches = static_cast<int>(kLevelLaunches_[level]);
+        if (totalMs) *totalMs = kLevelMs_[level];
         return FC_OK;
     }
 
@@ -1729,6 +1739,7 @@ public:
         int* dBandXY = nullptr;
         int* dBandIndex = nullptr;
         float* dBand = nullptr;
+        float* dRestrict = nullptr;  // child-average scratch of the restriction into the parent
         int nband = 0;
         long long bytes = 0;
         WMLevel wm;                            // the level's wall-cell list and slip field
@@ -1770,6 +1781,7 @@ public:
         hipRelease(l.dBandXY);
         hipRelease(l.dBandIndex);
         hipRelease(l.dBand);
+        hipRelease(l.dRestrict);
         l.wm.free();
         l = FineLevel{};
     }
@@ -1853,7 +1865,10 @@ public:
             if (finer_.active && launchBandSnapshot(fv, fine_.layout, geomOf(fine_.dims, finer_), finer_.dBandXY,
                                                     finer_.nband, finer_.dBand, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer band snapshot failed");
-            if (launchStep(fv, fine_.layout, fa, stream_) != 0) return fail(FC_ERR_DEVICE, "fine step failed");
+            std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(1);
+            const int fe = launchStep(fv, fine_.layout, fa, stream_);
+            timedEnd(ev);
+            if (fe != 0) return fail(FC_ERR_DEVICE, "fine step failed");
             if (finer_.active) {
                 if (int e = finerSubSteps(fa, 1 - fine_.layout); e != FC_OK) return e;
             }
@@ -1862,7 +1877,8 @@ public:
         const float neqScaleInv = (static_cast<float>(m) * tauC) / tauF;
         const bool macro = coarseArgs.writeMacro != 0;
         if (launchFineRestrict(fv, fine_.layout, cv, cLayout, g, neqScaleInv, macro ? rho_ : nullptr,
-                               macro ? u_ : nullptr, macro ? v_ : nullptr, macro ? w_ : nullptr, stream_) != 0)
+                               macro ? u_ : nullptr, macro ? v_ : nullptr, macro ? w_ : nullptr, fine_.dRestrict,
+                               stream_) != 0)
             return fail(FC_ERR_DEVICE, "fine restriction failed");
         return FC_OK;
     }
@@ -1917,6 +1933,8 @@ public:
         rc |= hipAlloc(&fl.dBandXY, bandXY.size() * sizeof(int));
         rc |= hipAlloc(&fl.dBandIndex, bandIndex.size() * sizeof(int));
         rc |= hipAlloc(&fl.dBand, bandBytes);
+        const std::size_t rsBytes = static_cast<std::size_t>(restrictScratchFloats(geomOf(parent, fl))) * sizeof(float);
+        rc |= hipAlloc(&fl.dRestrict, rsBytes);
         if (rc != FC_OK)
             return fail(FC_ERR_ALLOC, "refined level allocation failed (" + std::to_string(fBytes) + " bytes)");
         if (hipMemcpyAsync(fl.dBandXY, bandXY.data(), bandXY.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -1924,7 +1942,7 @@ public:
             hipMemcpyAsync(fl.dBandIndex, bandIndex.data(), bandIndex.size() * sizeof(int), hipMemcpyHostToDevice,
                            stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "band table upload failed");
-        fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes);
+        fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes + rsBytes);
         return FC_OK;
     }
 
@@ -1944,12 +1962,15 @@ public:
             if (launchFineFill(fv, fLayout, finer_.dBand, finer_.dBandIndex, finer_.nband, gv, finer_.layout, g,
                                static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer interface fill failed");
-            if (launchStep(gv, finer_.layout, ga, stream_) != 0) return fail(FC_ERR_DEVICE, "finer step failed");
+            std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(2);
+            const int ge = launchStep(gv, finer_.layout, ga, stream_);
+            timedEnd(ev);
+            if (ge != 0) return fail(FC_ERR_DEVICE, "finer step failed");
             finer_.layout = 1 - finer_.layout;
         }
         const float neqScaleInv = (static_cast<float>(m2) * tauF) / tauG;
         if (launchFineRestrict(gv, finer_.layout, fv, fLayout, g, neqScaleInv, nullptr, nullptr, nullptr, nullptr,
-                               stream_) != 0)
+                               finer_.dRestrict, stream_) != 0)
             return fail(FC_ERR_DEVICE, "finer restriction failed");
         return FC_OK;
     }
@@ -2156,7 +2177,7 @@ private:
         }
     }
 
-    std::pair<hipEvent_t, hipEvent_t>* nextKernelEvents() {
+    std::pair<hipEvent_t, hipEvent_t>* nextKernelEvents(int level) {
         if (kEventsUsed_ >= 4096) {
             (void)hipStreamSynchronize(stream_);
             harvestKernelEvents();
@@ -2166,16 +2187,33 @@ private:
             if (hipEventCreate(&pr.first) != hipSuccess || hipEventCreate(&pr.second) != hipSuccess)
                 return nullptr;
             kEvents_.push_back(pr);
+            kEventLevel_.push_back(0);
         }
+        kEventLevel_[kEventsUsed_] = level;
         return &kEvents_[kEventsUsed_++];
+    }
+    // HIP events around one step launch of a level while timing is on.
+    std::pair<hipEvent_t, hipEvent_t>* timedBegin(int level) {
+        if (!kernelTiming_) return nullptr;
+        std::pair<hipEvent_t, hipEvent_t>* ev = nextKernelEvents(level);
+        if (ev) (void)hipEventRecord(ev->first, stream_);
+        return ev;
+    }
+    void timedEnd(std::pair<hipEvent_t, hipEvent_t>* ev) {
+        if (ev) (void)hipEventRecord(ev->second, stream_);
     }
 
     void harvestKernelEvents() {
         for (std::size_t i = 0; i < kEventsUsed_; ++i) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, kEvents_[i].first, kEvents_[i].second) == hipSuccess) {
-                kTimedMs_ += ms;
-                ++kTimedLaunches_;
+                const int lv = kEventLevel_[i];
+                kLevelMs_[lv] += ms;
+                ++kLevelLaunches_[lv];
+                if (lv == 0) {
+                    kTimedMs_ += ms;
+                    ++kTimedLaunches_;
+                }
             }
         }
         kEventsUsed_ = 0;
@@ -2262,6 +2300,9 @@ private:
 
     bool kernelTiming_ = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> kEvents_;
+    std::vector<int> kEventLevel_;  // level of each event pair (0 coarse, 1 fine, 2 finer)
+    double kLevelMs_[3] = {0, 0, 0};
+    long long kLevelLaunches_[3] = {0, 0, 0};
     std::size_t kEventsUsed_ = 0;
     double kTimedMs_ = 0.0;
     long long kTimedLaunches_ = 0;
@@ -2739,6 +2780,10 @@ long long fc_device_bytes(fc_solver* s) { return s ? s->impl.deviceBytes() : 0; 
 
 int fc_set_kernel_timing(fc_solver* s, int enabled) {
     return s ? s->impl.setKernelTiming(enabled != 0) : FC_ERR_INVALID;
+}
+
+int fc_kernel_timing_level(fc_solver* s, int level, int* launches, double* total_ms) {
+    return s ? s->impl.kernelTimingLevel(level, launches, total_ms) : FC_ERR_INVALID;
 }
 
 int fc_kernel_timing(fc_solver* s, int* launches, double* total_ms) {

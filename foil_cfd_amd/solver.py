@@ -661,6 +661,14 @@ class LBMSolver:
         check(self._lib.fc_kernel_timing(self._h, C.byref(n), C.byref(ms)), "fc_kernel_timing")
         return n.value, ms.value
 
+    def kernel_timing_level(self, level: int):
+        """(step launches, summed ms) of level 0 coarse / 1 fine / 2 finer."""
+        n = C.c_int(0)
+        ms = C.c_double(0.0)
+        check(self._lib.fc_kernel_timing_level(self._h, int(level), C.byref(n), C.byref(ms)),
+              "fc_kernel_timing_level")
+        return n.value, ms.value
+
     def time_steps(self, calls: int, steps_per_call: int) -> float:
         ms = C.c_double(0)
         check(self._lib.fc_time_steps(self._h, calls, steps_per_call, C.byref(ms)),

@@ -634,6 +634,9 @@ int fc_audit_vg_vortex_strength(fc_solver* s, const fc_vg_params* vg, int chord_
  * the last enable, after synchronizing. */
 int fc_set_kernel_timing(fc_solver* s, int enabled);
 int fc_kernel_timing(fc_solver* s, int* launches, double* total_ms);
+/* The same per level: step launches of level 0 (coarse), 1 (fine patch) or
+ * 2 (nested finer patch) and their summed milliseconds. */
+int fc_kernel_timing_level(fc_solver* s, int level, int* launches, double* total_ms);
 /* Wall time of n fc_step calls of one step each, bracketed by HIP events on
  * the solver's stream (ms). */
 int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms);
