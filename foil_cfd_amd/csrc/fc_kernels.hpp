@@ -255,9 +255,13 @@ int launchBandSnapshot(const LatticeView& coarse, int layout, const RefineGeom& 
                        float* band, void* stream);
 // Coarse-to-fine fill (shell from band + live coarse, or full volume from the
 // live coarse array alone), written in the fine level's layout fLayout.
+// Shell fills of the m sub-steps of one parent step interpolate the same two
+// parent time levels: with `cache` (fillCacheFloats(g) floats) cacheMode 1
+// stores them, cacheMode 2 reuses them (nullptr / 0: no cache).
+long long fillCacheFloats(const RefineGeom& g);
 int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, const int* bandIndex, int nband,
                    const LatticeView& fine, int fLayout, const RefineGeom& g, float timeWeight, float neqScale,
-                   int fullVolume, void* stream);
+                   int fullVolume, float* cache, int cacheMode, void* stream);
 // Fine-to-coarse restriction into the coarse array in layout cLayout;
 // optional coarse macro fields (all four or none). With `scratch`
 // (restrictScratchFloats(g) floats of device memory) the child averages run

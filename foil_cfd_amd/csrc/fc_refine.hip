@@ -214,13 +214,36 @@ __device__ inline bool shellCell(long long t, const RArgs& r, int& x, int& y, in
 // live coarse array (seeding). Otherwise: the Interface shell, t from the
 // band snapshot and t+1 from the live coarse array.
 // ---------------------------------------------------------------------------
+// Moments, fneq projection and rescale of an interpolated shell cell, written
+// into the fine slots of layout fLayout.
+template <class S>
+__device__ inline void fillFinish(void* ff, int fLayout, const RArgs& r, int xf, int yf, int zf, const float h[kQ],
+                                  float neqScale) {
+    const Mom mo = momentsOf(h);
+    float fneq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) fneq[q] = h[q] - heq(q, mo);
+    regularize(fneq);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+        writeH<S>(ff, r.fn, fLayout, q, xf, yf, zf, r.fnx, r.fny, r.fnz, heq(q, mo) + neqScale * fneq[q]);
+}
+
 template <class S, bool FULL>
 __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::uint8_t* ckind, int cLayout,
                                                     const float* band, const int* bandIndex, int nband,
                                                     void* ff, const std::uint8_t* fkind, int fLayout, RArgs r,
-                                                    float timeWeight, float neqScale) {
+                                                    float timeWeight, float neqScale, float* cache,
+                                                    int cacheMode) {
     const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     int xf, yf, zf;
+    // Shell interpolation cache: the m sub-steps of one parent step blend the
+    // same two interpolated time levels with different weights, so the first
+    // stores them (cacheMode 1) and the others reuse them (cacheMode 2).
+    // cache[(2q) ns + t] = level t, cache[(2q + 1) ns + t] = level t + 1,
+    // cache[38 ns + t] = 1 for a filled shell cell (ns = shell lanes).
+    const long long ns = (4LL * r.fnx + 4LL * (r.fny - 4)) * r.fnz;
+    float h[kQ];
     if (FULL) {
         if (t >= r.fn) return;
         zf = static_cast<int>(t / (static_cast<long long>(r.fnx) * r.fny));
@@ -230,7 +253,20 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
         if ((fkind[t] & kKindFlagMask) == kSolid) return;
     } else {
         if (!shellCell(t, r, xf, yf, zf)) return;
-        if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) return;
+        if (cacheMode == 2) {
+            if (cache[38 * ns + t] == 0.0f) return;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const float s0 = cache[(2 * q) * ns + t], s1 = cache[(2 * q + 1) * ns + t];
+                h[q] = s0 + timeWeight * (s1 - s0);
+            }
+            fillFinish<S>(ff, fLayout, r, xf, yf, zf, h, neqScale);
+            return;
+        }
+        if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) {
+            if (cacheMode == 1) cache[38 * ns + t] = 0.0f;
+            return;
+        }
     }
     const float invM = 1.0f / static_cast<float>(r.m);
     const float gx = static_cast<float>(r.x0) + (static_cast<float>(xf) + 0.5f) * invM - 0.5f;
@@ -255,7 +291,10 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
         if ((ckind[cellOf(cx[c], cy[c], cz[c], r.cnx, r.cny)] & kKindFlagMask) == kSolid) wgt[c] = 0.0f;
         wsum += wgt[c];
     }
-    if (wsum < 1e-6f) return;
+    if (wsum < 1e-6f) {
+        if (!FULL && cacheMode == 1) cache[38 * ns + t] = 0.0f;
+        return;
+    }
     const float invW = 1.0f / wsum;
     // Ring slot of each corner in the band snapshot (every shell corner lies in
     // the ring; a corner outside it would fall back to the live array).
@@ -269,7 +308,6 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
             if (ex >= 0 && ex < r.ew && ey >= 0 && ey < r.eh) slot[c] = bandIndex[ey * r.ew + ex];
         }
     }
-    float h[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         float s0 = 0.0f, s1 = 0.0f;
@@ -282,16 +320,14 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
             s0 += wgt[c] * v0;
             s1 += wgt[c] * v1;
         }
+        if (!FULL && cacheMode == 1) {
+            cache[(2 * q) * ns + t] = s0;
+            cache[(2 * q + 1) * ns + t] = s1;
+        }
         h[q] = s0 + timeWeight * (s1 - s0);
     }
-    const Mom mo = momentsOf(h);
-    float fneq[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) fneq[q] = h[q] - heq(q, mo);
-    regularize(fneq);
-#pragma unroll
-    for (int q = 0; q < kQ; ++q)
-        writeH<S>(ff, r.fn, fLayout, q, xf, yf, zf, r.fnx, r.fny, r.fnz, heq(q, mo) + neqScale * fneq[q]);
+    if (!FULL && cacheMode == 1) cache[38 * ns + t] = 1.0f;
+    fillFinish<S>(ff, fLayout, r, xf, yf, zf, h, neqScale);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,14 +546,15 @@ struct RefineOps {
     }
     static int fill(const LatticeView& c, int cLayout, const float* band, const int* bandIndex, int nband,
                     const LatticeView& f, int fLayout, const RArgs& r, float tw, float neqScale, bool full,
-                    hipStream_t st) {
+                    float* cache, int cacheMode, hipStream_t st) {
         if (full) {
             fillKernel<S, true><<<static_cast<unsigned>((r.fn + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale);
+                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale, nullptr, 0);
         } else {
             const long long n = (4LL * r.fnx + 4LL * (r.fny - 4)) * r.fnz;
             fillKernel<S, false><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale);
+                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale,
+                cache ? cache : nullptr, cache ? cacheMode : 0);
         }
         return static_cast<int>(hipGetLastError());
     }
@@ -569,9 +606,13 @@ int launchBandSnapshot(const LatticeView& coarse, int layout, const RefineGeom& 
     }
 }
 
+long long fillCacheFloats(const RefineGeom& g) {
+    return 39LL * (4LL * g.fine.nx + 4LL * (g.fine.ny - 4)) * g.fine.nz;
+}
+
 int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, const int* bandIndex, int nband,
                    const LatticeView& fine, int fLayout, const RefineGeom& g, float timeWeight, float neqScale,
-                   int fullVolume, void* stream) {
+                   int fullVolume, float* cache, int cacheMode, void* stream) {
     if (!geomOk(g) || coarse.storage != fine.storage) return static_cast<int>(hipErrorInvalidValue);
     if (!fullVolume && (!band || !bandIndex || nband <= 0)) return static_cast<int>(hipErrorInvalidValue);
     const RArgs r = makeRArgs(g);
@@ -580,19 +621,19 @@ int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, co
     switch (coarse.storage) {
         case kStoreF32:
             return RefineOps<StoreF32>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
-                                             neqScale, full, st);
+                                             neqScale, full, cache, cacheMode, st);
         case kStoreF16:
             return RefineOps<StoreF16>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
-                                             neqScale, full, st);
+                                             neqScale, full, cache, cacheMode, st);
         case kStoreBF16:
             return RefineOps<StoreBF16>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r,
-                                              timeWeight, neqScale, full, st);
+                                              timeWeight, neqScale, full, cache, cacheMode, st);
         case kStoreF64:
             return RefineOps<StoreF64>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r, timeWeight,
-                                             neqScale, full, st);
+                                             neqScale, full, cache, cacheMode, st);
         case kStoreF16C:
             return RefineOps<StoreF16C>::fill(coarse, cLayout, band, bandIndex, nband, fine, fLayout, r,
-                                              timeWeight, neqScale, full, st);
+                                              timeWeight, neqScale, full, cache, cacheMode, st);
         default: return static_cast<int>(hipErrorInvalidValue);
     }
 }

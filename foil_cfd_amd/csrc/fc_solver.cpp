@@ -1740,6 +1740,7 @@ public:
         int* dBandIndex = nullptr;
         float* dBand = nullptr;
         float* dRestrict = nullptr;  // child-average scratch of the restriction into the parent
+        float* dFillCache = nullptr; // shell interpolation of the current parent step
         int nband = 0;
         long long bytes = 0;
         WMLevel wm;                            // the level's wall-cell list and slip field
@@ -1782,6 +1783,7 @@ public:
         hipRelease(l.dBandIndex);
         hipRelease(l.dBand);
         hipRelease(l.dRestrict);
+        hipRelease(l.dFillCache);
         l.wm.free();
         l = FineLevel{};
     }
@@ -1838,7 +1840,7 @@ public:
         const float neqScale = tauF / (static_cast<float>(fine_.factor) * tauC);
         if (launchInitEquilibrium(fineView(), 0.0f, stream_) != 0 ||
             launchFineFill(view(), layout_, nullptr, nullptr, 0, fineView(), kLayoutL, fineGeom(), 0.0f, neqScale,
-                           1, stream_) != 0)
+                           1, nullptr, 0, stream_) != 0)
             return fail(FC_ERR_DEVICE, "fine level seeding failed");
         return FC_OK;
     }
@@ -1858,7 +1860,8 @@ public:
         const LatticeView fv = fineView();
         for (int k = 0; k < m; ++k) {
             if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
-                               static_cast<float>(k) / static_cast<float>(m), neqScale, 0, stream_) != 0)
+                               static_cast<float>(k) / static_cast<float>(m), neqScale, 0, fine_.dFillCache,
+                               k == 0 ? 1 : 2, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "fine interface fill failed");
             // The fine step overwrites fine time t in place: keep the ring the
             // finer shell fill interpolates from first.
@@ -1935,6 +1938,8 @@ public:
         rc |= hipAlloc(&fl.dBand, bandBytes);
         const std::size_t rsBytes = static_cast<std::size_t>(restrictScratchFloats(geomOf(parent, fl))) * sizeof(float);
         rc |= hipAlloc(&fl.dRestrict, rsBytes);
+        const std::size_t fcBytes = static_cast<std::size_t>(fillCacheFloats(geomOf(parent, fl))) * sizeof(float);
+        rc |= hipAlloc(&fl.dFillCache, fcBytes);
         if (rc != FC_OK)
             return fail(FC_ERR_ALLOC, "refined level allocation failed (" + std::to_string(fBytes) + " bytes)");
         if (hipMemcpyAsync(fl.dBandXY, bandXY.data(), bandXY.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -1942,7 +1947,7 @@ public:
             hipMemcpyAsync(fl.dBandIndex, bandIndex.data(), bandIndex.size() * sizeof(int), hipMemcpyHostToDevice,
                            stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "band table upload failed");
-        fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes + rsBytes);
+        fl.bytes = static_cast<long long>(fBytes + fl.ncells + bandBytes + rsBytes + fcBytes);
         return FC_OK;
     }
 
@@ -1960,7 +1965,8 @@ public:
         const LatticeView fv = viewOf(fine_), gv = viewOf(finer_);
         for (int q = 0; q < m2; ++q) {
             if (launchFineFill(fv, fLayout, finer_.dBand, finer_.dBandIndex, finer_.nband, gv, finer_.layout, g,
-                               static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, stream_) != 0)
+                               static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, finer_.dFillCache,
+                               q == 0 ? 1 : 2, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer interface fill failed");
             std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(2);
             const int ge = launchStep(gv, finer_.layout, ga, stream_);
@@ -2045,7 +2051,7 @@ public:
         const float neqScale = tauG / (static_cast<float>(finer_.factor) * tauF);
         if (launchInitEquilibrium(viewOf(finer_), 0.0f, stream_) != 0 ||
             launchFineFill(viewOf(fine_), fine_.layout, nullptr, nullptr, 0, viewOf(finer_), kLayoutL,
-                           geomOf(fine_.dims, finer_), 0.0f, neqScale, 1, stream_) != 0)
+                           geomOf(fine_.dims, finer_), 0.0f, neqScale, 1, nullptr, 0, stream_) != 0)
             return fail(FC_ERR_DEVICE, "finer level seeding failed");
         return FC_OK;
     }
