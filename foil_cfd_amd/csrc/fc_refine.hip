@@ -346,8 +346,9 @@ constexpr int kHalo = kHX * kHY * kHZ;  // 648 cells, 19 floats each = 49 KB
 // of the wave is one contiguous run of fine x. Each lane sums its m x m
 // children in (y, z), the m lanes of a cell are combined by cross-lane reads,
 // and the cell's first lane stores fneq = h_avg - heq(h_avg) (19 floats),
-// the moments (4) and a valid mark (1) into the scratch, SoA over the box:
-// scratch[k * nR + (xc - rx0) + rw ((yc - ry0) + rh zc)], k = 0..23.
+// the moments (4) and a valid mark (1) as one 96-byte record per cell:
+// scratch[24 ri + k], ri = (xc - rx0) + rw ((yc - ry0) + rh zc), k = 0..23,
+// so the restriction's halo rows read contiguous memory.
 template <class S>
 __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
                                                          RArgs r, float* scratch) {
@@ -393,23 +394,27 @@ __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const 
         if (lx == 0) nf += on;
     }
     if (!live || lx != 0) return;
-    const long long nR = static_cast<long long>(rw) * rh * r.cnz;
     const long long ri = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
+    float rec[24];
     if (nf == 0) {
-        scratch[23 * nR + ri] = 0.0f;
-        return;
+#pragma unroll
+        for (int k = 0; k < 24; ++k) rec[k] = 0.0f;
+    } else {
+        const float invN = 1.0f / static_cast<float>(nf);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) h[q] *= invN;
+        const Mom mo = momentsOf(h);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) rec[q] = h[q] - heq(q, mo);
+        rec[19] = mo.rho;
+        rec[20] = mo.ux;
+        rec[21] = mo.uy;
+        rec[22] = mo.uz;
+        rec[23] = 1.0f;
     }
-    const float invN = 1.0f / static_cast<float>(nf);
+    float4* dst = reinterpret_cast<float4*>(scratch + 24 * ri);
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) h[q] *= invN;
-    const Mom mo = momentsOf(h);
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) scratch[q * nR + ri] = h[q] - heq(q, mo);
-    scratch[19 * nR + ri] = mo.rho;
-    scratch[20 * nR + ri] = mo.ux;
-    scratch[21 * nR + ri] = mo.uy;
-    scratch[22 * nR + ri] = mo.uz;
-    scratch[23 * nR + ri] = 1.0f;
+    for (int k = 0; k < 6; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
 template <class S, bool STAGED>
@@ -437,17 +442,26 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
         unsigned char ok = 0;
         if (STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
             const int rw = rx1 - rx0, rh = ry1 - ry0;
-            const long long nR = static_cast<long long>(rw) * rh * r.cnz;
             const long long ri =
                 (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
-            if (scratch[23 * nR + ri] != 0.0f) {
+            const float4* src = reinterpret_cast<const float4*>(scratch + 24 * ri);
+            float rec[24];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const float4 v = src[k];
+                rec[4 * k] = v.x;
+                rec[4 * k + 1] = v.y;
+                rec[4 * k + 2] = v.z;
+                rec[4 * k + 3] = v.w;
+            }
+            if (rec[23] != 0.0f) {
                 ok = 1;
 #pragma unroll
-                for (int q = 0; q < kQ; ++q) sFneq[q][i] = scratch[q * nR + ri];
+                for (int q = 0; q < kQ; ++q) sFneq[q][i] = rec[q];
                 if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY && hz >= 1 && hz <= kTZ) {
                     const int o = (hx - 1) + kTX * ((hy - 1) + kTY * (hz - 1));
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) sMom[c][o] = scratch[(19 + c) * nR + ri];
+                    for (int c = 0; c < 4; ++c) sMom[c][o] = rec[19 + c];
                 }
             }
         } else if (!STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
