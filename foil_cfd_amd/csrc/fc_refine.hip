@@ -331,24 +331,31 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
 }
 
 // ---------------------------------------------------------------------------
-// Fine-to-coarse restriction, LDS-tiled: a workgroup covers TX x TY x TZ
-// coarse cells of the restricted box; phase 1 stores the child-averaged fneq
-// (and, for the tile's own cells, the capped moments) of the tile plus a
-// one-cell halo in LDS; phase 2 filters, projects, rescales and blends.
+// Fine-to-coarse restriction. With a scratch array (the solver's path) a
+// child-average pass writes one record per coarse cell of the restricted box
+// and an LDS-tiled filter pass reads them back; without one, a single LDS-
+// tiled kernel averages the children of its tile plus halo itself. The
+// staged filter tile (TX, TY, TZ) is FC_RESTRICT_TILE.
 // ---------------------------------------------------------------------------
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;  // 648 cells, 19 floats each = 49 KB
+#ifndef FC_RESTRICT_TILE
+#define FC_RESTRICT_TILE 16, 8, 4
+#endif
+constexpr int kStagedTile[3] = {FC_RESTRICT_TILE};
+constexpr int kStagedTX = kStagedTile[0], kStagedTY = kStagedTile[1], kStagedTZ = kStagedTile[2];
 
 // Child-average pass of the staged restriction: one wavefront per row
 // segment of 64 / m coarse cells of the restricted box, m lanes per coarse
 // cell, lane l on fine column m (xc - x0) + l % m, so every population load
 // of the wave is one contiguous run of fine x. Each lane sums its m x m
 // children in (y, z), the m lanes of a cell are combined by cross-lane reads,
-// and the cell's first lane stores fneq = h_avg - heq(h_avg) (19 floats),
-// the moments (4) and a valid mark (1) as one 96-byte record per cell:
-// scratch[24 ri + k], ri = (xc - rx0) + rw ((yc - ry0) + rh zc), k = 0..23,
-// so the restriction's halo rows read contiguous memory.
+// and the cell's first lane stores its record as six float4 planes over the
+// restricted box, plane c at scratch4[c nR + ri], ri = (xc - rx0) + rw ((yc -
+// ry0) + rh zc): planes 0..3 fneq[4c..4c+3] of fneq = h_avg - heq(h_avg),
+// plane 4 fneq[16..18] and the valid mark (1 when the cell has a non-solid
+// child), plane 5 the moments rho, ux, uy, uz.
 template <class S>
 __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
                                                          RArgs r, float* scratch) {
@@ -394,6 +401,7 @@ __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const 
         if (lx == 0) nf += on;
     }
     if (!live || lx != 0) return;
+    const long long nR = static_cast<long long>(rw) * rh * r.cnz;
     const long long ri = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
     float rec[24];
     if (nf == 0) {
@@ -406,22 +414,166 @@ __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const 
         const Mom mo = momentsOf(h);
 #pragma unroll
         for (int q = 0; q < kQ; ++q) rec[q] = h[q] - heq(q, mo);
-        rec[19] = mo.rho;
-        rec[20] = mo.ux;
-        rec[21] = mo.uy;
-        rec[22] = mo.uz;
-        rec[23] = 1.0f;
+        rec[19] = 1.0f;
+        rec[20] = mo.rho;
+        rec[21] = mo.ux;
+        rec[22] = mo.uy;
+        rec[23] = mo.uz;
     }
-    float4* dst = reinterpret_cast<float4*>(scratch + 24 * ri);
+    float4* dst = reinterpret_cast<float4*>(scratch);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    for (int k = 0; k < 6; ++k) dst[k * nR + ri] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
-template <class S, bool STAGED>
-__global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
-                                                        void* cf, const std::uint8_t* ckind, int cLayout, RArgs r,
-                                                        float neqScaleInv, float* mRho, float* mU, float* mV,
-                                                        float* mW, const float* scratch) {
+// Restriction of one coarse cell from its filtered fneq (already divided by
+// the number of valid cells in the 19-point neighbourhood) and the child-
+// averaged moments: Latt-Chopard projection, rescale to the coarse tau, edge
+// blend into the existing populations, optional macro store.
+template <class S>
+__device__ inline void restrictFinish(void* cf, int cLayout, const RArgs& r, int xc, int yc, int zc, long long cc,
+                                      int dEdge, float* fneq, const Mom& mo, float neqScaleInv, float* mRho,
+                                      float* mU, float* mV, float* mW) {
+    const float blend = fminf(1.0f, static_cast<float>(dEdge + 1) / static_cast<float>(kRestrictBlend + 1));
+    regularize(fneq);
+    float br = 0, bjx = 0, bjy = 0, bjz = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float existing = readH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz);
+        const float restricted = heq(q, mo) + neqScaleInv * fneq[q];
+        const float blended = existing + blend * (restricted - existing);
+        writeH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz, blended);
+        br += blended;
+        bjx += kCx[q] * blended;
+        bjy += kCy[q] * blended;
+        bjz += kCz[q] * blended;
+    }
+    if (mRho) {
+        const float rr = fmaxf(1.0f + br, 0.05f);
+        const float inv = 1.0f / rr;
+        mRho[cc] = rr;
+        mU[cc] = bjx * inv;
+        mV[cc] = bjy * inv;
+        mW[cc] = bjz * inv;
+    }
+}
+
+// Staged restriction filter over the planes of the child-average pass. A
+// workgroup covers TX x TY x TZ coarse cells of the restricted box and
+// stages the tile plus a one-cell halo through a double-buffered LDS tile,
+// one plane (four populations) at a time, the valid plane first; while a
+// plane is summed the next one is in flight. Each thread accumulates the
+// 19-point filter of its own cell in the neighbour order of the unstaged
+// kernel, so both give identical bits.
+template <class S, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX * TY * TZ) restrictStagedKernel(void* cf, const std::uint8_t* ckind,
+                                                                    int cLayout, RArgs r, float neqScaleInv,
+                                                                    float* mRho, float* mU, float* mV, float* mW,
+                                                                    const float4* scratch4) {
+    constexpr int kT = TX * TY * TZ;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, kH = HX * HY * HZ;
+    constexpr int kPer = (kH + kT - 1) / kT;
+    __shared__ float4 sRec[2][kH];
+    const int rx0 = r.x0 + kRestrictBand, ry0 = r.y0 + kRestrictBand;
+    const int rx1 = r.x1 - kRestrictBand, ry1 = r.y1 - kRestrictBand;
+    const int rw = rx1 - rx0, rh = ry1 - ry0;
+    const long long nR = static_cast<long long>(rw) * rh * r.cnz;
+    const int ntx = (rw + TX - 1) / TX, nty = (rh + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    const int ox = rx0 + bx * TX, oy = ry0 + by * TY, oz = bz * TZ;
+    const int t = static_cast<int>(threadIdx.x);
+
+    // record index of each halo slot this thread stages (-1: outside the box)
+    long long src[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = t + j * kT;
+        src[j] = -1;
+        if (i < kH) {
+            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+            const int xc = ox + hx - 1, yc = oy + hy - 1;
+            const int zc = ((oz + hz - 1) % r.cnz + r.cnz) % r.cnz;
+            if (xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1)
+                src[j] = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
+        }
+    }
+    float4 pre[kPer];
+    auto fetch = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            pre[j] = src[j] >= 0 ? scratch4[c * nR + src[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (t + j * kT < kH) sRec[buf][t + j * kT] = pre[j];
+    };
+
+    const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+    const int xc = ox + lx, yc = oy + ly, zc = oz + lz;
+    const int hc = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
+    const bool inBox = xc < rx1 && yc < ry1 && zc < r.cnz;
+    const long long cc = inBox ? cellOf(xc, yc, zc, r.cnx, r.cny) : 0;
+    const long long own = inBox ? (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc) : 0;
+    bool active = inBox && (ckind[cc] & kKindFlagMask) == kFluid;
+    const float4 mom = active ? scratch4[5 * nR + own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    fetch(4);
+    stage(0);
+    __syncthreads();
+    unsigned nmask = 0;
+    if (active) {
+#pragma unroll
+        for (int n = 0; n < kQ; ++n)
+            if (sRec[0][hc + kCx[n] + HX * (kCy[n] + HY * kCz[n])].w != 0.0f) nmask |= 1u << n;
+        active = (nmask & 1u) != 0;
+    }
+    float fneq[kQ];
+    constexpr int kOrder[5] = {4, 0, 1, 2, 3};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k < 4) fetch(kOrder[k + 1]);
+        if (active) {
+            const float4* s = sRec[k & 1];
+            float4 a = s[hc];
+#pragma unroll
+            for (int n = 1; n < kQ; ++n) {
+                if (!(nmask & (1u << n))) continue;
+                const float4 v = s[hc + kCx[n] + HX * (kCy[n] + HY * kCz[n])];
+                a.x += v.x;
+                a.y += v.y;
+                a.z += v.z;
+                a.w += v.w;
+            }
+            const int q0 = 4 * kOrder[k];
+            fneq[q0] = a.x;
+            fneq[q0 + 1] = a.y;
+            fneq[q0 + 2] = a.z;
+            if (q0 + 3 < kQ) fneq[q0 + 3] = a.w;
+        }
+        if (k < 4) {
+            stage((k + 1) & 1);
+            __syncthreads();
+        }
+    }
+    if (!active) return;
+    const float invS = 1.0f / static_cast<float>(__popc(nmask));
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) fneq[q] *= invS;
+    const int dEdge = min(min(xc - rx0, rx1 - 1 - xc), min(yc - ry0, ry1 - 1 - yc));
+    restrictFinish<S>(cf, cLayout, r, xc, yc, zc, cc, dEdge, fneq, Mom{mom.x, mom.y, mom.z, mom.w}, neqScaleInv,
+                      mRho, mU, mV, mW);
+}
+
+// Unstaged restriction (no scratch): a workgroup covers kTX x kTY x kTZ
+// coarse cells; phase 1 averages the children of the tile plus a one-cell
+// halo into LDS (fneq, and the moments of the tile's own cells), phase 2
+// filters, projects, rescales and blends one coarse cell per thread.
+template <class S>
+__global__ void __launch_bounds__(kBlock) restrictDirectKernel(const void* ff, const std::uint8_t* fkind,
+                                                              int fLayout, void* cf, const std::uint8_t* ckind,
+                                                              int cLayout, RArgs r, float neqScaleInv, float* mRho,
+                                                              float* mU, float* mV, float* mW) {
     __shared__ float sFneq[kQ][kHalo];
     __shared__ unsigned char sValid[kHalo];
     __shared__ float sMom[4][kTX * kTY * kTZ];
@@ -432,39 +584,13 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
     const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
     const int ox = rx0 + bx * kTX, oy = ry0 + by * kTY, oz = bz * kTZ;
 
-    // Phase 1: child averages of the halo tile (from the scratch of the
-    // child-average pass, or computed here from the fine array).
     for (int i = threadIdx.x; i < kHalo; i += kBlock) {
         const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
         const int xc = ox + hx - 1, yc = oy + hy - 1;
         int zc = oz + hz - 1;
         zc = (zc % r.cnz + r.cnz) % r.cnz;
         unsigned char ok = 0;
-        if (STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
-            const int rw = rx1 - rx0, rh = ry1 - ry0;
-            const long long ri =
-                (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
-            const float4* src = reinterpret_cast<const float4*>(scratch + 24 * ri);
-            float rec[24];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const float4 v = src[k];
-                rec[4 * k] = v.x;
-                rec[4 * k + 1] = v.y;
-                rec[4 * k + 2] = v.z;
-                rec[4 * k + 3] = v.w;
-            }
-            if (rec[23] != 0.0f) {
-                ok = 1;
-#pragma unroll
-                for (int q = 0; q < kQ; ++q) sFneq[q][i] = rec[q];
-                if (hx >= 1 && hx <= kTX && hy >= 1 && hy <= kTY && hz >= 1 && hz <= kTZ) {
-                    const int o = (hx - 1) + kTX * ((hy - 1) + kTY * (hz - 1));
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) sMom[c][o] = rec[19 + c];
-                }
-            }
-        } else if (!STAGED && xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
+        if (xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1) {
             float h[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) h[q] = 0.0f;
@@ -500,7 +626,6 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
     }
     __syncthreads();
 
-    // Phase 2: one coarse cell per thread.
     const int lx = threadIdx.x % kTX, ly = (threadIdx.x / kTX) % kTY, lz = threadIdx.x / (kTX * kTY);
     const int xc = ox + lx, yc = oy + ly, zc = oz + lz;
     if (xc >= rx1 || yc >= ry1 || zc >= r.cnz) return;
@@ -508,8 +633,6 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
     if ((ckind[cc] & kKindFlagMask) != kFluid) return;
     const int hc = (lx + 1) + kHX * ((ly + 1) + kHY * (lz + 1));
     if (!sValid[hc]) return;
-    const int dEdge = min(min(xc - rx0, rx1 - 1 - xc), min(yc - ry0, ry1 - 1 - yc));
-    const float blend = fminf(1.0f, static_cast<float>(dEdge + 1) / static_cast<float>(kRestrictBlend + 1));
     float fneq[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) fneq[q] = sFneq[q][hc];
@@ -524,29 +647,10 @@ __global__ void __launch_bounds__(kBlock) restrictKernel(const void* ff, const s
     const float invS = 1.0f / static_cast<float>(nS);
 #pragma unroll
     for (int q = 0; q < kQ; ++q) fneq[q] *= invS;
-    regularize(fneq);
     const int o = lx + kTX * (ly + kTY * lz);
-    const Mom mo{sMom[0][o], sMom[1][o], sMom[2][o], sMom[3][o]};
-    float br = 0, bjx = 0, bjy = 0, bjz = 0;
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-        const float existing = readH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz);
-        const float restricted = heq(q, mo) + neqScaleInv * fneq[q];
-        const float blended = existing + blend * (restricted - existing);
-        writeH<S>(cf, r.cn, cLayout, q, xc, yc, zc, r.cnx, r.cny, r.cnz, blended);
-        br += blended;
-        bjx += kCx[q] * blended;
-        bjy += kCy[q] * blended;
-        bjz += kCz[q] * blended;
-    }
-    if (mRho) {
-        const float rr = fmaxf(1.0f + br, 0.05f);
-        const float inv = 1.0f / rr;
-        mRho[cc] = rr;
-        mU[cc] = bjx * inv;
-        mV[cc] = bjy * inv;
-        mW[cc] = bjz * inv;
-    }
+    const int dEdge = min(min(xc - rx0, rx1 - 1 - xc), min(yc - ry0, ry1 - 1 - yc));
+    restrictFinish<S>(cf, cLayout, r, xc, yc, zc, cc, dEdge, fneq, Mom{sMom[0][o], sMom[1][o], sMom[2][o], sMom[3][o]},
+                      neqScaleInv, mRho, mU, mV, mW);
 }
 
 template <class S>
@@ -577,20 +681,23 @@ struct RefineOps {
                           hipStream_t st) {
         const int rw = r.x1 - r.x0 - 2 * kRestrictBand, rh = r.y1 - r.y0 - 2 * kRestrictBand;
         if (rw <= 0 || rh <= 0) return static_cast<int>(hipErrorInvalidValue);
-        const unsigned blocks = static_cast<unsigned>(((rw + kTX - 1) / kTX) * ((rh + kTY - 1) / kTY) *
-                                                      ((r.cnz + kTZ - 1) / kTZ));
         const bool macro = rho && u && v && w;
         if (scratch) {
             const int cpw = kWave / r.m;
             const long long waves = static_cast<long long>((rw + cpw - 1) / cpw) * rh * r.cnz;
             childFneqKernel<S><<<static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0,
                                  st>>>(f.f, f.kind, fLayout, r, scratch);
-            restrictKernel<S, true><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r,
-                                                               neqScaleInv, macro ? rho : nullptr, u, v, w, scratch);
+            constexpr int TX = kStagedTX, TY = kStagedTY, TZ = kStagedTZ;
+            const unsigned blocks = static_cast<unsigned>(((rw + TX - 1) / TX) * ((rh + TY - 1) / TY) *
+                                                          ((r.cnz + TZ - 1) / TZ));
+            restrictStagedKernel<S, TX, TY, TZ><<<blocks, TX * TY * TZ, 0, st>>>(
+                c.f, c.kind, cLayout, r, neqScaleInv, macro ? rho : nullptr, u, v, w,
+                reinterpret_cast<const float4*>(scratch));
         } else {
-            restrictKernel<S, false><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r,
-                                                                neqScaleInv, macro ? rho : nullptr, u, v, w,
-                                                                nullptr);
+            const unsigned blocks = static_cast<unsigned>(((rw + kTX - 1) / kTX) * ((rh + kTY - 1) / kTY) *
+                                                          ((r.cnz + kTZ - 1) / kTZ));
+            restrictDirectKernel<S><<<blocks, kBlock, 0, st>>>(f.f, f.kind, fLayout, c.f, c.kind, cLayout, r,
+                                                               neqScaleInv, macro ? rho : nullptr, u, v, w);
         }
         return static_cast<int>(hipGetLastError());
     }
