@@ -268,9 +268,21 @@ int launchFineFill(const LatticeView& coarse, int cLayout, const float* band, co
 // as a separate coalesced pass first; with nullptr the restriction computes
 // them itself per tile halo.
 long long restrictScratchFloats(const RefineGeom& g);
+// recordsReady: the scratch already holds this step's child-average records
+// (written by launchStepRecords), so the child-average pass is skipped.
 int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
                        const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
-                       float* scratch, void* stream);
+                       float* scratch, int recordsReady, void* stream);
+
+// Fine step (as launchStep with the default variant) that also writes the
+// restriction's child-average records into `scratch` (restrictScratchFloats
+// floats), bit-identical to the child-average pass of launchFineRestrict.
+// Factor 2 and a fine level that is not periodic in x / y only
+// (stepRecordsSupported); the caller uses it for the last fine sub-step of a
+// coarse step when nothing changes the fine state before the restriction.
+int stepRecordsSupported(const LatticeView& fine, const RefineGeom& g);
+int launchStepRecords(const LatticeView& fine, int layout, const StepArgs& args, const RefineGeom& g,
+                      float* scratch, void* stream);
 
 int launchQLibbPrepass(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* packed,
                        int count, double* scratch, void* stream);

@@ -22,6 +22,7 @@
 #include <cstring>
 
 #include "fc_kcommon.hpp"
+#include "fc_refine_math.hpp"
 
 // No implicit contraction in this file: fused multiply-adds are written out
 // (fmac), so every variant of the step rounds identically.
@@ -303,6 +304,103 @@ stepKernelBuf(KArgs a) {
     RowPos p;
     if (!rowPos<PX, PY>(a, p)) return;
     stepCell<S, PopBuf, FROM_L, WM, PX, PY>(a, p);
+}
+
+// ---------------------------------------------------------------------------
+// Fine step that also writes the restriction's child-average records (factor
+// 2, the last fine sub-step of a coarse step). The four waves of a workgroup
+// take the same 64-cell segment of the rows (y, y+1) x (z, z+1), so its 256
+// cells are the children of 32 coarse cells. After the ordinary step every
+// wave leaves its cells' stored post-collision deviations in LDS (through the
+// storage codec, as a read of the fine array returns them), and wave 0 forms
+// the records exactly as the child-average pass does from the fine array:
+// children summed in (z, y) order per fine column, the column pair combined
+// by a cross-lane read, then the shared record arithmetic. The records are
+// therefore bit-identical to that pass, which this launch replaces.
+// ---------------------------------------------------------------------------
+struct RecArgs {
+    float4* rec;  // kRecordPlanes planes of nR records
+    long long nR;
+    int cx0, cx1;  // restricted coarse columns, relative to the box
+    int cy0, cy1;  // restricted coarse rows, relative to the box
+    int rw, rh;
+};
+
+// A value as it reads back from the fine array: stored, loaded, unscaled.
+template <class S>
+__device__ inline float storedDeviation(typename S::C v) {
+    typename S::T t;
+    S::store(&t, v);
+    return static_cast<float>(S::load(&t)) / S::K;
+}
+
+template <class S, template <class> class M, bool FROM_L, bool WM>
+__global__ void __launch_bounds__(kBlock) stepRecordKernel(KArgs a, RecArgs rg) {
+    static_assert(kWavesPerBlock == 4, "one wave per child row of a 2 x 2 row group");
+    __shared__ float sH[kWavesPerBlock][kQ][kWave];
+    __shared__ unsigned char sLive[kWavesPerBlock][kWave];
+    using C = typename S::C;
+    const int hb = blockIdx.x;
+    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
+    if (lb >= a.nblocks) return;  // workgroup-uniform
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int seg = lb % a.nseg;
+    const int pr = lb / a.nseg;
+    const int yp = pr % (a.ny >> 1), zp = pr / (a.ny >> 1);
+    const int gw = ((2 * zp + (wv >> 1)) * a.ny + 2 * yp + (wv & 1)) * a.nseg + seg;
+    RowPos p;
+    const bool live = taskPos<false, false>(a, gw, p);
+    C H[kQ];
+    std::uint8_t k = kSolid;
+    if (live) {
+        const M<S> pop(a, p);
+        k = a.kind[nb(p, 0, 0, 0)];
+        if ((k & kKindFlagMask) == kFluid)
+            staticFor<kQ>([&](auto qc) {
+                using L = PullSlot<FROM_L, decltype(qc)::value>;
+                H[decltype(qc)::value] = pop.ld(L::dir, L::ex, L::ey, L::ez);
+            });
+        cellBody<S, M, FROM_L, WM, false, false>(a, p, pop, k, H);
+    }
+    if (yp < rg.cy0 || yp >= rg.cy1) return;  // workgroup-uniform: rows outside the restricted box
+
+    const bool fluid = live && (k & kKindFlagMask) == kFluid;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) sH[wv][q][lane] = fluid ? storedDeviation<S>(H[q]) : 0.0f;
+    sLive[wv][lane] = (live && (k & kKindFlagMask) != kSolid) ? 1 : 0;
+    __syncthreads();
+    if (wv != 0) return;
+
+    const int lx = lane & 1;
+    float h[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) h[q] = 0.0f;
+    int nf = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {  // child (dz, dy) = (w >> 1, w & 1)
+        if (!sLive[w][lane]) continue;
+        ++nf;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) h[q] += sH[w][q][lane];
+    }
+    const int src = min(lane + 1, kWave - 1);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const float o = __shfl(h[q], src, kWave);
+        if (lx == 0) h[q] += o;
+    }
+    const int on = __shfl(nf, src, kWave);
+    if (lx == 0) nf += on;
+    const int cx = (seg * kWave + lane) >> 1;
+    if (lx != 0 || cx < rg.cx0 || cx >= rg.cx1) return;
+    const long long ri =
+        (cx - rg.cx0) + static_cast<long long>(rg.rw) * ((yp - rg.cy0) + static_cast<long long>(rg.rh) * zp);
+    float rec[4 * kRecordPlanes];
+    childRecord(h, nf, rec);
+#pragma unroll
+    for (int c = 0; c < kRecordPlanes; ++c)
+        rg.rec[c * rg.nR + ri] = make_float4(rec[4 * c], rec[4 * c + 1], rec[4 * c + 2], rec[4 * c + 3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1166,6 +1264,37 @@ struct StepOp {
 };
 
 template <class S>
+struct StepRecordOp {
+    static int run(const LatticeView& lat, int layout, const StepArgs& sa, const RecArgs& rg, hipStream_t st) {
+        KArgs a = makeArgs(lat);
+        a.tau0 = sa.tau0;
+        a.lam = sa.magicLambda;
+        a.pre = sa.smagPrefactor;
+        a.uIn = sa.uInlet;
+        a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+        const bool wm = lat.uwx && lat.uwy && lat.uwz;
+        const bool fromL = layout == kLayoutL;
+        const dim3 g(rowGrid(a)), b(kBlock);
+        const unsigned long long bytes =
+            static_cast<unsigned long long>(kQ) * static_cast<unsigned long long>(a.ncells) * sizeof(typename S::T);
+        if constexpr (!std::is_same_v<S, StoreF64>) {
+            if (bytes < (1ull << 31)) {
+                if (fromL && wm) stepRecordKernel<S, PopBuf, true, true><<<g, b, 0, st>>>(a, rg);
+                else if (fromL) stepRecordKernel<S, PopBuf, true, false><<<g, b, 0, st>>>(a, rg);
+                else if (wm) stepRecordKernel<S, PopBuf, false, true><<<g, b, 0, st>>>(a, rg);
+                else stepRecordKernel<S, PopBuf, false, false><<<g, b, 0, st>>>(a, rg);
+                return lastError();
+            }
+        }
+        if (fromL && wm) stepRecordKernel<S, PopPtr, true, true><<<g, b, 0, st>>>(a, rg);
+        else if (fromL) stepRecordKernel<S, PopPtr, true, false><<<g, b, 0, st>>>(a, rg);
+        else if (wm) stepRecordKernel<S, PopPtr, false, true><<<g, b, 0, st>>>(a, rg);
+        else stepRecordKernel<S, PopPtr, false, false><<<g, b, 0, st>>>(a, rg);
+        return lastError();
+    }
+};
+
+template <class S>
 struct InitOp {
     static int run(const LatticeView& lat, float u0, hipStream_t st) {
         KArgs a = makeArgs(lat);
@@ -1306,6 +1435,29 @@ int launchFlagEdits(const LatticeView& lat, int layout, const FlagEdit* edits, i
 
 int launchStep(const LatticeView& lat, int layout, const StepArgs& args, void* stream) {
     return byStorage<StepOp>(lat.storage, lat, layout, args, static_cast<hipStream_t>(stream));
+}
+
+int stepRecordsSupported(const LatticeView& fine, const RefineGeom& g) {
+    const int w = g.box.x1 - g.box.x0, h = g.box.y1 - g.box.y0;
+    return g.factor == 2 && fine.dims.nx == g.fine.nx && fine.dims.ny == g.fine.ny && fine.dims.nz == g.fine.nz &&
+           g.fine.nx == 2 * w && g.fine.ny == 2 * h && g.fine.nz == 2 * g.coarse.nz && !fine.periodicX &&
+           !fine.periodicY && w > 2 * kRestrictBand && h > 2 * kRestrictBand;
+}
+
+int launchStepRecords(const LatticeView& fine, int layout, const StepArgs& args, const RefineGeom& g,
+                      float* scratch, void* stream) {
+    if (!scratch || !stepRecordsSupported(fine, g)) return static_cast<int>(hipErrorInvalidValue);
+    const int w = g.box.x1 - g.box.x0, h = g.box.y1 - g.box.y0;
+    RecArgs rg{};
+    rg.rec = reinterpret_cast<float4*>(scratch);
+    rg.cx0 = kRestrictBand;
+    rg.cx1 = w - kRestrictBand;
+    rg.cy0 = kRestrictBand;
+    rg.cy1 = h - kRestrictBand;
+    rg.rw = rg.cx1 - rg.cx0;
+    rg.rh = rg.cy1 - rg.cy0;
+    rg.nR = static_cast<long long>(rg.rw) * rg.rh * g.coarse.nz;
+    return byStorage<StepRecordOp>(fine.storage, fine, layout, args, rg, static_cast<hipStream_t>(stream));
 }
 
 int forcePartialSlots(int count) { return (count + kBlock - 1) / kBlock + 1; }

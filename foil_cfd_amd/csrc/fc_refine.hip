@@ -27,13 +27,12 @@
 #include <hip/hip_runtime.h>
 
 #include "fc_kcommon.hpp"
+#include "fc_refine_math.hpp"
 
 namespace fc {
 namespace {
 
-constexpr int kRestrictBand = 2;   // coarse cells skipped at the box edge (lbm_refine.cuh:77)
 constexpr int kRestrictBlend = 3;  // blend ramp width (lbm_refine.cuh:85)
-constexpr float kMaxSpeed = 0.30f;
 
 struct RArgs {
     int x0, y0, x1, y1, m;
@@ -128,41 +127,6 @@ __device__ inline void regularize(float fneq[kQ]) {
                           2.0f * (cx * cy * pxy + cx * cz * pxz + cy * cz * pyz) - trace;
         fneq[q] = 4.5f * wOf<float>(q) * qPi;
     }
-}
-
-struct Mom {
-    float rho, ux, uy, uz;
-};
-
-// Capped moments of a deviation set (density floor 0.05, |u| cap 0.30, as the
-// collision; lbm_refine.cu:211-226).
-__device__ inline Mom momentsOf(const float h[kQ]) {
-    float r = 1.0f, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-        r += h[q];
-        jx += kCx[q] * h[q];
-        jy += kCy[q] * h[q];
-        jz += kCz[q] * h[q];
-    }
-    r = fmaxf(r, 0.05f);
-    const float inv = 1.0f / r;
-    Mom mo{r, jx * inv, jy * inv, jz * inv};
-    const float u2 = mo.ux * mo.ux + mo.uy * mo.uy + mo.uz * mo.uz;
-    if (u2 > kMaxSpeed * kMaxSpeed) {
-        const float sc = kMaxSpeed * frsqrt(u2);
-        mo.ux *= sc;
-        mo.uy *= sc;
-        mo.uz *= sc;
-    }
-    return mo;
-}
-
-// feq_q(rho, u) - w_q.
-__device__ inline float heq(int q, const Mom& mo) {
-    const float usq = 1.5f * (mo.ux * mo.ux + mo.uy * mo.uy + mo.uz * mo.uz);
-    const float cu = 3.0f * (kCx[q] * mo.ux + kCy[q] * mo.uy + kCz[q] * mo.uz);
-    return wOf<float>(q) * (mo.rho - 1.0f + mo.rho * (cu + 0.5f * cu * cu - usq));
 }
 
 // ---------------------------------------------------------------------------
@@ -403,26 +367,12 @@ __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const 
     if (!live || lx != 0) return;
     const long long nR = static_cast<long long>(rw) * rh * r.cnz;
     const long long ri = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
-    float rec[24];
-    if (nf == 0) {
-#pragma unroll
-        for (int k = 0; k < 24; ++k) rec[k] = 0.0f;
-    } else {
-        const float invN = 1.0f / static_cast<float>(nf);
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) h[q] *= invN;
-        const Mom mo = momentsOf(h);
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) rec[q] = h[q] - heq(q, mo);
-        rec[19] = 1.0f;
-        rec[20] = mo.rho;
-        rec[21] = mo.ux;
-        rec[22] = mo.uy;
-        rec[23] = mo.uz;
-    }
+    float rec[4 * kRecordPlanes];
+    childRecord(h, nf, rec);
     float4* dst = reinterpret_cast<float4*>(scratch);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) dst[k * nR + ri] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    for (int k = 0; k < kRecordPlanes; ++k)
+        dst[k * nR + ri] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
 // Restriction of one coarse cell from its filtered fneq (already divided by
@@ -678,15 +628,16 @@ struct RefineOps {
     }
     static int restrictTo(const LatticeView& f, int fLayout, const LatticeView& c, int cLayout, const RArgs& r,
                           float neqScaleInv, float* rho, float* u, float* v, float* w, float* scratch,
-                          hipStream_t st) {
+                          bool recordsReady, hipStream_t st) {
         const int rw = r.x1 - r.x0 - 2 * kRestrictBand, rh = r.y1 - r.y0 - 2 * kRestrictBand;
         if (rw <= 0 || rh <= 0) return static_cast<int>(hipErrorInvalidValue);
         const bool macro = rho && u && v && w;
         if (scratch) {
             const int cpw = kWave / r.m;
             const long long waves = static_cast<long long>((rw + cpw - 1) / cpw) * rh * r.cnz;
-            childFneqKernel<S><<<static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0,
-                                 st>>>(f.f, f.kind, fLayout, r, scratch);
+            if (!recordsReady)
+                childFneqKernel<S><<<static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock,
+                                     0, st>>>(f.f, f.kind, fLayout, r, scratch);
             constexpr int TX = kStagedTX, TY = kStagedTY, TZ = kStagedTZ;
             const unsigned blocks = static_cast<unsigned>(((rw + TX - 1) / TX) * ((rh + TY - 1) / TY) *
                                                           ((r.cnz + TZ - 1) / TZ));
@@ -766,23 +717,23 @@ long long restrictScratchFloats(const RefineGeom& g) {
 
 int launchFineRestrict(const LatticeView& fine, int fLayout, const LatticeView& coarse, int cLayout,
                        const RefineGeom& g, float neqScaleInv, float* rho, float* u, float* v, float* w,
-                       float* scratch, void* stream) {
+                       float* scratch, int recordsReady, void* stream) {
     if (!geomOk(g) || coarse.storage != fine.storage) return static_cast<int>(hipErrorInvalidValue);
     const RArgs r = makeRArgs(g);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     switch (coarse.storage) {
         case kStoreF32:
-            return RefineOps<StoreF32>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
+            return RefineOps<StoreF32>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, recordsReady != 0, st);
         case kStoreF16:
-            return RefineOps<StoreF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
+            return RefineOps<StoreF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, recordsReady != 0, st);
         case kStoreBF16:
             return RefineOps<StoreBF16>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
-                                                    scratch, st);
+                                                    scratch, recordsReady != 0, st);
         case kStoreF64:
-            return RefineOps<StoreF64>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, st);
+            return RefineOps<StoreF64>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w, scratch, recordsReady != 0, st);
         case kStoreF16C:
             return RefineOps<StoreF16C>::restrictTo(fine, fLayout, coarse, cLayout, r, neqScaleInv, rho, u, v, w,
-                                                    scratch, st);
+                                                    scratch, recordsReady != 0, st);
         default: return static_cast<int>(hipErrorInvalidValue);
     }
 }

@@ -1741,6 +1741,7 @@ public:
         float* dBand = nullptr;
         float* dRestrict = nullptr;  // child-average scratch of the restriction into the parent
         float* dFillCache = nullptr; // shell interpolation of the current parent step
+        bool recordsOk = false;      // restricted box holds only Fluid / Solid cells
         int nband = 0;
         long long bytes = 0;
         WMLevel wm;                            // the level's wall-cell list and slip field
@@ -1818,6 +1819,7 @@ public:
             hipStreamSynchronize(stream_) != hipSuccess)
             return fail(FC_ERR_DEVICE, "fine flag upload failed");
         fine_.hostKind = kb.kind;
+        fine_.recordsOk = restrictedBoxPlain(fine_);
         if (int e = rebuildLevelWall(fine_); e != FC_OK) return e;
         const int x0 = fine_.box.x0 + 2, x1 = fine_.box.x1 - 2, y0 = fine_.box.y0 + 2, y1 = fine_.box.y1 - 2;
         fine_.forcesFromFine = true;
@@ -1845,8 +1847,29 @@ public:
         return FC_OK;
     }
 
+    // Whether the fine cells under the restricted box (the patch box shrunk by
+    // 2 coarse cells) are all Fluid or Solid: the record-writing fine step
+    // takes its children from the collided registers, which other kinds do
+    // not have, so any other kind keeps the separate child-average pass.
+    static bool restrictedBoxPlain(const FineLevel& l) {
+        constexpr int kBand = 2;
+        const int m = l.factor;
+        const int x0 = m * kBand, x1 = l.dims.nx - m * kBand, y0 = m * kBand, y1 = l.dims.ny - m * kBand;
+        if (static_cast<long long>(l.hostFlags.size()) != l.dims.cells() || x1 <= x0 || y1 <= y0) return false;
+        for (int z = 0; z < l.dims.nz; ++z)
+            for (int y = y0; y < y1; ++y)
+                for (int x = x0; x < x1; ++x) {
+                    const std::uint8_t f = l.hostFlags[static_cast<std::size_t>(
+                        x + static_cast<long long>(l.dims.nx) * (y + static_cast<long long>(l.dims.ny) * z))];
+                    if (f != kFluid && f != kSolid) return false;
+                }
+        return true;
+    }
+
     // The m fine sub-steps of one coarse step plus the restriction into the
     // coarse state just written in layout cLayout (lbm_solver.cpp:1172-1290).
+    // With factor 2, no nested level and the default step variant, the last
+    // sub-step writes the restriction's child-average records itself.
     int fineSubSteps(const StepArgs& coarseArgs, int cLayout) {
         const int m = fine_.factor;
         const float tauC = coarseArgs.tau0;
@@ -1858,6 +1881,8 @@ public:
         const RefineGeom g = fineGeom();
         const LatticeView cv = view();
         const LatticeView fv = fineView();
+        const bool records = m == 2 && !finer_.active && fine_.recordsOk && fine_.dRestrict &&
+                             fa.variant == kStepAuto && stepRecordsSupported(fv, g) != 0;
         for (int k = 0; k < m; ++k) {
             if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
                                static_cast<float>(k) / static_cast<float>(m), neqScale, 0, fine_.dFillCache,
@@ -1869,7 +1894,9 @@ public:
                                                     finer_.nband, finer_.dBand, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer band snapshot failed");
             std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(1);
-            const int fe = launchStep(fv, fine_.layout, fa, stream_);
+            const int fe = records && k == m - 1
+                               ? launchStepRecords(fv, fine_.layout, fa, g, fine_.dRestrict, stream_)
+                               : launchStep(fv, fine_.layout, fa, stream_);
             timedEnd(ev);
             if (fe != 0) return fail(FC_ERR_DEVICE, "fine step failed");
             if (finer_.active) {
@@ -1881,7 +1908,7 @@ public:
         const bool macro = coarseArgs.writeMacro != 0;
         if (launchFineRestrict(fv, fine_.layout, cv, cLayout, g, neqScaleInv, macro ? rho_ : nullptr,
                                macro ? u_ : nullptr, macro ? v_ : nullptr, macro ? w_ : nullptr, fine_.dRestrict,
-                               stream_) != 0)
+                               records ? 1 : 0, stream_) != 0)
             return fail(FC_ERR_DEVICE, "fine restriction failed");
         return FC_OK;
     }
@@ -1976,7 +2003,7 @@ public:
         }
         const float neqScaleInv = (static_cast<float>(m2) * tauF) / tauG;
         if (launchFineRestrict(gv, finer_.layout, fv, fLayout, g, neqScaleInv, nullptr, nullptr, nullptr, nullptr,
-                               finer_.dRestrict, stream_) != 0)
+                               finer_.dRestrict, 0, stream_) != 0)
             return fail(FC_ERR_DEVICE, "finer restriction failed");
         return FC_OK;
     }

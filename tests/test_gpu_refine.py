@@ -172,6 +172,39 @@ def test_refined_run_stays_finite_and_counts_updates(gpu):
     s.close()
 
 
+@pytest.mark.parametrize("storage,wall", [("f32", "off"), ("f16c", "off"), ("f16c", "newton"), ("f64", "off")])
+def test_record_writing_fine_step_matches_separate_pass(gpu, storage, wall):
+    """With the default step variant the last fine sub-step of each coarse
+    step writes the restriction's child-average records from its registers;
+    the single-cell variant keeps the separate child-average pass over the
+    fine array. Coarse and fine states and the macro fields must agree bit
+    for bit after several coupled steps."""
+    say(f"case: refined tunnel 160x64x8 {storage} wall={wall}, record-writing fine step vs separate pass")
+    case, box, fine = refined_case()
+    out = []
+    for variant in ("auto", "single"):
+        s = cases.make_solver(case, storage)
+        s.init_refinement(box, 2, fine)
+        if wall != "off":
+            s.set_refined_surface_reference(fine)
+            s.set_wall_model(wall)
+        s.set_step_variant(variant)
+        for _ in range(3):
+            s.step_n(5)
+        s.synchronize()
+        out.append((s.download_f(f64=True), s.download_fine_f(f64=True), s.read_macro()))
+        assert not s.nan_detected()
+        s.close()
+    (ca, fa, ma), (cb, fb, mb) = out
+    # slots of cells without populations (solid, Interface) read back as NaN
+    for name, a, b in (("coarse", ca, cb), ("fine", fa, fb)):
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name
+        d = np.abs(a - b)[~np.isnan(a)]
+        assert not np.count_nonzero(d), f"{name}: {np.count_nonzero(d)} values differ, max {d.max():.3e}"
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x, y, equal_nan=True)
+
+
 # ---- nested third level --------------------------------------------------------
 def test_three_level_uniform_freestream_stays_uniform(gpu):
     """Body-free tunnel with a factor-2 fine patch and a factor-2 finer patch
