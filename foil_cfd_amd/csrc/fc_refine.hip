@@ -198,8 +198,10 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
                                                     const float* band, const int* bandIndex, int nband,
                                                     void* ff, const std::uint8_t* fkind, int fLayout, RArgs r,
                                                     float timeWeight, float neqScale, float* cache,
-                                                    int cacheMode) {
-    const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+                                                    int cacheMode, long long lanes) {
+    const long long lb = xcdBlock((lanes + kBlock - 1) / kBlock);
+    const long long t = lb * kBlock + threadIdx.x;
+    if (t >= lanes) return;
     int xf, yf, zf;
     // Shell interpolation cache: the m sub-steps of one parent step blend the
     // same two interpolated time levels with different weights, so the first
@@ -428,7 +430,10 @@ __global__ void __launch_bounds__(TX * TY * TZ) restrictStagedKernel(void* cf, c
     const int rw = rx1 - rx0, rh = ry1 - ry0;
     const long long nR = static_cast<long long>(rw) * rh * r.cnz;
     const int ntx = (rw + TX - 1) / TX, nty = (rh + TY - 1) / TY;
-    const int b = blockIdx.x;
+    const long long nb = static_cast<long long>(ntx) * nty * ((r.cnz + TZ - 1) / TZ);
+    const long long lbk = xcdBlock(nb);
+    if (lbk >= nb) return;  // workgroup-uniform
+    const int b = static_cast<int>(lbk);
     const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
     const int ox = rx0 + bx * TX, oy = ry0 + by * TY, oz = bz * TZ;
     const int t = static_cast<int>(threadIdx.x);
@@ -616,13 +621,13 @@ struct RefineOps {
                     const LatticeView& f, int fLayout, const RArgs& r, float tw, float neqScale, bool full,
                     float* cache, int cacheMode, hipStream_t st) {
         if (full) {
-            fillKernel<S, true><<<static_cast<unsigned>((r.fn + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale, nullptr, 0);
+            fillKernel<S, true><<<xcdGrid((r.fn + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale, nullptr, 0, r.fn);
         } else {
             const long long n = (4LL * r.fnx + 4LL * (r.fny - 4)) * r.fnz;
-            fillKernel<S, false><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+            fillKernel<S, false><<<xcdGrid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
                 c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale,
-                cache ? cache : nullptr, cache ? cacheMode : 0);
+                cache ? cache : nullptr, cache ? cacheMode : 0, n);
         }
         return static_cast<int>(hipGetLastError());
     }
@@ -639,9 +644,9 @@ struct RefineOps {
                 childFneqKernel<S><<<static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock,
                                      0, st>>>(f.f, f.kind, fLayout, r, scratch);
             constexpr int TX = kStagedTX, TY = kStagedTY, TZ = kStagedTZ;
-            const unsigned blocks = static_cast<unsigned>(((rw + TX - 1) / TX) * ((rh + TY - 1) / TY) *
-                                                          ((r.cnz + TZ - 1) / TZ));
-            restrictStagedKernel<S, TX, TY, TZ><<<blocks, TX * TY * TZ, 0, st>>>(
+            const long long blocks = static_cast<long long>((rw + TX - 1) / TX) * ((rh + TY - 1) / TY) *
+                                     ((r.cnz + TZ - 1) / TZ);
+            restrictStagedKernel<S, TX, TY, TZ><<<xcdGrid(blocks), TX * TY * TZ, 0, st>>>(
                 c.f, c.kind, cLayout, r, neqScaleInv, macro ? rho : nullptr, u, v, w,
                 reinterpret_cast<const float4*>(scratch));
         } else {

@@ -14,6 +14,18 @@ constexpr int kRestrictBand = 2;  // coarse cells skipped at the box edge (lbm_r
 constexpr float kRefineMaxSpeed = 0.30f;
 constexpr int kRecordPlanes = 6;  // float4 planes of one restriction record
 
+// Workgroups are dealt to the XCDs round-robin. A grid of xcdGrid(n)
+// workgroups maps to n logical blocks through xcdBlock so that each XCD
+// takes one contiguous range of them (neighbouring blocks share an L2);
+// blocks at or past n have no work.
+inline unsigned xcdGrid(long long n) { return static_cast<unsigned>((n + kXcds - 1) / kXcds * kXcds); }
+
+__device__ inline long long xcdBlock(long long n) {
+    const long long chunk = (n + kXcds - 1) / kXcds;
+    const long long hb = blockIdx.x;
+    return (hb % kXcds) * chunk + hb / kXcds;
+}
+
 struct Mom {
     float rho, ux, uy, uz;
 };
