@@ -296,6 +296,146 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
     fillFinish<S>(ff, fLayout, r, xf, yf, zf, h, neqScale);
 }
 
+// Shell interpolation for factor 2, grouped by shared corners. The fine
+// cells 2k - 1 and 2k of any axis have the same floor((i + 1/2) / 2 - 1/2),
+// so the shell splits into groups of up to four cells with one set of eight
+// coarse corners: an x pair of a shell row, or a y pair of a shell column,
+// times a z pair (2k - 1, 2k), z = -1 being plane fnz - 1. One lane per
+// group loads each corner value once for all members and writes both
+// interpolated time levels of every member into the fill cache, with each
+// member's own weights and the per-cell summation order of fillKernel, so
+// the cache holds the same bits fillKernel's first sub-step stores; the
+// cached finish (cacheMode 2) then writes the fine populations.
+// Groups g = (kz * 4 + lane kind) ...: per z pair, 4 rows of w + 1 x pairs
+// (x = 2kx - 1 + d), then 4 side columns of h - 1 y pairs (y = 2ky - 1 + d,
+// ky = 1 .. h - 1, members kept when 2 <= y < fny - 2).
+template <class S>
+__global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, const std::uint8_t* ckind, int cLayout,
+                                                            const float* band, const int* bandIndex, int nband,
+                                                            const std::uint8_t* fkind, RArgs r, float* cache,
+                                                            long long groups) {
+    const long long g = xcdBlock((groups + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (g >= groups) return;
+    const int w = r.x1 - r.x0, h = r.y1 - r.y0;
+    const int perZ = 4 * (w + h);
+    const int kz = static_cast<int>(g / perZ);
+    const int rem = static_cast<int>(g - static_cast<long long>(kz) * perZ);
+    const long long ring = 4LL * r.fnx + 4LL * (r.fny - 4);
+    const long long ns = ring * r.fnz;
+    // members (d, e): pair member d along x (rows) or y (columns), z member e
+    int mx[2], my[2];
+    bool ok[2];
+    long long off[2];  // shell offset within a z plane
+    if (rem < 4 * (w + 1)) {
+        const int row = rem / (w + 1), kx = rem - row * (w + 1);
+        const int y = row < 2 ? row : r.fny - 4 + row;
+        for (int d = 0; d < 2; ++d) {
+            mx[d] = 2 * kx - 1 + d;
+            my[d] = y;
+            ok[d] = mx[d] >= 0 && mx[d] < r.fnx;
+            off[d] = (row < 2 ? static_cast<long long>(row) * r.fnx
+                              : 2LL * r.fnx + static_cast<long long>(row - 2) * r.fnx) + mx[d];
+        }
+    } else {
+        const int rem2 = rem - 4 * (w + 1);
+        const int side = rem2 / (h - 1), ky = 1 + rem2 - side * (h - 1);
+        const int x = side < 2 ? side : r.fnx - 4 + side;
+        for (int d = 0; d < 2; ++d) {
+            mx[d] = x;
+            my[d] = 2 * ky - 1 + d;
+            ok[d] = my[d] >= 2 && my[d] < r.fny - 2;
+            off[d] = 4LL * r.fnx + 4LL * (my[d] - 2) + side;
+        }
+    }
+    const int mz[2] = {kz == 0 ? r.fnz - 1 : 2 * kz - 1, 2 * kz};
+    // shared corners, from the representative cell (2 kx | x, 2 ky | y, 2 kz)
+    const float invM = 1.0f / static_cast<float>(r.m);
+    const int rx = ok[1] ? mx[1] : mx[0], ry = ok[1] ? my[1] : my[0];
+    const int ix = static_cast<int>(floorf(static_cast<float>(r.x0) + (static_cast<float>(rx) + 0.5f) * invM - 0.5f));
+    const int iy = static_cast<int>(floorf(static_cast<float>(r.y0) + (static_cast<float>(ry) + 0.5f) * invM - 0.5f));
+    const int iz = kz - 1;
+    int cx[8], cy[8], cz[8], slot[8];
+    bool solid[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        cx[c] = min(max(ix + (c & 1), 0), r.cnx - 1);
+        cy[c] = min(max(iy + ((c >> 1) & 1), 0), r.cny - 1);
+        int zz = iz + ((c >> 2) & 1);
+        if (zz < 0) zz += r.cnz;
+        if (zz >= r.cnz) zz -= r.cnz;
+        cz[c] = zz;
+        solid[c] = (ckind[cellOf(cx[c], cy[c], cz[c], r.cnx, r.cny)] & kKindFlagMask) == kSolid;
+        slot[c] = -1;
+        const int ex = cx[c] - (r.x0 - 1), ey = cy[c] - (r.y0 - 1);
+        if (ex >= 0 && ex < r.ew && ey >= 0 && ey < r.eh) slot[c] = bandIndex[ey * r.ew + ex];
+    }
+    // per member: shell index and normalised weights (as fillKernel), or t = -1
+    long long t[4];
+    float wgt[4][8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int d = m & 1, e = m >> 1;
+        t[m] = -1;
+        if (!ok[d]) continue;
+        const int xf = mx[d], yf = my[d], zf = mz[e];
+        const long long tm = static_cast<long long>(zf) * ring + off[d];
+        if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) {
+            cache[38 * ns + tm] = 0.0f;
+            continue;
+        }
+        const float gx = static_cast<float>(r.x0) + (static_cast<float>(xf) + 0.5f) * invM - 0.5f;
+        const float gy = static_cast<float>(r.y0) + (static_cast<float>(yf) + 0.5f) * invM - 0.5f;
+        const float gz = (static_cast<float>(zf) + 0.5f) * invM - 0.5f;
+        const int jx = static_cast<int>(floorf(gx)), jy = static_cast<int>(floorf(gy)),
+                  jz = static_cast<int>(floorf(gz));
+        const float tx = gx - jx, ty = gy - jy, tz = gz - jz;
+        float wv[8] = {(1 - tx) * (1 - ty) * (1 - tz), tx * (1 - ty) * (1 - tz), (1 - tx) * ty * (1 - tz),
+                       tx * ty * (1 - tz),             (1 - tx) * (1 - ty) * tz, tx * (1 - ty) * tz,
+                       (1 - tx) * ty * tz,             tx * ty * tz};
+        float wsum = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (solid[c]) wv[c] = 0.0f;
+            wsum += wv[c];
+        }
+        if (wsum < 1e-6f) {
+            cache[38 * ns + tm] = 0.0f;
+            continue;
+        }
+        const float invW = 1.0f / wsum;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) wgt[m][c] = wv[c] * invW;
+        t[m] = tm;
+    }
+    if (t[0] < 0 && t[1] < 0 && t[2] < 0 && t[3] < 0) return;
+    for (int q = 0; q < kQ; ++q) {
+        float v0[8], v1[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            v0[c] = v1[c] = 0.0f;
+            if (solid[c]) continue;
+            v1[c] = readH<S>(cf, r.cn, cLayout, q, cx[c], cy[c], cz[c], r.cnx, r.cny, r.cnz);
+            v0[c] = slot[c] < 0 ? v1[c] : band[(static_cast<long long>(q) * r.cnz + cz[c]) * nband + slot[c]];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (t[m] < 0) continue;
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                if (wgt[m][c] == 0.0f) continue;
+                s0 += wgt[m][c] * v0[c];
+                s1 += wgt[m][c] * v1[c];
+            }
+            cache[(2 * q) * ns + t[m]] = s0;
+            cache[(2 * q + 1) * ns + t[m]] = s1;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        if (t[m] >= 0) cache[38 * ns + t[m]] = 1.0f;
+}
+
 // ---------------------------------------------------------------------------
 // Fine-to-coarse restriction. With a scratch array (the solver's path) a
 // child-average pass writes one record per coarse cell of the restricted box
@@ -625,6 +765,13 @@ struct RefineOps {
                 c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale, nullptr, 0, r.fn);
         } else {
             const long long n = (4LL * r.fnx + 4LL * (r.fny - 4)) * r.fnz;
+            if (cache && cacheMode == 1 && r.m == 2) {
+                // grouped interpolation into the cache, then the cached finish
+                const long long groups = 4LL * ((r.x1 - r.x0) + (r.y1 - r.y0)) * r.cnz;
+                shellInterp2Kernel<S><<<xcdGrid((groups + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                    c.f, c.kind, cLayout, band, bandIndex, nband, f.kind, r, cache, groups);
+                cacheMode = 2;
+            }
             fillKernel<S, false><<<xcdGrid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
                 c.f, c.kind, cLayout, band, bandIndex, nband, f.f, f.kind, fLayout, r, tw, neqScale,
                 cache ? cache : nullptr, cache ? cacheMode : 0, n);
