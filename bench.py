@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"f16": 76.0, "f16c": 76.0, "bf16": 76.0, "f32": 152.0, "f64": 304.0}
+RECORD_BYTES = 96.0  # restriction record per coarse cell: 19 fneq, valid mark, 4 moments (FP32)
 METRIC = "GLUPS & % HBM peak, 23.6M-cell wing, FP16-store vs FP32 at ≤1% Cl/Cd err"
 REFERENCE_PUBLISHED_GLUPS_FP32 = 8.151  # RTX 5090, 23.6M cells, FP32 (notes/PERF.md:10-20)
 
@@ -144,11 +145,18 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
     wr = s.wall_model_readout()
     s.close()
     bpc = BYTES_PER_CELL[storage]
+    # With the default variant the second fine sub-step of each coarse step
+    # also writes the restriction's child-average records (96 B per coarse
+    # cell of the box shrunk by 2): counted as half a record set per launch.
+    x0, y0, x1, y1 = box
+    n_rec = (x1 - x0 - 4) * (y1 - y0 - 4) * case.dims[2]
+    rec_bytes = RECORD_BYTES * n_rec / 2 if variant == "auto" else 0.0
     per_level = {}
-    for name, (launches, ms), cells in (("coarse", lv[0], n_c), ("fine", lv[1], n_f)):
+    for name, (launches, ms), cells, extra in (("coarse", lv[0], n_c, 0.0), ("fine", lv[1], n_f, rec_bytes)):
         avg = ms / launches if launches else float("nan")
         per_level[name] = {"cells": cells, "launches": launches, "kernel_ms": avg,
-                           "kernel_gbs": bpc * cells / (avg * 1e-3) / 1e9}
+                           "algorithmic_bytes_per_launch": bpc * cells + extra,
+                           "kernel_gbs": (bpc * cells + extra) / (avg * 1e-3) / 1e9}
     tot_ms = lv[0][1] + lv[1][1]
     res = {
         "glups": updates * steps / wall / 1e9,
@@ -156,6 +164,7 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
         "cell_updates_per_step": updates,
         "kernel_ms": per_level["fine"]["kernel_ms"],
         "kernel_gbs": per_level["fine"]["kernel_gbs"],
+        "kernel_bytes": per_level["fine"]["algorithmic_bytes_per_launch"],
         "all_step_kernels_gbs": bpc * (n_c * lv[0][0] + n_f * lv[1][0]) / (tot_ms * 1e-3) / 1e9,
         "step_kernel_share_of_wall": tot_ms / (kernel_steps / steps * wall * 1e3) if wall > 0 else None,
         "per_level": per_level,
@@ -405,9 +414,10 @@ def main():
                   "cell_updates_per_step": r["cell_updates_per_step"], "storage": store,
                   "wall_model": "iMEM Reichardt (Newton), coarse + fine wall cells",
                   "particles": PARTICLES}
-        kernel = "stepKernel (fused stream-collide), fine-level launch (dominant)"
+        kernel = ("fine-level stream-collide launches (dominant; one of the two per coarse step also writes "
+                  "the restriction's child-average records)")
         traffic = measured_traffic(head, "fine")
-        bytes_per_launch = BYTES_PER_CELL[head] * fd[0] * fd[1] * fd[2]
+        bytes_per_launch = r["kernel_bytes"]
     config.update({"collision": "TRT(3/16) + Smagorinsky(0.1733)", "streaming": "in-place AA",
                    "step_variant": args.variant, "steps_per_call": args.per_call, "parallelism": "single_gpu"})
     out = {

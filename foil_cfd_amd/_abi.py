@@ -261,6 +261,8 @@ SIGNATURES = {
     "fc_group_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(FcGroupInfo)]),
     "fc_set_qlibb_enabled": (C.c_int, [_vp, C.c_int]),
     "fc_set_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
+    "fc_set_fine_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
+    "fc_set_finer_qlinks": (C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), C.c_int, C.c_int]),
     "fc_qlibb_readout": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "fc_particles_seed": (C.c_int, [_vp, C.c_int, C.c_uint]),
     "fc_particles_advect": (C.c_int, [_vp, C.c_float, C.c_uint]),

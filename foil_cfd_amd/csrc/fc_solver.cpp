@@ -137,6 +137,7 @@ Airfoil airfoilFrom(const float* xy, int npts) {
 class Solver {
 public:
     struct WMLevel;    // one level's wall-cell list and slip field
+    struct QLevel;     // one level's sparse q-LIBB link list
     struct FineLevel;  // a refinement patch level
     ~Solver() { freeAll(); }
 
@@ -248,7 +249,7 @@ public:
         wm_.free();
         hipRelease(dMlp_);
         hipRelease(dParticles_);
-        freeQLinks();
+        qc_.free();
         particleCount_ = 0;
         if (hForce_) (void)hipHostFree(hForce_);
         if (hNan_) (void)hipHostFree(hNan_);
@@ -357,14 +358,13 @@ public:
             a.uInlet = effectiveU();
             a.writeMacro = (i == n - 1) ? 1 : 0;
             a.variant = stepVariant_;
-            if (qCount_ > 0 && qlibbEnabled_ &&
-                launchQLibbPrepass(lat, layout_, dQCells_, dQPacked_, qCount_, dQScratch_, stream_) != 0)
-                return fail(FC_ERR_DEVICE, "q-LIBB pre-pass failed");
             // The coarse step overwrites time t in place: keep the ring the
-            // fine shell fill interpolates from (time t) first.
+            // fine shell fill interpolates from (time t) first, before the
+            // q-LIBB pre-pass rewrites any bounce-back slot.
             if (fine_.active &&
                 launchBandSnapshot(lat, layout_, fineGeom(), fine_.dBandXY, fine_.nband, fine_.dBand, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "refinement band snapshot failed");
+            if (int qe = qPrepass(qc_, lat, layout_); qe != FC_OK) return qe;
             std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(0);
             const int e = launchStep(lat, layout_, a, stream_);
             timedEnd(ev);
@@ -418,7 +418,7 @@ public:
     bool groupReady(std::string& why) const {
         if (!initialized_) why = "member not initialized";
         else if (fine_.active) why = "member has a refinement patch";
-        else if (qCount_ > 0 && qlibbEnabled_) why = "member has q-LIBB links";
+        else if (qc_.count > 0 && qlibbEnabled_) why = "member has q-LIBB links";
         else if (!groupEligible(view())) why = "member needs 16/32-bit storage under 2 GiB, x / y not periodic";
         else return true;
         return false;
@@ -482,45 +482,44 @@ public:
     }
 
     // ---- q-LIBB (reference setQLIBBEnabled / setCoarseQLinks, lbm_solver.cpp:1027-1080)
-    void freeQLinks() {
-        hipRelease(dQCells_);
-        hipRelease(dQPacked_);
-        hipRelease(dQScratch_);
-        qCount_ = 0;
-    }
-
     int setQLibbEnabled(bool on) {
         qlibbEnabled_ = on;
-        if (!on) {
-            freeQLinks();
-            qLinks_ = qFallback_ = 0;
+        if (!on) {  // every level reverts to half-way bounce-back (lbm_solver.cpp:1027-1037)
+            qc_.free();
+            fine_.q.free();
+            finer_.q.free();
         }
         return FC_OK;
     }
 
     // Dense per-cell-per-direction cut bytes (cells*19) + per-cell two-node
-    // mask -> the sparse link list the pre-pass walks. Links whose result is
-    // the plain half-way value (qf < 1/2 without a second fluid node) are
-    // dropped from the list.
-    int setQLinks(const std::uint8_t* qFrac, const std::uint32_t* ffMask, int links, int fallback) {
-        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
-        freeQLinks();
-        qLinks_ = qFallback_ = 0;
+    // mask of one level -> the sparse link list the pre-pass walks. Links
+    // whose result is the plain half-way value (qf < 1/2 without a second
+    // fluid node) are dropped from the list. The level's own flags decide
+    // which links exist (Fluid cell, Solid upstream neighbour).
+    int buildQLevel(QLevel& ql, const Dims& d, const std::vector<std::uint8_t>& flags, bool px, bool py,
+                    const std::uint8_t* qFrac, const std::uint32_t* ffMask, int links, int fallback) {
+        ql.free();
         if (!qlibbEnabled_ || !qFrac || !ffMask) return FC_OK;
+        if (static_cast<long long>(flags.size()) != d.cells()) return fail(FC_ERR_STATE, "level flags missing");
+        const auto idx = [&](int x, int y, int z) {
+            return static_cast<std::size_t>(x) +
+                   static_cast<std::size_t>(d.nx) * (static_cast<std::size_t>(y) + static_cast<std::size_t>(d.ny) * z);
+        };
         std::vector<int> cells;
         std::vector<std::uint32_t> packed;
-        for (int z = 0; z < dims_.nz; ++z)
-            for (int y = 0; y < dims_.ny; ++y)
-                for (int x = 0; x < dims_.nx; ++x) {
-                    const std::size_t c = at(x, y, z);
-                    if (hostFlags_[c] != kFluid) continue;
+        for (int z = 0; z < d.nz; ++z)
+            for (int y = 0; y < d.ny; ++y)
+                for (int x = 0; x < d.nx; ++x) {
+                    const std::size_t c = idx(x, y, z);
+                    if (flags[c] != kFluid) continue;
                     for (int q = 1; q < kQ; ++q) {
                         const std::uint8_t b = qFrac[c * kQ + q];
                         if (!b) continue;
-                        const int xn = stepAxis(x, -kCx[q], dims_.nx, periodicX_);
-                        const int yn = stepAxis(y, -kCy[q], dims_.ny, periodicY_);
-                        const int zn = stepAxis(z, -kCz[q], dims_.nz, true);
-                        if (hostFlags_[at(xn, yn, zn)] != kSolid) continue;
+                        const int xn = stepAxis(x, -kCx[q], d.nx, px);
+                        const int yn = stepAxis(y, -kCy[q], d.ny, py);
+                        const int zn = stepAxis(z, -kCz[q], d.nz, true);
+                        if (xn < 0 || yn < 0 || flags[idx(xn, yn, zn)] != kSolid) continue;
                         const bool ff = (ffMask[c] >> q) & 1u;
                         const float qf = static_cast<float>(b) * (1.0f / 255.0f);
                         if (qf < 0.5f && !ff) continue;
@@ -531,32 +530,61 @@ public:
                 }
         const std::size_t n = cells.size();
         if (n > 0) {
-            int rc = hipAlloc(&dQCells_, n * sizeof(int));
-            rc |= hipAlloc(&dQPacked_, n * sizeof(std::uint32_t));
-            rc |= hipAlloc(&dQScratch_, n * sizeof(double));
+            int rc = hipAlloc(&ql.dCells, n * sizeof(int));
+            rc |= hipAlloc(&ql.dPacked, n * sizeof(std::uint32_t));
+            rc |= hipAlloc(&ql.dScratch, n * sizeof(double));
             if (rc != FC_OK) {
-                freeQLinks();
+                ql.free();
                 return fail(FC_ERR_ALLOC, "q-link allocation failed");
             }
-            if (hipMemcpyAsync(dQCells_, cells.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_) != hipSuccess ||
-                hipMemcpyAsync(dQPacked_, packed.data(), n * sizeof(std::uint32_t), hipMemcpyHostToDevice, stream_) !=
+            if (hipMemcpyAsync(ql.dCells, cells.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_) !=
                     hipSuccess ||
+                hipMemcpyAsync(ql.dPacked, packed.data(), n * sizeof(std::uint32_t), hipMemcpyHostToDevice,
+                               stream_) != hipSuccess ||
                 hipStreamSynchronize(stream_) != hipSuccess) {
-                freeQLinks();
+                ql.free();
                 return fail(FC_ERR_DEVICE, "q-link upload failed");
             }
-            qCount_ = static_cast<int>(n);
+            ql.count = static_cast<int>(n);
         }
-        qLinks_ = links;
-        qFallback_ = fallback;
+        ql.links = links;
+        ql.fallback = fallback;
         return FC_OK;
     }
 
+    // setCoarseQLinks / setFineQLinks / setFinerQLinks (lbm_solver.cpp:
+    // 1041-1063): level 0, 1 or 2. A patch level that is not active refuses.
+    int setQLinks(int level, const std::uint8_t* qFrac, const std::uint32_t* ffMask, int links, int fallback) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (level == 0) return buildQLevel(qc_, dims_, hostFlags_, periodicX_, periodicY_, qFrac, ffMask, links, fallback);
+        FineLevel* l = level == 1 ? &fine_ : level == 2 ? &finer_ : nullptr;
+        if (!l) return fail(FC_ERR_INVALID, "level must be 0, 1 or 2");
+        if (!l->active) return fail(FC_ERR_STATE, "refinement level not active");
+        return buildQLevel(l->q, l->dims, l->hostFlags, false, false, qFrac, ffMask, links, fallback);
+    }
+
+    // The sparse q-LIBB pre-pass of a level, just before its step.
+    int qPrepass(const QLevel& ql, const LatticeView& lat, int layout) {
+        if (!qlibbEnabled_ || ql.count <= 0) return FC_OK;
+        if (launchQLibbPrepass(lat, layout, ql.dCells, ql.dPacked, ql.count, ql.dScratch, stream_) != 0)
+            return fail(FC_ERR_DEVICE, "q-LIBB pre-pass failed");
+        return FC_OK;
+    }
+
+    // qlibbReadout (lbm_solver.cpp:1073-1085): links and fallbacks summed
+    // over the levels that hold links; active = links in the pre-pass lists.
     void qlibbReadout(int* enabled, int* links, int* fallback, int* active) const {
+        int l = 0, fb = 0, a = 0;
+        if (qlibbEnabled_)
+            for (const QLevel* q : {&qc_, &fine_.q, &finer_.q}) {
+                l += q->links;
+                fb += q->fallback;
+                a += q->count;
+            }
         if (enabled) *enabled = qlibbEnabled_ ? 1 : 0;
-        if (links) *links = qLinks_;
-        if (fallback) *fallback = qFallback_;
-        if (active) *active = qCount_;
+        if (links) *links = l;
+        if (fallback) *fallback = fb;
+        if (active) *active = a;
     }
 
     // ---- tracer particles (reference particles.cu / viz.cpp) ---------------
@@ -1675,6 +1703,19 @@ private:
 public:
     // ---- two-level refinement (reference initRefinement / stepN coupling,
     // lbm_solver.cpp:1816-1895, :1172-1290) ----------------------------------
+    struct QLevel {
+        int* dCells = nullptr;            // link cells
+        std::uint32_t* dPacked = nullptr;  // q | cut byte << 8 | two-node bit << 16
+        double* dScratch = nullptr;
+        int count = 0, links = 0, fallback = 0;
+        void free() {
+            hipRelease(dCells);
+            hipRelease(dPacked);
+            hipRelease(dScratch);
+            count = links = fallback = 0;
+        }
+    };
+
     struct WMLevel {
         bool active = false;
         int count = 0;
@@ -1745,6 +1786,7 @@ public:
         int nband = 0;
         long long bytes = 0;
         WMLevel wm;                            // the level's wall-cell list and slip field
+        QLevel q;                              // the level's q-LIBB links
         std::vector<std::uint8_t> hostKind;    // host copy of the level's kind bytes
         std::vector<std::uint8_t> surfaceRef;  // clean (VG-free) flags for wall normals
     };
@@ -1786,6 +1828,7 @@ public:
         hipRelease(l.dRestrict);
         hipRelease(l.dFillCache);
         l.wm.free();
+        l.q.free();
         l = FineLevel{};
     }
     // The finer level nests in the fine one: both go together.
@@ -1893,6 +1936,7 @@ public:
             if (finer_.active && launchBandSnapshot(fv, fine_.layout, geomOf(fine_.dims, finer_), finer_.dBandXY,
                                                     finer_.nband, finer_.dBand, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer band snapshot failed");
+            if (int qe = qPrepass(fine_.q, fv, fine_.layout); qe != FC_OK) return qe;
             std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(1);
             const int fe = records && k == m - 1
                                ? launchStepRecords(fv, fine_.layout, fa, g, fine_.dRestrict, stream_)
@@ -1995,6 +2039,7 @@ public:
                                static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, finer_.dFillCache,
                                q == 0 ? 1 : 2, stream_) != 0)
                 return fail(FC_ERR_DEVICE, "finer interface fill failed");
+            if (int qe = qPrepass(finer_.q, gv, finer_.layout); qe != FC_OK) return qe;
             std::pair<hipEvent_t, hipEvent_t>* ev = timedBegin(2);
             const int ge = launchStep(gv, finer_.layout, ga, stream_);
             timedEnd(ev);
@@ -2262,10 +2307,7 @@ private:
     MlpParams* dMlp_ = nullptr;
     float* dParticles_ = nullptr;
     bool qlibbEnabled_ = false;
-    int* dQCells_ = nullptr;
-    std::uint32_t* dQPacked_ = nullptr;
-    double* dQScratch_ = nullptr;
-    int qCount_ = 0, qLinks_ = 0, qFallback_ = 0;
+    QLevel qc_;  // coarse-level q-LIBB links
     int particleCount_ = 0;
     ParticleParams particleParams_;
     long long ncells_ = 0;
@@ -2932,7 +2974,13 @@ int fc_set_qlibb_enabled(fc_solver* s, int enabled) {
     return s ? s->impl.setQLibbEnabled(enabled != 0) : FC_ERR_INVALID;
 }
 int fc_set_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback) {
-    return s ? s->impl.setQLinks(qfrac, ffmask, links, fallback) : FC_ERR_INVALID;
+    return s ? s->impl.setQLinks(0, qfrac, ffmask, links, fallback) : FC_ERR_INVALID;
+}
+int fc_set_fine_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback) {
+    return s ? s->impl.setQLinks(1, qfrac, ffmask, links, fallback) : FC_ERR_INVALID;
+}
+int fc_set_finer_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback) {
+    return s ? s->impl.setQLinks(2, qfrac, ffmask, links, fallback) : FC_ERR_INVALID;
 }
 int fc_qlibb_readout(fc_solver* s, int* enabled, int* links, int* fallback, int* active_links) {
     if (!s) return FC_ERR_INVALID;

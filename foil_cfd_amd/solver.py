@@ -622,6 +622,25 @@ class LBMSolver:
         check(self._lib.fc_set_qlinks(self._h, _u8(qf), ff.ctypes.data_as(C.POINTER(C.c_uint32)),
                                       links, fallback), "fc_set_qlinks")
 
+    def set_fine_qlinks(self, qfrac, ffmask, links=0, fallback=0, level=1):
+        """setFineQLinks / setFinerQLinks (lbm_solver.cpp:1049-1063): q-link
+        fields over the fine patch (level 1) or the nested finer patch
+        (level 2), sized by that level's cells."""
+        if level == 1:
+            info, key, fn = self.refinement_info(), "fine_dims", "fc_set_fine_qlinks"
+        elif level == 2:
+            info, key, fn = self.finer_refinement_info(), "finer_dims", "fc_set_finer_qlinks"
+        else:
+            raise ValueError("level must be 1 or 2")
+        n = int(np.prod(info[key]))
+        qf = np.ascontiguousarray(qfrac, np.uint8).reshape(-1)
+        ff = np.ascontiguousarray(ffmask, np.uint32).reshape(-1)
+        # an inactive level is refused by the library (FC_ERR_STATE)
+        if info["active"] and (qf.size != Q * n or ff.size != n):
+            raise ValueError("q-link field size mismatch")
+        check(getattr(self._lib, fn)(self._h, _u8(qf), ff.ctypes.data_as(C.POINTER(C.c_uint32)), links,
+                                     fallback), fn)
+
     def qlibb_readout(self):
         v = [C.c_int(0) for _ in range(4)]
         check(self._lib.fc_qlibb_readout(self._h, *[C.byref(x) for x in v]), "fc_qlibb_readout")

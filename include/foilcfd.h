@@ -467,6 +467,13 @@ int fc_set_qlibb_enabled(fc_solver* s, int enabled);
  * b -> q = b/255) and the per-cell two-node mask ffmask[cell] (bit q); links /
  * fallback are the readout counts (QLIBBReadout, lbm_solver.h:97-101). */
 int fc_set_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback);
+/* setFineQLinks / setFinerQLinks, :332-337 (lbm_solver.cpp:1049-1063): the
+ * same fields over the fine patch (fine cells) or the nested finer patch
+ * (finer cells); FC_ERR_STATE when that level is not active. The link field
+ * is freed with its level. The pre-pass of a level runs just before each of
+ * its sub-steps, after that sub-step's shell fill and band snapshots. */
+int fc_set_fine_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback);
+int fc_set_finer_qlinks(fc_solver* s, const uint8_t* qfrac, const uint32_t* ffmask, int links, int fallback);
 /* qlibbReadout, :346 (+ active_links: links the kernel interpolates). */
 int fc_qlibb_readout(fc_solver* s, int* enabled, int* links, int* fallback, int* active_links);
 

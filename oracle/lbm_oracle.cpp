@@ -831,7 +831,14 @@ void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* f
                 const uint8_t* fFlags, const Grid& fg, const PBox& box, int m, int nsteps,
                 const T* taus, const T* uins, T lam, T pre, T* rho, T* u, T* v, T* w,
                 const FinerLevel<T>* finer = nullptr, const uint16_t* const* cSlip = nullptr,
-                const uint16_t* const* fSlip = nullptr) {
+                const uint16_t* const* fSlip = nullptr, const uint8_t* const* qFracs = nullptr,
+                const uint32_t* const* ffMasks = nullptr) {
+    // optional q-LIBB fields per level (coarse, fine), as the reference's
+    // streamCollideKernel<WM, QLIBB> takes each level's QLinkView
+    const uint8_t* cq = qFracs ? qFracs[0] : nullptr;
+    const uint8_t* fq = qFracs ? qFracs[1] : nullptr;
+    const uint32_t* cff = ffMasks ? ffMasks[0] : nullptr;
+    const uint32_t* fff = ffMasks ? ffMasks[1] : nullptr;
     // optional wall-model slip fields (half bits) of the coarse and fine
     // levels, held fixed over the run as within one stepN batch
     const uint16_t* cux = cSlip ? cSlip[0] : nullptr;
@@ -851,14 +858,14 @@ void runRefined(T* cA, T* cB, const uint8_t* cFlags, const Grid& cg, T* fA, T* f
         const float tauF = fineTauFor(tauC, m);
         const bool last = i == nsteps - 1;
         StepArgs a{taus[i], lam, pre, uins[i], last};
-        streamCollide<T, T>(cs, cd, cFlags, cg, a, rho, u, v, w, cux, cuy, cuz, nullptr, nullptr);
+        streamCollide<T, T>(cs, cd, cFlags, cg, a, rho, u, v, w, cux, cuy, cuz, cq, cff);
         StepArgs af{(T)tauF, lam, pre, uins[i], 0};
         const float neqScale = tauF / ((float)m * tauC);
         for (int k = 0; k < m; ++k) {
             coarseToFineFill<T>(cs, cd, cFlags, cg, fFlags, fs, fg, box, m,
                                 (T)((float)k / (float)m), (T)neqScale, false);
             streamCollide<T, T>(fs, fd, fFlags, fg, af, (T*)nullptr, (T*)nullptr, (T*)nullptr,
-                                (T*)nullptr, fux, fuy, fuz, nullptr, nullptr);
+                                (T*)nullptr, fux, fuy, fuz, fq, fff);
             if (finer) {
                 const int m2 = finer->m2;
                 const float tauG = fineTauFor(tauF, m2);
@@ -1224,19 +1231,23 @@ void orc_run_refined_f64(double* cA, double* cB, const uint8_t* cFlags, int cnx,
                        taus, uins, lam, pre, rho, u, v, w);
 }
 // Two levels with the coarse and fine wall-model slip fields (3 half-bit
-// arrays each, NULL = no slip on that level).
+// arrays each, NULL = no slip on that level) and q-LIBB fields (cut bytes
+// and two-node masks per level, NULL = plain bounce-back on that level).
 void orc_run_refined_slip_f64(double* cA, double* cB, const uint8_t* cFlags, int cnx, int cny, int cnz,
                               double* fA, double* fB, const uint8_t* fFlags, int x0, int y0, int x1, int y1,
                               int m, int nsteps, const double* taus, const double* uins, double lam, double pre,
                               double* rho, double* u, double* v, double* w, const uint16_t* cux,
                               const uint16_t* cuy, const uint16_t* cuz, const uint16_t* fux, const uint16_t* fuy,
-                              const uint16_t* fuz) {
+                              const uint16_t* fuz, const uint8_t* cq, const uint32_t* cff, const uint8_t* fq,
+                              const uint32_t* fff) {
     const Grid cg = makeGrid(cnx, cny, cnz, 0, 0);
     const Grid fg = makeGrid(m * (x1 - x0), m * (y1 - y0), m * cnz, 0, 0);
     const uint16_t* cs[3] = {cux, cuy, cuz};
     const uint16_t* fs[3] = {fux, fuy, fuz};
+    const uint8_t* qs[2] = {cff ? cq : nullptr, fff ? fq : nullptr};
+    const uint32_t* ms[2] = {cq ? cff : nullptr, fq ? fff : nullptr};
     runRefined<double>(cA, cB, cFlags, cg, fA, fB, fFlags, fg, PBox{x0, y0, x1, y1}, m, nsteps, taus, uins, lam,
-                       pre, rho, u, v, w, nullptr, cux ? cs : nullptr, fux ? fs : nullptr);
+                       pre, rho, u, v, w, nullptr, cux ? cs : nullptr, fux ? fs : nullptr, qs, ms);
 }
 void orc_run_refined_f32(float* cA, float* cB, const uint8_t* cFlags, int cnx, int cny, int cnz,
                          float* fA, float* fB, const uint8_t* fFlags, int x0, int y0, int x1,

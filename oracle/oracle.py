@@ -95,7 +95,8 @@ def lib():
                                                                    C.c_double] + [_dp] * 4
         L.orc_run_refined_slip_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                                _u8p] + [C.c_int] * 6 + [_dp, _dp, C.c_double,
-                                                                        C.c_double] + [_dp] * 4 + [_u16p] * 6
+                                                                        C.c_double] + [_dp] * 4 + [_u16p] * 6 + \
+            [_u8p, _u32p, _u8p, _u32p]
         L.orc_run_refined3_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                            _u8p] + [C.c_int] * 5 + [_dp, _dp, _u8p] + [C.c_int] * 5 + \
             [C.c_int, _dp, _dp, C.c_double, C.c_double] + [_dp] * 4
@@ -428,15 +429,16 @@ def seed_fine(cf, cflags, cdims, fflags, box, m, tau_c, f64=True):
 
 
 def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.0, cs=0.1733,
-                coarse_slip=None, fine_slip=None):
+                coarse_slip=None, fine_slip=None, coarse_q=None, fine_q=None):
     """len(taus) refined steps (coarse + m fine sub-steps + restriction,
     lbm_solver.cpp:1144-1290); cf / ff are advanced in place. Returns the
     coarse (rho, u, v, w) of the last step. coarse_slip / fine_slip: optional
-    wall-model slip fields (3 half-bit arrays per level, FP64 runs only)."""
+    wall-model slip fields (3 half-bit arrays per level); coarse_q / fine_q:
+    optional q-LIBB fields (qfrac, ffmask) per level (FP64 runs only)."""
     nx, ny, nz = cdims
     n = nx * ny * nz
     f64 = cf.dtype == np.float64
-    if coarse_slip is not None or fine_slip is not None:
+    if any(a is not None for a in (coarse_slip, fine_slip, coarse_q, fine_q)):
         assert f64 and ff.dtype == np.float64
         ctmp, ftmp = np.empty_like(cf), np.empty_like(ff)
         taus = np.ascontiguousarray(taus, dtype=np.float64)
@@ -447,6 +449,15 @@ def run_refined(cf, ff, cflags, fflags, cdims, box, m, taus, uins, lam=3.0 / 16.
         ptrs = []
         for a in sl:
             ptrs += [None] * 3 if a is None else [_p(x, _u16p) for x in a]
+        keep = []
+        for qa in (coarse_q, fine_q):
+            if qa is None:
+                ptrs += [None, None]
+            else:
+                qf = np.ascontiguousarray(qa[0], np.uint8)
+                fm = np.ascontiguousarray(qa[1], np.uint32)
+                keep += [qf, fm]
+                ptrs += [_p(qf, _u8p), _p(fm, _u32p)]
         lib().orc_run_refined_slip_f64(_p(cf, _dp), _p(ctmp, _dp), _p(np.ascontiguousarray(cflags, np.uint8), _u8p),
                                        nx, ny, nz, _p(ff, _dp), _p(ftmp, _dp),
                                        _p(np.ascontiguousarray(fflags, np.uint8), _u8p), *box, m, len(taus),

@@ -95,3 +95,78 @@ def test_qlibb_parity(gpu, storage):
     assert np.linalg.norm(plain - a) / np.linalg.norm(a) > 100 * tol
     s.close()
     s2.close()
+
+
+def synthetic_qlinks_fast(flags, dims, seed):
+    """Cut bytes in [13, 255] from a seeded generator on every Fluid cell
+    link whose upstream neighbour is Solid, two-node bit where the
+    downstream neighbour is Fluid (x / y borders of a patch level are
+    Interface cells, so the wrap of np.roll never pairs two fluid cells)."""
+    nx, ny, nz = dims
+    g = np.asarray(flags, np.uint8).reshape(nz, ny, nx)
+    rng = np.random.default_rng(seed)
+    qfrac = np.zeros((nz, ny, nx, Q), np.uint8)
+    ffm = np.zeros((nz, ny, nx), np.uint32)
+    fluid = g == fc.FLUID
+    links = 0
+    for q in range(1, Q):
+        sh = (int(fc.CZ[q]), int(fc.CY[q]), int(fc.CX[q]))
+        up = np.roll(g, sh, axis=(0, 1, 2))
+        down = np.roll(g, tuple(-v for v in sh), axis=(0, 1, 2))
+        link = fluid & (up == fc.SOLID)
+        qfrac[..., q][link] = rng.integers(13, 256, size=int(link.sum()), dtype=np.uint8)
+        ffm[link & (down == fc.FLUID)] |= np.uint32(1 << q)
+        links += int(link.sum())
+    return qfrac.reshape(-1), ffm.reshape(-1), links
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_fine_level_qlibb_parity(gpu, storage):
+    """setFineQLinks (lbm_solver.cpp:1049-1055): q-links on the fine patch.
+    The coupled run matches the oracle's refined run with the fine level's
+    q fields at the refinement tolerance (the coupling computes in FP32), the
+    links change the fine flow, the readout sums the levels, and switching
+    q-LIBB off frees every level's links."""
+    print(f"case: refined tunnel 160x64x8 with fine-level q-links, {storage}", flush=True)
+    case = cases.naca_tunnel(160, 64, 8, 32, aoa=8.0, cs=0.1, u_lat=0.05, tau=0.56, ramp=False)
+    box, fine = cases.refinement_patch(case)
+    s = cases.make_solver(case, storage)
+    s.init_refinement(box, 2, fine)
+    fd = s.refinement_info()["fine_dims"]
+    qfrac, ffm, links = synthetic_qlinks_fast(fine, fd, seed=99)
+    s.set_qlibb_enabled(True)
+    with pytest.raises(RuntimeError):
+        s.set_fine_qlinks(qfrac, ffm, links, 0, level=2)  # no nested level
+    s.set_fine_qlinks(qfrac, ffm, links, 3)
+    r = s.qlibb_readout()
+    assert r["enabled"] == 1 and r["links"] == links and r["fallback"] == 3 and r["active"] > 0
+    cf = s.download_f(f64=True)
+    ff = s.download_fine_f(f64=True)
+    steps = 12
+    s.step_n(steps)
+    s.synchronize()
+    got_c = s.download_f(f64=True).reshape(Q, -1)
+    got_f = s.download_fine_f(f64=True).reshape(Q, -1)
+    sc = case.scaling
+    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, case.dims[0], steps, ramp=case.ramp)
+    orc.run_refined(cf, ff, case.flags, fine, case.dims, box, 2, taus.astype(np.float64), uins.astype(np.float64),
+                    cs=case.cs, fine_q=(qfrac, ffm))
+    cfl = case.flags == fc.FLUID
+    ffl = fine == fc.FLUID
+    ec = np.linalg.norm(got_c[:, cfl] - cf.reshape(Q, -1)[:, cfl]) / np.linalg.norm(cf.reshape(Q, -1)[:, cfl])
+    ef = np.linalg.norm(got_f[:, ffl] - ff.reshape(Q, -1)[:, ffl]) / np.linalg.norm(ff.reshape(Q, -1)[:, ffl])
+    print(f"  {r['active']} active links; coarse {ec:.3e}, fine {ef:.3e} rel L2 vs the oracle", flush=True)
+    assert ec < 1e-6 and ef < 1e-6
+    plain = cases.make_solver(case, storage)
+    plain.init_refinement(box, 2, fine)
+    plain.step_n(steps)
+    plain.synchronize()
+    pf = plain.download_fine_f(f64=True).reshape(Q, -1)[:, ffl]
+    dq = np.linalg.norm(pf - got_f[:, ffl]) / np.linalg.norm(got_f[:, ffl])
+    print(f"  plain bounce-back vs q-LIBB on the fine level: {dq:.3e}", flush=True)
+    assert dq > 3e-5  # the links' effect is far above the parity tolerance
+    s.set_qlibb_enabled(False)
+    r = s.qlibb_readout()
+    assert r["links"] == 0 and r["active"] == 0
+    s.close()
+    plain.close()

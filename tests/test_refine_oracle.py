@@ -270,3 +270,65 @@ def test_refined_run_slip_wiring():
     e_c, _ = f0.copy(), g0.copy()
     O.run_refined(e_c, g0.copy(), flags, fine, dims, box, m, taus[:1], uins[:1])
     assert not np.array_equal(e_c.reshape(19, -1)[:, sel], cc)
+
+
+def test_refined_run_qlinks_wiring():
+    """The q-LIBB-aware refined runner: zero cut fields equal the plain run
+    bit for bit; a coarse q field changes the coarse step outside the patch
+    exactly as the single-level step with that field does; a fine q field
+    around a solid block inside the patch changes the fine state."""
+    import numpy as np
+    from oracle import oracle as O
+    import foil_cfd_amd as fc
+    dims, box, m = (48, 32, 4), (14, 10, 30, 22), 2
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    flags = fc.build_boundary_flags(*dims).reshape(nz, ny, nx)
+    flags[:, 4:6, 4:8] = 1  # a small solid block outside the patch
+    flags = flags.reshape(-1)
+    fd = O.fine_dims(box, dims, m)
+    nf = fd[0] * fd[1] * fd[2]
+    fine = np.zeros(nf, np.uint8)
+    fc.stamp_interface_shell(*fd, fine)
+    fine.reshape(fd[2], fd[1], fd[0])[:, 10:13, 12:16] = 1  # a solid block inside the patch
+    rng = np.random.default_rng(1)
+    f0 = O.init_equilibrium(flags, dims, 0.05, f64=True)
+    f0 += 1e-4 * rng.standard_normal(f0.size)
+    g0 = O.init_equilibrium(fine, fd, 0.05, f64=True)
+    taus, uins = np.full(2, 0.56), np.full(2, 0.05)
+
+    def qfield(fl, d):
+        x, y, z = d
+        g = fl.reshape(z, y, x)
+        qf = np.zeros((z, y, x, 19), np.uint8)
+        ff = np.zeros((z, y, x), np.uint32)
+        for q in range(1, 19):
+            sh = (int(O.CZ[q]), int(O.CY[q]), int(O.CX[q]))
+            link = (g == 0) & (np.roll(g, sh, axis=(0, 1, 2)) == 1)
+            qf[..., q][link] = 200
+            ff[link & (np.roll(g, tuple(-v for v in sh), axis=(0, 1, 2)) == 0)] |= np.uint32(1 << q)
+        return qf.reshape(-1), ff.reshape(-1)
+
+    a_c, a_f = f0.copy(), g0.copy()
+    O.run_refined(a_c, a_f, flags, fine, dims, box, m, taus, uins)
+    b_c, b_f = f0.copy(), g0.copy()
+    zero = (np.zeros(n * 19, np.uint8), np.zeros(n, np.uint32))
+    O.run_refined(b_c, b_f, flags, fine, dims, box, m, taus, uins, coarse_q=zero,
+                  fine_q=(np.zeros(nf * 19, np.uint8), np.zeros(nf, np.uint32)))
+    assert np.array_equal(a_c, b_c) and np.array_equal(a_f, b_f)
+    cq = qfield(flags, dims)
+    assert cq[0].any()
+    c_c, c_f = f0.copy(), g0.copy()
+    O.run_refined(c_c, c_f, flags, fine, dims, box, m, taus[:1], uins[:1], coarse_q=cq)
+    d = f0.copy()
+    O.run(d, flags, dims, taus[:1], uins[:1], qfrac=cq[0], ffmask=cq[1], cs=0.1733)
+    x, y = np.meshgrid(np.arange(nx), np.arange(ny))
+    outside = np.tile(((x < box[0] - 2) | (x >= box[2] + 2) | (y < box[1] - 2) | (y >= box[3] + 2)).reshape(-1), nz)
+    sel = outside & (flags == 0)
+    assert np.array_equal(c_c.reshape(19, -1)[:, sel], d.reshape(19, -1)[:, sel])
+    p_c, p_f = f0.copy(), g0.copy()
+    O.run_refined(p_c, p_f, flags, fine, dims, box, m, taus[:1], uins[:1])
+    assert not np.array_equal(c_c, p_c)  # the coarse links took effect
+    e_c, e_f = f0.copy(), g0.copy()
+    O.run_refined(e_c, e_f, flags, fine, dims, box, m, taus, uins, fine_q=qfield(fine, fd))
+    assert not np.array_equal(e_f, a_f)
