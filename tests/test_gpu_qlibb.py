@@ -170,3 +170,51 @@ def test_fine_level_qlibb_parity(gpu, storage):
     assert r["links"] == 0 and r["active"] == 0
     s.close()
     plain.close()
+
+
+def test_vg_vanes_on_fine_level(gpu):
+    """The vortex-generator path on the patch level, as the reference app
+    runs it (main.cpp:520-560): vanes voxelized into the fine flags
+    (buildFlagsWithVGs on the fine layout), their cut links (buildVaneQLinks
+    + densifyQLinks) uploaded with setFineQLinks, the wall model on both
+    levels against the clean references. The run stays finite with every
+    link active, and the record-writing fine step (vanes are Solid, so the
+    restricted box stays Fluid / Solid) equals the separate child-average
+    pass bit for bit."""
+    from foil_cfd_amd import solver as S
+    print("case: refined tunnel 192x72x16 at 10 deg, VG pair on the fine level, f16c", flush=True)
+    case = cases.naca_tunnel(192, 72, 16, 48, aoa=10.0, cs=0.1, u_lat=0.05, tau=0.56, ramp=False)
+    box, clean = cases.refinement_patch(case, aoa=10.0)
+    layout = S.make_layout(*case.dims, case.scaling.chord_cells)
+    fl = S.make_fine_layout(layout, S.FcPatchBox(*box), 2)
+    pts, le = S.generate_naca4("2412", 200)
+    vg = S.vg_params(x_c=0.2, height_c=0.04, length_h=2.5, beta_deg=16.0, pitch_c=0.24, gap_h=2.5, count=2)
+    flags, slabs = S.build_flags_with_vgs(pts, le, 10.0, fl, [vg], clean)
+    fd = (fl.dims.nx, fl.dims.ny, fl.dims.nz)
+    assert ((flags == fc.SOLID) & (clean != fc.SOLID)).sum() > 100
+    qf, ff, links = S.build_vane_qlinks(fd, flags, clean, slabs)
+    assert links > 100
+    out = []
+    for variant in ("auto", "single"):
+        s = cases.make_solver(case, "f16c")
+        s.set_surface_reference(case.flags)
+        s.init_refinement(box, 2, flags)
+        s.set_refined_surface_reference(clean)
+        s.set_wall_model("newton")
+        s.set_qlibb_enabled(True)
+        s.set_fine_qlinks(qf, ff, links, 0)
+        s.set_step_variant(variant)
+        r = s.qlibb_readout()
+        assert r["links"] == links and r["active"] > 0
+        for _ in range(4):
+            s.step_n(25)
+        s.synchronize()
+        assert not s.nan_detected()
+        assert s.wall_cell_count_level(1) > 0
+        out.append((s.download_f(f64=True), s.download_fine_f(f64=True)))
+        print(f"  {variant}: {r['active']} active fine links, fine wall cells {s.wall_cell_count_level(1)}",
+              flush=True)
+        s.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert not np.count_nonzero((a - b)[~np.isnan(a)])
