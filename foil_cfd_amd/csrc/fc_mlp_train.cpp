@@ -2,7 +2,7 @@
 // samples and batched inference (include/foilcfd.h, fc_mlp_*). No reference
 // counterpart: the reference solves Reichardt's law per wall cell by Newton
 // (src/sim/lbm_wallmodel.cu:55-73), which stays the fallback outside the
-// training envelope and the judge of the network's error.
+// training envelope and the arbiter of the network's error.
 //
 // Data on the GPU: normalised inputs X (SoA [3][n]) and targets T[n], the
 // FP32 master parameters with Adam's moments, the minibatch index buffer and

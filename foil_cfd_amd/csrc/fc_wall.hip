@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(128) wallUpdateKernel(KArgs a, int layout, Wal
             const float itx = tx / ut, ity = ty / ut, itz = tz / ut;
             const float prev = wl.uTau[i];
             const float solved = usedMlp ? net : solveUTau(ut, ys, p.nuLat, prev);
-            if (harvest) {  // the Newton solution is the training target and the network's judge
+            if (harvest) {  // the Newton solution is the training target and the network's arbiter
                 float* h = harvest + 4 * static_cast<long long>(i);
                 h[0] = ys;
                 h[1] = ut;

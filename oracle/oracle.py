@@ -6,9 +6,10 @@ reference's own units.h + NACA generator, built only where /root/reference
 exists). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 leg may import this module; the product never does.
 
-Parity status of the oracle functions (what pins each one) is listed in
-DESIGN.md; every function here checked by at least one known answer is
-exercised in tests/test_oracle.py.
+Every function here is pinned by at least one known answer, closed form or
+invariant (exact moments, conservation, planted solutions, the reference's
+own test values and the compiled reference's golden vectors); those checks
+live in tests/test_oracle.py and the other tests/test_*_oracle.py files.
 """
 from __future__ import annotations
 

@@ -6,7 +6,7 @@
 * Adam's first step (|dP| = lr wherever the gradient is not tiny);
 * the fused wave-cooperative inference against the oracle forward pass;
 * training on samples the solver harvests from its own wall cells, and the
-  trained network serving the wall model (Newton as fallback and judge);
+  trained network serving the wall model (Newton as fallback and arbiter);
 * what the network gains over the Newton solve without it, measured against
   the exact root of Reichardt's law (one Newton iteration saved from the
   reference's analytic start, which already converges in two to three).
@@ -222,12 +222,12 @@ def test_train_on_harvest_and_serve_in_solver(gpu):
     r = s.wall_model_readout()
     say(f"  wall cells {r.cells}, served by the network {r.mlp_cells}")
     assert r.mode == 2 and r.mlp_cells > 0.8 * r.cells
-    judge = s.wall_harvest()  # Newton solution seeded by the network
-    live = judge[:, 3] > 0
-    net = mlp.infer(prm, judge[live])
+    newton = s.wall_harvest()  # Newton solution seeded by the network
+    live = newton[:, 3] > 0
+    net = mlp.infer(prm, newton[live])
     ok = net > 0
-    e = np.abs(net[ok] - judge[live][ok, 3]) / judge[live][ok, 3]
-    say(f"  network vs Newton judge: median {np.median(e):.2e} max {e.max():.2e}")
+    e = np.abs(net[ok] - newton[live][ok, 3]) / newton[live][ok, 3]
+    say(f"  network vs Newton: median {np.median(e):.2e} max {e.max():.2e}")
     assert np.median(e) < 5e-3 and e.max() < 5e-2
     s.step_n(200)
     s.synchronize()

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
 same inputs, plus size-independent properties at the full workload size.
 
-Tolerances (north-star bar): FP32 fields within 1e-5 relative L2 of the FP32
+Tolerances: FP32 fields within 1e-5 relative L2 of the FP32
 oracle after the stated step counts; FP64 storage within 1e-12; FP16 / BF16
 shifted storage stated against the FP64 run of the same case.
 """
