@@ -177,10 +177,30 @@ __device__ inline bool shellCell(long long t, const RArgs& r, int& x, int& y, in
 // Coarse-to-fine fill. FULL: every non-solid fine cell, time levels both the
 // live coarse array (seeding). Otherwise: the Interface shell, t from the
 // band snapshot and t+1 from the live coarse array.
+//
+// Which shell populations are ever read: a level's step pulls f*_q of cell x
+// only into x + c_q, and only cells inside the shell collide. The band
+// snapshot of a nested level reads its parent 3+ cells inside the shell, the
+// restriction reads children from fine x, y >= 2 on, and q-LIBB reads only
+// links whose target is Solid. So the shell fill stores f*_q(x) only when
+// x + c_q lies inside the shell: the 5 inward populations of the inner ring
+// (x or y = 1, n - 2), nothing of the outer ring (x or y = 0, n - 1). The
+// other shell slots keep older values; a download shows Interface cells as
+// NaN either way.
 // ---------------------------------------------------------------------------
+__device__ inline bool insideShell(int x, int y, const RArgs& r) {
+    return x >= 2 && y >= 2 && x < r.fnx - 2 && y < r.fny - 2;
+}
+
+// Outer ring of the shell: no population of these cells reaches a collided cell.
+__device__ inline bool outerRing(int x, int y, const RArgs& r) {
+    return x == 0 || y == 0 || x == r.fnx - 1 || y == r.fny - 1;
+}
+
 // Moments, fneq projection and rescale of an interpolated shell cell, written
-// into the fine slots of layout fLayout.
-template <class S>
+// into the fine slots of layout fLayout (SHELL: only the populations that
+// stream into the shell's interior).
+template <class S, bool SHELL>
 __device__ inline void fillFinish(void* ff, int fLayout, const RArgs& r, int xf, int yf, int zf, const float h[kQ],
                                   float neqScale) {
     const Mom mo = momentsOf(h);
@@ -189,8 +209,10 @@ __device__ inline void fillFinish(void* ff, int fLayout, const RArgs& r, int xf,
     for (int q = 0; q < kQ; ++q) fneq[q] = h[q] - heq(q, mo);
     regularize(fneq);
 #pragma unroll
-    for (int q = 0; q < kQ; ++q)
+    for (int q = 0; q < kQ; ++q) {
+        if (SHELL && !insideShell(xf + kCx[q], yf + kCy[q], r)) continue;
         writeH<S>(ff, r.fn, fLayout, q, xf, yf, zf, r.fnx, r.fny, r.fnz, heq(q, mo) + neqScale * fneq[q]);
+    }
 }
 
 template <class S, bool FULL>
@@ -218,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
         xf = static_cast<int>(rem - static_cast<long long>(yf) * r.fnx);
         if ((fkind[t] & kKindFlagMask) == kSolid) return;
     } else {
-        if (!shellCell(t, r, xf, yf, zf)) return;
+        if (!shellCell(t, r, xf, yf, zf) || outerRing(xf, yf, r)) return;
         if (cacheMode == 2) {
             if (cache[38 * ns + t] == 0.0f) return;
 #pragma unroll
@@ -226,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
                 const float s0 = cache[(2 * q) * ns + t], s1 = cache[(2 * q + 1) * ns + t];
                 h[q] = s0 + timeWeight * (s1 - s0);
             }
-            fillFinish<S>(ff, fLayout, r, xf, yf, zf, h, neqScale);
+            fillFinish<S, true>(ff, fLayout, r, xf, yf, zf, h, neqScale);
             return;
         }
         if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) {
@@ -293,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
         h[q] = s0 + timeWeight * (s1 - s0);
     }
     if (!FULL && cacheMode == 1) cache[38 * ns + t] = 1.0f;
-    fillFinish<S>(ff, fLayout, r, xf, yf, zf, h, neqScale);
+    fillFinish<S, !FULL>(ff, fLayout, r, xf, yf, zf, h, neqScale);
 }
 
 // Shell interpolation for factor 2, grouped by shared corners. The fine
@@ -332,7 +354,7 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
         for (int d = 0; d < 2; ++d) {
             mx[d] = 2 * kx - 1 + d;
             my[d] = y;
-            ok[d] = mx[d] >= 0 && mx[d] < r.fnx;
+            ok[d] = mx[d] >= 0 && mx[d] < r.fnx && !outerRing(mx[d], y, r);
             off[d] = (row < 2 ? static_cast<long long>(row) * r.fnx
                               : 2LL * r.fnx + static_cast<long long>(row - 2) * r.fnx) + mx[d];
         }
@@ -343,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
         for (int d = 0; d < 2; ++d) {
             mx[d] = x;
             my[d] = 2 * ky - 1 + d;
-            ok[d] = my[d] >= 2 && my[d] < r.fny - 2;
+            ok[d] = my[d] >= 2 && my[d] < r.fny - 2 && !outerRing(x, my[d], r);
             off[d] = 4LL * r.fnx + 4LL * (my[d] - 2) + side;
         }
     }

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 #include <algorithm>
 
@@ -654,6 +655,14 @@ void cappedMoments(const T fi[kQ], T& rho, T& ux, T& uy, T& uz) {
 // populations trilinearly interpolated at their centre (solid corners dropped
 // and weights renormalised), blended between coarse t (cT0) and t+1 (cT1),
 // re-split into feq + regularised fneq, fneq rescaled by neqScale.
+//
+// Test switch (orc_set_shell_poison): when set, the shell fill stores NaN in
+// every population f_q of a shell cell x whose target x + c_q lies outside the
+// shell's interior (x, y in [2, n - 2)). A run that stays finite and bit-
+// identical with it set shows that the next step, a nested level's band and
+// the restriction read only the populations that stream into the interior.
+static int g_shellPoison = 0;
+
 template <class T>
 void coarseToFineFill(const T* cT0, const T* cT1, const uint8_t* cFlags, const Grid& cg,
                       const uint8_t* fFlags, T* ff, const Grid& fg, const PBox& box, int m,
@@ -708,7 +717,14 @@ void coarseToFineFill(const T* cT0, const T* cT1, const uint8_t* cFlags, const G
         equilibrium19<T>(rho, ux, uy, uz, feq);
         for (int q = 0; q < kQ; ++q) fneq[q] = fi[q] - feq[q];
         regularizeFneq(fneq);
-        for (int q = 0; q < kQ; ++q) ff[(long long)q * fn + cell] = feq[q] + neqScale * fneq[q];
+        for (int q = 0; q < kQ; ++q) {
+            T v = feq[q] + neqScale * fneq[q];
+            if (!fullVolume && g_shellPoison) {
+                const int tx = xf + kCx[q], ty = yf + kCy[q];
+                if (tx < 2 || ty < 2 || tx >= fg.nx - 2 || ty >= fg.ny - 2) v = std::numeric_limits<T>::quiet_NaN();
+            }
+            ff[(long long)q * fn + cell] = v;
+        }
     }
 }
 
@@ -1193,6 +1209,8 @@ void orc_run_stored_f64(double* fA, double* fB, const uint8_t* flags, int nx, in
 
 // ---- refinement wrappers ----
 void orc_regularize_fneq_f64(double* fneq) { regularizeFneq<double>(fneq); }
+
+void orc_set_shell_poison(int on) { g_shellPoison = on ? 1 : 0; }
 
 void orc_fine_fill_f64(const double* cT0, const double* cT1, const uint8_t* cFlags, int cnx,
                        int cny, int cnz, const uint8_t* fFlags, double* ff, int fnx, int fny,

@@ -79,6 +79,7 @@ def lib():
         L.orc_f16c_decode.restype = C.c_float
         L.orc_f16c_decode.argtypes = [C.c_uint16]
         L.orc_regularize_fneq_f64.argtypes = [_dp]
+        L.orc_set_shell_poison.argtypes = [C.c_int]
         i6 = [C.c_int] * 6
         L.orc_fine_fill_f64.argtypes = [_dp, _dp, _u8p, C.c_int, C.c_int, C.c_int, _u8p, _dp,
                                         C.c_int, C.c_int, C.c_int] + [C.c_int] * 5 + \
@@ -403,6 +404,12 @@ def fine_fill(c_t0, c_t1, cflags, cdims, fflags, ff, box, m, time_weight, neq_sc
     fn(_p(a0, pt), _p(a1, pt), _p(np.ascontiguousarray(cflags, np.uint8), _u8p), *cdims,
        _p(np.ascontiguousarray(fflags, np.uint8), _u8p), _p(ff, pt), *fd, *box, m, time_weight,
        neq_scale, int(full_volume))
+
+
+def set_shell_poison(on: bool) -> None:
+    """Test switch of the shell fill: NaN into every shell population whose
+    target cell lies outside the shell's interior (see lbm_oracle.cpp)."""
+    lib().orc_set_shell_poison(int(on))
 
 
 def fine_restrict(ff, fflags, cf, cflags, cdims, box, m, neq_scale_inv, macro=False):

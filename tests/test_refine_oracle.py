@@ -332,3 +332,49 @@ def test_refined_run_qlinks_wiring():
     e_c, e_f = f0.copy(), g0.copy()
     O.run_refined(e_c, e_f, flags, fine, dims, box, m, taus, uins, fine_q=qfield(fine, fd))
     assert not np.array_equal(e_f, a_f)
+
+
+def test_shell_fill_reads_only_inward_populations():
+    """The shell fill's read set. Each level's step pulls f_q of a shell cell
+    x only into x + c_q, so only the populations whose target lies inside the
+    shell's interior can matter; the HIP fill stores only those. With the
+    oracle's poison switch every other shell population is NaN after each
+    fill. Two- and three-level runs over a foil (solid cells in every level,
+    wall-adjacent fluid, restriction, the finer level's band from the fine
+    grid) must stay finite and equal the unpoisoned runs bit for bit. A fill
+    that dropped one inward population, or a step, band or restriction that
+    read an outward one, would fail here."""
+    from foil_cfd_amd import cases
+    case = cases.naca_tunnel(96, 48, 4, 24, aoa=8.0, cs=0.1, u_lat=0.05, tau=0.56)
+    box, fine = cases.refinement_patch(case)
+    fd = orc.fine_dims(box, case.dims, 2)
+    gbox = (6, max(3, fd[1] // 2 - 10), min(fd[0] - 3, 6 + 30), min(fd[1] - 3, fd[1] // 2 + 10))
+    finer = cases.finer_patch(case, box, 2, gbox, 2)
+    gd = orc.fine_dims(gbox, fd, 2)
+    assert (fine == fc.SOLID).any() and (finer == fc.SOLID).any()
+    rng = np.random.default_rng(3)
+    c0 = orc.init_equilibrium(case.flags, case.dims, 0.05, f64=True)
+    c0 += 1e-4 * rng.standard_normal(c0.size)
+    f0 = orc.seed_fine(c0, case.flags, case.dims, fine, box, 2, 0.56)
+    g0 = orc.seed_fine(f0, fine, fd, finer, gbox, 2, orc.fine_tau_for(0.56, 2))
+    taus, uins = np.full(3, 0.56), np.full(3, 0.05)
+    runs = {}
+    try:
+        for poison in (False, True):
+            orc.set_shell_poison(poison)
+            c, f = c0.copy(), f0.copy()
+            orc.run_refined(c, f, case.flags, fine, case.dims, box, 2, taus, uins, cs=0.1)
+            c3, f3, g3 = c0.copy(), f0.copy(), g0.copy()
+            orc.run_refined3(c3, f3, g3, case.flags, fine, finer, case.dims, box, 2, gbox, 2,
+                             taus, uins, cs=0.1)
+            runs[poison] = (c, f, c3, f3, g3)
+    finally:
+        orc.set_shell_poison(False)
+    masks = [case.flags, fine, case.flags, fine, finer]
+    for a, b, fl in zip(runs[False], runs[True], masks):
+        live = (fl == fc.FLUID)
+        x, y = a.reshape(Q, -1)[:, live], b.reshape(Q, -1)[:, live]
+        assert np.isfinite(y).all()
+        assert np.array_equal(x, y)
+    # the switch is live: the poisoned fine array holds NaN in its shell slots
+    assert np.isnan(runs[True][1].reshape(Q, -1)[:, fine == fc.INTERFACE]).any()
