@@ -348,8 +348,8 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16c,f32")
-    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe", "stage"],
-                    help="step kernel: auto (library default), single, pair, pipe or stage")
+    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe", "stage", "buf"],
+                    help="step kernel: auto (library default), single, pair, pipe, stage or buf")
     ap.add_argument("--coarse-only", action="store_true",
                     help="bare coarse wing (no refinement patch, wall model or particles)")
     ap.add_argument("--polar", action="store_true",

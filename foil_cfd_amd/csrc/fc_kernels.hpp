@@ -21,8 +21,11 @@ namespace fc {
 // kStepPipe runs the software-pipelined persistent kernel (one cell per lane,
 // next task's loads in flight during the collision; arrays under 2 GiB).
 // kStepStage runs the LDS-staged persistent kernel (next task's loads staged
-// through direct-to-LDS buffer loads; arrays under 2 GiB).
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3, kStepStage = 4 };
+// through direct-to-LDS buffer loads; arrays under 2 GiB). kStepBuf runs the
+// one-cell kernel with buffer addressing (arrays under 2 GiB); the default
+// one-cell kernel uses 64-bit addressing, measured faster at 16 bits.
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3, kStepStage = 4,
+                         kStepBuf = 5 };
 
 struct StepArgs {
     float tau0 = 0.6f;

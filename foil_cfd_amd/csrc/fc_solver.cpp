@@ -336,7 +336,8 @@ public:
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
     int setStepVariant(int v) {
-        if (v != kStepAuto && v != kStepSingle && v != kStepPair && v != kStepPipe && v != kStepStage)
+        if (v != kStepAuto && v != kStepSingle && v != kStepPair && v != kStepPipe && v != kStepStage &&
+            v != kStepBuf)
             return fail(FC_ERR_INVALID, "unknown step variant");
         stepVariant_ = v;
         return FC_OK;

@@ -390,9 +390,11 @@ class LBMSolver:
 
     def set_step_variant(self, variant: str):
         """"auto" (library default), "single" (one cell per lane), "pair"
-        (two x-adjacent cells per lane where the grid allows it) or "pipe"
-        (persistent, software-pipelined one-cell-per-lane kernel)."""
-        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3, "stage": 4}[variant]
+        (two x-adjacent cells per lane where the grid allows it), "pipe"
+        (persistent, software-pipelined one-cell-per-lane kernel), "stage"
+        (persistent, LDS-staged) or "buf" (one cell per lane with buffer
+        addressing)."""
+        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3, "stage": 4, "buf": 5}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):

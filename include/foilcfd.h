@@ -196,14 +196,19 @@ int fc_set_smagorinsky_cs(fc_solver* s, float cs);
 
 /* Step kernel selection (no reference counterpart: the reference has one
  * kernel). FC_STEP_AUTO (default) is the library's choice, currently one
- * cell per lane (the fastest measured on MI355X); FC_STEP_SINGLE forces it;
- * FC_STEP_PAIR runs two x-adjacent cells per lane with packed population
- * accesses where the grid allows it (even nx, x / y not periodic, population
- * array under 2 GiB); FC_STEP_PIPE runs a persistent, software-pipelined
- * kernel that keeps the next row segment's population loads in flight while
- * it collides the current one (population array under 2 GiB). All give the
- * same cell update. */
-enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2, FC_STEP_PIPE = 3 };
+ * cell per lane with 64-bit population addressing (the fastest measured on
+ * MI355X); FC_STEP_SINGLE forces it; FC_STEP_PAIR runs two x-adjacent cells
+ * per lane with packed population accesses where the grid allows it (even
+ * nx, x / y not periodic, population array under 2 GiB); FC_STEP_PIPE runs a
+ * persistent, software-pipelined kernel that keeps the next row segment's
+ * population loads in flight while it collides the current one;
+ * FC_STEP_STAGE runs a persistent kernel that stages the next row segment's
+ * populations in LDS; FC_STEP_BUF runs the one-cell kernel with 32-bit buffer
+ * addressing and eight waves per SIMD. PIPE, STAGE and BUF need a population
+ * array under 2 GiB and otherwise fall back to the default. All give the same
+ * cell update, bit for bit. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2, FC_STEP_PIPE = 3, FC_STEP_STAGE = 4,
+       FC_STEP_BUF = 5 };
 int fc_set_step_variant(fc_solver* s, int variant);
 
 /* ---- stepping ------------------------------------------------------------ */

@@ -305,13 +305,14 @@ def test_pair_kernel_matches_single_cell_kernel(gpu, storage):
         assert rel_l2(a, b) < 1e-5
 
 
-@pytest.mark.parametrize("variant", ["pipe", "stage"])
+@pytest.mark.parametrize("variant", ["pipe", "stage", "buf"])
 @pytest.mark.parametrize("storage", ["f16c", "f16", "bf16", "f32"])
 @pytest.mark.parametrize("flow", ["tunnel", "tunnel+wall", "periodic"])
 def test_pipelined_kernel_matches_single_cell_kernel(gpu, storage, flow, variant):
-    """The persistent software-pipelined step (register prefetch, "pipe") and
-    the LDS-staged persistent step ("stage") run the same cell update as the
-    one-cell-per-lane step: populations and macro fields bit-identical after
+    """The persistent software-pipelined step (register prefetch, "pipe"), the
+    LDS-staged persistent step ("stage") and the buffer-addressed one-cell
+    step ("buf") run the same cell update as the default one-cell-per-lane
+    step: populations and macro fields bit-identical after
     an odd number of steps (both AA layouts), on a tunnel with boundary
     waves, with the wall model's slip term, and on the periodic box (the step
     kernels place every fused multiply-add explicitly, so variants round
