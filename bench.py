@@ -17,6 +17,7 @@ Prints ONE JSON line. value = FP16C-storage GLUPS (whole job, one GPU).
 particles).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -179,6 +180,58 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
         f"kernel {res['kernel_ms']:.3f} ms ({res['kernel_gbs']:.0f} GB/s), coarse step kernel "
         f"{per_level['coarse']['kernel_ms']:.3f} ms ({per_level['coarse']['kernel_gbs']:.0f} GB/s)")
     return res
+
+
+def accuracy_run(args):
+    """Time-averaged Cl / Cd of the full default-tunnel configuration for one
+    storage format (the reference's protocol: one momentum-exchange sample per
+    call of --per-call steps folded into the 256-sample ring, the readout
+    valid after two flow-throughs, the average over the samples after the
+    first one; lbm_solver.h:25, lbm_solver.cpp:633-654, :1414-1509). Every
+    raw sample is kept as well, so other windows can be averaged afterwards.
+    Writes the record to --out after every progress line."""
+    import foil_cfd_amd as fc
+    from foil_cfd_amd import cases
+    fc.load()
+    storage = args.storages.split(",")[0].strip()
+    case = cases.config3_wing(args.aoa)
+    s, box, info = full_setup(case, storage, args.aoa, wall=args.wall)
+    ft = fc.flow_through_steps(case.scaling, case.dims[0])
+    total = int(round(args.flow_throughs * ft / args.per_call)) * args.per_call
+    span, fscale = info["fine_dims"][2], info["fine_scaling"]
+    rec = {"workload": f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_imem_patch2x_particles1M",
+           "storage": storage, "aoa": args.aoa, "wall_model": args.wall, "per_call": args.per_call,
+           "flow_through_steps": ft, "steps_planned": total, "patch_box": list(box),
+           "fine_dims": list(info["fine_dims"]), "forces_from_fine": info["forces_from_fine"],
+           "tau": case.scaling.tau, "fine_tau": fscale.tau, "samples": []}
+    t0 = time.perf_counter()
+    seed = 1
+    done = 0
+    status = "ok"
+    while done < total:
+        s.step_n(args.per_call)
+        s.particles_advect(float(args.per_call), seed)
+        seed += 1
+        done += args.per_call
+        s.synchronize()
+        F, at = s.force_raw()
+        cl = fc.force_to_coefficient(fscale if info["forces_from_fine"] else case.scaling, float(F[1]), span)
+        cd = fc.force_to_coefficient(fscale if info["forces_from_fine"] else case.scaling, float(F[0]), span)
+        rec["samples"].append([int(at), float(F[0]), float(F[1]), cl, cd])
+        if s.nan_detected():
+            status = f"watchdog verdict {s.nan_kind()} at step {s.step_count()}"
+        if done % (10 * args.per_call) == 0 or done >= total or status != "ok":
+            fr = s.forces()
+            rec.update({"steps_done": done, "status": status, "wall_s": round(time.perf_counter() - t0, 2),
+                        "readout": dataclasses.asdict(fr)})
+            with open(args.out, "w") as fh:
+                json.dump(rec, fh)
+            print(f"[accuracy] {storage} aoa {args.aoa:g}: step {done}/{total} ({done / ft:.2f} FT) "
+                  f"Cl {cl:.4f} Cd {cd:.4f} avg Cl {fr.cl_avg:.4f} Cd {fr.cd_avg:.4f} valid {fr.valid} "
+                  f"{time.perf_counter() - t0:.0f} s {status}", flush=True)
+        if status != "ok":
+            break
+    s.close()
 
 
 def cpu_baseline(case, seconds=12.0):
@@ -348,8 +401,8 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16c,f32")
-    ap.add_argument("--variant", default="auto", choices=["auto", "single", "pair", "pipe", "stage", "buf"],
-                    help="step kernel: auto (library default), single, pair, pipe, stage or buf")
+    ap.add_argument("--variant", default="auto", choices=["auto", "single", "buf"],
+                    help="step kernel: auto (library default), single or buf")
     ap.add_argument("--coarse-only", action="store_true",
                     help="bare coarse wing (no refinement patch, wall model or particles)")
     ap.add_argument("--polar", action="store_true",
@@ -358,6 +411,12 @@ def main():
     ap.add_argument("--polar-wall", default="mlp", choices=["off", "newton", "mlp"])
     ap.add_argument("--polar-trial", type=int, default=10)
     ap.add_argument("--polar-solo-steps", type=int, default=50)
+    ap.add_argument("--accuracy", action="store_true",
+                    help="time-averaged Cl/Cd run of the full configuration for the first of --storages "
+                         "(record written to --out)")
+    ap.add_argument("--flow-throughs", type=float, default=4.0)
+    ap.add_argument("--wall", default="newton", choices=["off", "newton", "mlp"])
+    ap.add_argument("--out", default="accuracy.json")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -372,6 +431,9 @@ def main():
         return
     if args.polar:
         polar_bench(args)
+        return
+    if args.accuracy:
+        accuracy_run(args)
         return
 
     import foil_cfd_amd as fc

@@ -210,6 +210,12 @@ SIGNATURES = {
     "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),
     "fc_force_raw": (C.c_int, [_vp, _dp, _llp]),
     "fc_force_now": (C.c_int, [_vp, _dp]),
+    "fc_force_fold_create": (_vp, []),
+    "fc_force_fold_destroy": (None, [_vp]),
+    "fc_force_fold_clear": (C.c_int, [_vp]),
+    "fc_force_fold_add": (C.c_int, [_vp, C.c_longlong, C.c_float, C.c_float, C.c_double, C.c_double]),
+    "fc_force_fold_readout": (C.c_int, [_vp, C.POINTER(FcScaling), C.c_int, C.c_longlong, C.c_int,
+                                        C.POINTER(FcForceReadout)]),
     "fc_mass": (C.c_int, [_vp, _dp]),
     "fc_read_macro": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
     "fc_read_flags": (C.c_int, [_vp, _u8p]),

@@ -27,15 +27,15 @@ bool generateNACA4(const std::string& digits, int pointsPerSurface, Airfoil& out
         if (reason) *reason = why;
         return false;
     };
-    if (digits.size() != 4) return fail("expected exactly 4 digits, got '" + digits + "'");
+    if (digits.size() != 4) return fail("NACA 4-digit designation must have 4 characters: \"" + digits + "\"");
     for (char ch : digits)
-        if (ch < '0' || ch > '9') return fail("non-digit character in '" + digits + "'");
+        if (ch < '0' || ch > '9') return fail("NACA designation \"" + digits + "\" is not all decimal digits");
     // m = max camber, p = its chordwise position, t = max thickness.
     const float m = static_cast<float>(digits[0] - '0') / 100.0f;
     const float p = static_cast<float>(digits[1] - '0') / 10.0f;
     const float t = static_cast<float>((digits[2] - '0') * 10 + (digits[3] - '0')) / 100.0f;
-    if (t <= 0.0f) return fail("thickness digits are 00 (zero-thickness section)");
-    if (m > 0.0f && p <= 0.0f) return fail("camber position digit is 0 while camber digit is nonzero");
+    if (t <= 0.0f) return fail("NACA thickness of 0 % chord has no section");
+    if (m > 0.0f && p <= 0.0f) return fail("cambered NACA section needs a camber position > 0");
 
     const int n = std::max(pointsPerSurface, 20);
     out.name = "NACA " + digits;

@@ -37,8 +37,6 @@ struct KArgs {
     // (n * m) >> 40 with m = ceil(2^40 / d), exact for n < 2^24, d < 2^16
     // (0 when the grid exceeds those ranges: plain division).
     unsigned long long segMagic, nyMagic;
-    int pipeTasks;  // software-pipelined step: row-segment tasks per XCD slab
-    int pipeWaves;  // software-pipelined step: waves per XCD
     float tau0, lam, pre, uIn;
     int writeMacro;
 };
@@ -142,51 +140,6 @@ __device__ inline bool rowPos(const KArgs& a, RowPos& p) {
     const int gw = lb * kWavesPerBlock + wv;
     if (gw >= a.nwaves) return false;
     return taskPos<PX, PY>(a, gw, p);
-}
-
-// Two-cells-per-lane decomposition: a wave covers 128 cells of a row, lane l
-// the cells x0 = 128 seg + 2 l and x0 + 1 (nx even, x / y not periodic). The
-// kernel arguments carry the pair geometry (segments of 128 cells).
-inline KArgs pairArgs(const KArgs& a) {
-    KArgs b = a;
-    b.nseg = (a.nx + 2 * kWave - 1) / (2 * kWave);
-    b.nwaves = b.nseg * a.ny * a.nz;
-    b.nblocks = (b.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
-    b.chunk = (b.nblocks + kXcds - 1) / kXcds;
-    setDivMagic(b);
-    return b;
-}
-
-__device__ inline bool rowPosPair(const KArgs& a, RowPos& p) {
-    const int hb = blockIdx.x;
-    const int lb = (hb & (kXcds - 1)) * a.chunk + (hb >> 3);
-    if (lb >= a.nblocks) return false;
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int gw = lb * kWavesPerBlock + wv;
-    if (gw >= a.nwaves) return false;
-    const int row = taskDiv(gw, a.nseg, a.segMagic);
-    const int seg = gw - row * a.nseg;
-    const int z = taskDiv(row, a.ny, a.nyMagic);
-    const int y = row - z * a.ny;
-    const int x = seg * (2 * kWave) + 2 * static_cast<int>(threadIdx.x & (kWave - 1));
-    if (x >= a.nx) return false;
-    p.x = x;
-    p.y = y;
-    p.z = z;
-    p.xm = x - 1;
-    p.xp = x + 1;
-    int ys[3] = {y - 1, y, y + 1};
-    p.yOk[0] = ys[0] >= 0;
-    p.yOk[1] = true;
-    p.yOk[2] = ys[2] < a.ny;
-    if (!p.yOk[0]) ys[0] = 0;
-    if (!p.yOk[2]) ys[2] = a.ny - 1;
-    const int zs[3] = {z == 0 ? a.nz - 1 : z - 1, z, z == a.nz - 1 ? 0 : z + 1};
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) p.R[i][j] = (ys[i] + a.ny * zs[j]) * a.nx;
-    return true;
 }
 
 // Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N-1, so
@@ -296,16 +249,6 @@ struct PopBuf {
         else
             return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy));
     }
-    // Direct-to-LDS load of one slot per lane: the value lands zero-extended
-    // in the dword at lds + 4 * lane (the wave's LDS base must be uniform).
-    __device__ void ldLds(int q, int ex, int ey, int ez, __attribute__((address_space(3))) void* lds) const {
-        const int vo = static_cast<int>(vx[ex + 1]);
-        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
-        if constexpr (sizeof(Bits) == 2)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, lds, 2, vo, so, 0, kPopLoadPolicy);
-        else
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, lds, 4, vo, so, 0, kPopLoadPolicy);
-    }
     __device__ void st(int q, int ex, int ey, int ez, C v) const {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
@@ -315,38 +258,6 @@ struct PopBuf {
             __builtin_amdgcn_raw_buffer_store_b32(S::toBits(v), rs, vo, so, kPopStorePolicy);
     }
 
-    // Two consecutive slots (x+ex, x+ex+1) of one direction and row in one
-    // access: 4 bytes for 2-byte formats, 8 bytes for FP32.
-    using U2 = unsigned __attribute__((ext_vector_type(2)));
-    using Pair = std::conditional_t<sizeof(Bits) == 2, unsigned, U2>;
-    __device__ static Bits lo(Pair v) {
-        if constexpr (sizeof(Bits) == 2) return static_cast<Bits>(v & 0xFFFFu);
-        else return v.x;
-    }
-    __device__ static Bits hi(Pair v) {
-        if constexpr (sizeof(Bits) == 2) return static_cast<Bits>(v >> 16);
-        else return v.y;
-    }
-    __device__ static Pair pack(Bits a, Bits b) {
-        if constexpr (sizeof(Bits) == 2) return static_cast<unsigned>(a) | (static_cast<unsigned>(b) << 16);
-        else return U2{a, b};
-    }
-    __device__ Pair ld2(int q, int ex, int ey, int ez) const {
-        const int vo = static_cast<int>(vx[ex + 1]);
-        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
-        if constexpr (sizeof(Bits) == 2)
-            return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy);
-        else
-            return __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, kPopLoadPolicy);
-    }
-    __device__ void st2(int q, int ex, int ey, int ez, Pair v) const {
-        const int vo = static_cast<int>(vx[ex + 1]);
-        const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
-        if constexpr (sizeof(Bits) == 2)
-            __builtin_amdgcn_raw_buffer_store_b32(v, rs, vo, so, kPopStorePolicy);
-        else
-            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, so, kPopStorePolicy);
-    }
 };
 
 // Post-collision population f*_q of cell (x,y,z) in the given layout, as a

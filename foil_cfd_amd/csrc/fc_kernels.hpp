@@ -14,18 +14,14 @@ namespace fc {
 // StepParams, src/sim/lbm_core.cuh:233-241, with Cs turned into the
 // precomputed Smagorinsky prefactor 18*sqrt(2)*Cs^2 exactly as the reference
 // launch wrapper forms it, src/sim/lbm_core.cu:785-787).
-// Step kernel variants: kStepAuto is the library's default (one cell per
-// lane, the fastest measured on MI355X), kStepSingle forces one cell per
-// lane, kStepPair runs two x-adjacent cells per lane with packed accesses
-// where the grid allows it (even nx, non-periodic x / y, array under 2 GiB),
-// kStepPipe runs the software-pipelined persistent kernel (one cell per lane,
-// next task's loads in flight during the collision; arrays under 2 GiB).
-// kStepStage runs the LDS-staged persistent kernel (next task's loads staged
-// through direct-to-LDS buffer loads; arrays under 2 GiB). kStepBuf runs the
-// one-cell kernel with buffer addressing (arrays under 2 GiB); the default
-// one-cell kernel uses 64-bit addressing, measured faster at 16 bits.
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepPair = 2, kStepPipe = 3, kStepStage = 4,
-                         kStepBuf = 5 };
+// Step kernel variants: kStepAuto is the library's default (the one-cell-
+// per-lane kernel with 64-bit population addressing, measured fastest on
+// MI355X; on a factor-2 patch level the last sub-step of a coarse step also
+// writes the restriction records). kStepSingle runs the same one-cell kernel
+// on every sub-step and leaves the records to the separate restriction pass.
+// kStepBuf runs the one-cell kernel with 32-bit buffer addressing (arrays
+// under 2 GiB, the default beyond) and the separate restriction pass.
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepBuf = 2 };
 
 struct StepArgs {
     float tau0 = 0.6f;

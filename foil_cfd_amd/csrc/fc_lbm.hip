@@ -404,180 +404,6 @@ __global__ void __launch_bounds__(kBlock) stepRecordKernel(KArgs a, RecArgs rg) 
 }
 
 // ---------------------------------------------------------------------------
-// Software-pipelined step. The row-segment tasks are split into eight
-// contiguous slabs, one per XCD (workgroups are dealt to XCDs round-robin);
-// wave i of an XCD takes tasks slab + i, slab + i + W, ... (W waves per XCD),
-// so the XCD sweeps its slab front by front. While a wave collides task j it
-// already has the kind bytes and the 19 population loads of task j+1 in
-// flight (raw storage bits, decoded only when used), so the collision's ALU
-// work overlaps the next task's memory latency instead of waiting for other
-// waves to cover it. Every slot an AA step reads is written by the same cell,
-// so prefetching across cells is race-free.
-// ---------------------------------------------------------------------------
-template <class S>
-struct PipeStage {
-    using Bits = typename S::Bits;
-    int t;           // task (wave-uniform); row geometry is recomputed from it
-    Bits B[kQ];      // raw pulled populations
-    std::uint8_t k;  // kind byte
-};
-
-// Row geometry of task t for the loads: lanes past the row end and x-1 / x+1
-// beyond a non-periodic face are clamped onto an in-domain slot (only fluid
-// cells use what they read).
-template <bool PX, bool PY>
-__device__ inline void pipeLoadPos(const KArgs& a, int t, RowPos& q) {
-    if (!taskPos<PX, PY>(a, t, q)) q.x = a.nx - 1;
-    if (!PX) {
-        q.xm = max(q.x - 1, 0);
-        q.xp = min(q.x + 1, a.nx - 1);
-    }
-}
-
-// Issues the kind byte and the 19 pull loads of task t for every lane with no
-// branch around them. Unconditional issue keeps the number of memory
-// operations between a load and its use fixed, so the waits the compiler
-// inserts let the next task's loads stay in flight.
-template <class S, bool FROM_L, bool PX, bool PY>
-__device__ inline void pipeIssue(const KArgs& a, int t, PipeStage<S>& st) {
-    RowPos q;
-    pipeLoadPos<PX, PY>(a, t, q);
-    const PopBuf<S> pop(a, q);
-    st.t = t;
-    st.k = a.kind[nb(q, 0, 0, 0)];
-    staticFor<kQ>([&](auto qc) {
-        using L = PullSlot<FROM_L, decltype(qc)::value>;
-        st.B[decltype(qc)::value] = pop.ldBits(L::dir, L::ex, L::ey, L::ez);
-    });
-}
-
-template <class S, bool FROM_L, bool WM, bool PX, bool PY>
-__device__ inline void pipeCompute(const KArgs& a, PipeStage<S>& st) {
-    using C = typename S::C;
-    RowPos p;
-    if (!taskPos<PX, PY>(a, st.t, p)) return;
-    RowPos q = p;
-    if (!PX) {
-        q.xm = max(q.x - 1, 0);
-        q.xp = min(q.x + 1, a.nx - 1);
-    }
-    const PopBuf<S> pop(a, q);
-    C H[kQ];
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) H[i] = S::fromBits(st.B[i]);
-    cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, p, pop, st.k, H);
-}
-
-template <class S, bool FROM_L, bool WM, bool PX, bool PY>
-__global__ void __launch_bounds__(kBlock) stepKernelPipe(KArgs a) {
-    const int hb = blockIdx.x;
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int W = a.pipeWaves;
-    const int i = (hb >> 3) * kWavesPerBlock + wv;  // wave index within the XCD
-    const int t0 = (hb & (kXcds - 1)) * a.pipeTasks + i;
-    const int tEnd = min((hb & (kXcds - 1)) * a.pipeTasks + a.pipeTasks, a.nwaves);
-    if (i >= W || t0 >= tEnd) return;
-    const int n = (tEnd - t0 + W - 1) / W;  // tasks of this wave: t0 + j W, j < n
-
-    // Two stages with fixed roles (no copies of registers with loads in
-    // flight): while one is collided the other's loads are outstanding.
-    PipeStage<S> A, B;
-    pipeIssue<S, FROM_L, PX, PY>(a, t0, A);
-    int j = 0;
-    for (;;) {
-        if (j + 1 >= n) {
-            pipeCompute<S, FROM_L, WM, PX, PY>(a, A);
-            return;
-        }
-        pipeIssue<S, FROM_L, PX, PY>(a, t0 + (j + 1) * W, B);
-        pipeCompute<S, FROM_L, WM, PX, PY>(a, A);
-        ++j;
-        if (j + 1 >= n) {
-            pipeCompute<S, FROM_L, WM, PX, PY>(a, B);
-            return;
-        }
-        pipeIssue<S, FROM_L, PX, PY>(a, t0 + (j + 1) * W, A);
-        pipeCompute<S, FROM_L, WM, PX, PY>(a, B);
-        ++j;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// LDS-staged persistent step. Same task order as the pipelined kernel, but
-// the next task's 19 pull loads and kind bytes travel by direct-to-LDS buffer
-// loads into the wave's own 5 KiB LDS slot, so the prefetch holds no VGPRs
-// and eight waves still fit per SIMD (20 KiB per workgroup, 160 KiB per CU).
-// Per task: wait for the staged loads, copy them into registers, issue the
-// next task's loads into the same slot, then collide and store.
-// ---------------------------------------------------------------------------
-inline constexpr int kStageRows = kQ + 1;  // 19 populations + the kind bytes
-using LdsPtr = __attribute__((address_space(3))) void*;
-
-// s_waitcnt immediates of the gfx9 encoding: vmcnt(0) with the other
-// counters left open, and lgkmcnt(0) with vmcnt / expcnt left open.
-inline constexpr int kWaitVm0 = 0x0F70;
-inline constexpr int kWaitLgkm0 = 0xC07F;
-
-template <class S, bool FROM_L, bool PX, bool PY>
-__device__ inline void stageIssue(const KArgs& a, int t, unsigned* slot) {
-    RowPos q;
-    pipeLoadPos<PX, PY>(a, t, q);
-    const PopBuf<S> pop(a, q);
-    staticFor<kQ>([&](auto qc) {
-        using L = PullSlot<FROM_L, decltype(qc)::value>;
-        pop.ldLds(L::dir, L::ex, L::ey, L::ez, (LdsPtr)(slot + decltype(qc)::value * kWave));
-    });
-    // kind bytes through the same path: an untyped byte buffer over the array
-    const __amdgpu_buffer_rsrc_t kr =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<std::uint8_t*>(a.kind), 0, 0x7FFFFFFF, 0x00020000);
-    const unsigned rowOff = static_cast<unsigned>(q.R[1][1]);
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(kr, (LdsPtr)(slot + kQ * kWave), 1, static_cast<int>(q.x),
-                                             static_cast<int>(rowOff), 0, 0);
-}
-
-template <class S, bool FROM_L, bool WM, bool PX, bool PY>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
-stepKernelStage(KArgs a) {
-    using C = typename S::C;
-    using Bits = typename S::Bits;
-    __shared__ unsigned stage[kWavesPerBlock][kStageRows * kWave];
-    const int hb = blockIdx.x;
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int W = a.pipeWaves;
-    const int i = (hb >> 3) * kWavesPerBlock + wv;
-    const int t0 = (hb & (kXcds - 1)) * a.pipeTasks + i;
-    const int tEnd = min((hb & (kXcds - 1)) * a.pipeTasks + a.pipeTasks, a.nwaves);
-    if (i >= W || t0 >= tEnd) return;
-    const int n = (tEnd - t0 + W - 1) / W;
-    unsigned* slot = stage[wv];
-    stageIssue<S, FROM_L, PX, PY>(a, t0, slot);
-    for (int j = 0; j < n; ++j) {
-        const int t = t0 + j * W;
-        __builtin_amdgcn_s_waitcnt(kWaitVm0);
-        Bits B[kQ];
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) B[q] = static_cast<Bits>(slot[q * kWave + lane]);
-        const std::uint8_t k = static_cast<std::uint8_t>(slot[kQ * kWave + lane]);
-        // the copies must have left LDS before the next loads overwrite it
-        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-        if (j + 1 < n) stageIssue<S, FROM_L, PX, PY>(a, t + W, slot);
-        RowPos p;
-        if (!taskPos<PX, PY>(a, t, p)) continue;
-        RowPos q = p;
-        if (!PX) {
-            q.xm = max(q.x - 1, 0);
-            q.xp = min(q.x + 1, a.nx - 1);
-        }
-        const PopBuf<S> pop(a, q);
-        C H[kQ];
-#pragma unroll
-        for (int r = 0; r < kQ; ++r) H[r] = S::fromBits(B[r]);
-        cellBody<S, PopBuf, FROM_L, WM, PX, PY>(a, p, pop, k, H);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Batched step of a group of independent lattices (e.g. the cases of a polar
 // sweep, each with its own grid size, layout parity, relaxation time and
 // inlet speed) in one launch. The row-segment tasks of all members are
@@ -648,72 +474,6 @@ stepKernelGroupMixed(const GroupEntry* __restrict__ tab, GroupLaunch g) {
     } else {
         if (e.wm) stepCell<S, PopBuf, false, true, false, false>(a, p);
         else stepCell<S, PopBuf, false, false, false, false>(a, p);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// The same step with two x-adjacent cells per lane (a wave covers 128 cells of
-// a row; nx even, x and y not periodic). A wave whose 128 cells are all plain
-// bulk fluid (kind byte 0) moves both cells' populations of a direction with
-// one 4-byte (FP16 / BF16) or 8-byte (FP32) buffer access: twice the bytes
-// per memory instruction, so twice the bytes in flight per wave. The slots of
-// x-shifted directions start at odd element offsets; the accesses are
-// unaligned and rely on the unaligned memory mode ROCm programs on gfx9
-// parts. Any other wave (boundary, special links, outlet feed, wall model)
-// runs the per-cell path for its two cells in turn.
-// ---------------------------------------------------------------------------
-template <class S, bool FROM_L, bool WM>
-__global__ void __launch_bounds__(kBlock) stepPairKernel(KArgs a) {
-    using C = typename S::C;
-    using Bits = typename S::Bits;
-    RowPos p;
-    if (!rowPosPair(a, p)) return;
-    const unsigned c0 = nb(p, 0, 0, 0);
-    const unsigned short kk = *reinterpret_cast<const unsigned short*>(a.kind + c0);
-    const bool fast = __builtin_amdgcn_read_exec() == ~0ull && __builtin_amdgcn_ballot_w64(kk != 0) == 0ull;
-    if (!fast) {
-        stepCell<S, PopBuf, FROM_L, WM, false, false>(a, p);
-        RowPos p1 = p;
-        p1.x = p.x + 1;
-        p1.xm = p.x;
-        p1.xp = p.x + 2;
-        stepCell<S, PopBuf, FROM_L, WM, false, false>(a, p1);
-        return;
-    }
-    const PopBuf<S> pop(a, p);
-    using P2 = typename PopBuf<S>::Pair;
-    P2 P[kQ];
-    staticFor<kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if (FROM_L)
-            P[q] = pop.ld2(kOpp[q], -kCx[q], -kCy[q], -kCz[q]);
-        else
-            P[q] = pop.ld2(q, 0, 0, 0);
-    });
-    const C K = C(S::K);
-    C H[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(PopBuf<S>::lo(P[q]));
-    const Macro<C> m0 = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
-    Bits O[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) O[q] = S::toBits(H[q]);
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) H[q] = S::fromBits(PopBuf<S>::hi(P[q]));
-    const Macro<C> m1 = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
-    staticFor<kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const P2 v = PopBuf<S>::pack(O[q], S::toBits(H[q]));
-        if (FROM_L)
-            pop.st2(q, kCx[q], kCy[q], kCz[q], v);
-        else
-            pop.st2(kOpp[q], 0, 0, 0, v);
-    });
-    if (a.writeMacro) {
-        *reinterpret_cast<float2*>(a.rho + c0) = make_float2(static_cast<float>(m0.rho), static_cast<float>(m1.rho));
-        *reinterpret_cast<float2*>(a.u + c0) = make_float2(static_cast<float>(m0.ux), static_cast<float>(m1.ux));
-        *reinterpret_cast<float2*>(a.v + c0) = make_float2(static_cast<float>(m0.uy), static_cast<float>(m1.uy));
-        *reinterpret_cast<float2*>(a.w + c0) = make_float2(static_cast<float>(m0.uz), static_cast<float>(m1.uz));
     }
 }
 
@@ -1146,97 +906,19 @@ struct BufStep {
     };
 };
 
-// The pipelined kernel runs as a persistent grid: per XCD as many workgroups
-// as its CUs hold at once (occupancy of the instantiation, queried once).
-template <class S>
-struct PipeStep {
-    template <bool FROM_L, bool WM, bool PX, bool PY>
-    struct K {
-        static void launch(dim3, dim3 b, hipStream_t st, KArgs a) {
-            static int occ = 0;
-            if (occ == 0) {
-                int o = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &o, reinterpret_cast<const void*>(&stepKernelPipe<S, FROM_L, WM, PX, PY>), kBlock, 0) !=
-                    hipSuccess)
-                    (void)hipGetLastError();
-                occ = o > 0 ? o : 1;
-            }
-            int dev = 0, cus = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
-                (void)hipGetLastError();
-                cus = 256;
-            }
-            const int perXcd = (cus + kXcds - 1) / kXcds;
-            a.pipeTasks = (a.nwaves + kXcds - 1) / kXcds;
-            const int blocksPerXcd = min((a.pipeTasks + kWavesPerBlock - 1) / kWavesPerBlock, perXcd * occ);
-            a.pipeWaves = blocksPerXcd * kWavesPerBlock;
-            stepKernelPipe<S, FROM_L, WM, PX, PY><<<dim3(blocksPerXcd * kXcds), b, 0, st>>>(a);
-        }
-    };
-};
-
-template <class S>
-struct StageStep {
-    template <bool FROM_L, bool WM, bool PX, bool PY>
-    struct K {
-        static void launch(dim3, dim3 b, hipStream_t st, KArgs a) {
-            static int occ = 0;
-            if (occ == 0) {
-                int o = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &o, reinterpret_cast<const void*>(&stepKernelStage<S, FROM_L, WM, PX, PY>), kBlock, 0) !=
-                    hipSuccess)
-                    (void)hipGetLastError();
-                occ = o > 0 ? o : 1;
-            }
-            int dev = 0, cus = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
-                (void)hipGetLastError();
-                cus = 256;
-            }
-            const int perXcd = (cus + kXcds - 1) / kXcds;
-            a.pipeTasks = (a.nwaves + kXcds - 1) / kXcds;
-            const int blocksPerXcd = min((a.pipeTasks + kWavesPerBlock - 1) / kWavesPerBlock, perXcd * occ);
-            a.pipeWaves = blocksPerXcd * kWavesPerBlock;
-            stepKernelStage<S, FROM_L, WM, PX, PY><<<dim3(blocksPerXcd * kXcds), b, 0, st>>>(a);
-        }
-    };
-};
-
 // The default one-cell kernel uses the flat-pointer accessor: on the 23.6 M-
 // cell wing it ran 10 % faster than the buffer-addressed kernel at 16 bits
-// (0.485 vs 0.539 ms) and level at FP32. The buffer-addressed variants (buf,
-// pair, pipe, stage) need every population byte offset in 31 bits (FP16 /
-// BF16 up to 56 M cells, FP32 up to 28 M) and fall back to the default
-// beyond; FP64 always takes the default. The two-cells-per-lane kernel needs
-// an even nx and non-periodic x / y.
+// (0.485 vs 0.539 ms) and level at FP32. The buffer-addressed kernel (kStepBuf)
+// needs every population byte offset in 31 bits (FP16 / BF16 up to 56 M
+// cells, FP32 up to 28 M) and falls back to the default beyond; FP64 always
+// takes the default.
 template <class S>
 int dispatchStep(const KArgs& a, const LatticeView& lat, int layout, int variant, hipStream_t st) {
     const unsigned long long bytes =
         static_cast<unsigned long long>(kQ) * static_cast<unsigned long long>(a.ncells) * sizeof(typename S::T);
     if constexpr (!std::is_same_v<S, StoreF64>) {
-        if (bytes < (1ull << 31)) {
-            const bool pairOk = (a.nx % 2 == 0) && !lat.periodicX && !lat.periodicY;
-            if (variant == kStepPair && pairOk) {
-                const KArgs a2 = pairArgs(a);
-                const bool wm = lat.uwx && lat.uwy && lat.uwz;
-                const dim3 g(rowGrid(a2)), b(kBlock);
-                if (layout == kLayoutL) {
-                    if (wm) stepPairKernel<S, true, true><<<g, b, 0, st>>>(a2);
-                    else stepPairKernel<S, true, false><<<g, b, 0, st>>>(a2);
-                } else {
-                    if (wm) stepPairKernel<S, false, true><<<g, b, 0, st>>>(a2);
-                    else stepPairKernel<S, false, false><<<g, b, 0, st>>>(a2);
-                }
-                return lastError();
-            }
-            if (variant == kStepPipe) return dispatchStepK<PipeStep<S>::template K>(a, lat, layout, st);
-            if (variant == kStepStage) return dispatchStepK<StageStep<S>::template K>(a, lat, layout, st);
-            if (variant == kStepBuf) return dispatchStepK<BufStep<S>::template K>(a, lat, layout, st);
-        }
+        if (variant == kStepBuf && bytes < (1ull << 31))
+            return dispatchStepK<BufStep<S>::template K>(a, lat, layout, st);
     }
     return dispatchStepK<PtrStep<S>::template K>(a, lat, layout, st);
 }
@@ -1278,17 +960,6 @@ struct StepRecordOp {
         const bool wm = lat.uwx && lat.uwy && lat.uwz;
         const bool fromL = layout == kLayoutL;
         const dim3 g(rowGrid(a)), b(kBlock);
-        const unsigned long long bytes =
-            static_cast<unsigned long long>(kQ) * static_cast<unsigned long long>(a.ncells) * sizeof(typename S::T);
-        if constexpr (!std::is_same_v<S, StoreF64>) {
-            if (bytes < (1ull << 31)) {
-                if (fromL && wm) stepRecordKernel<S, PopBuf, true, true><<<g, b, 0, st>>>(a, rg);
-                else if (fromL) stepRecordKernel<S, PopBuf, true, false><<<g, b, 0, st>>>(a, rg);
-                else if (wm) stepRecordKernel<S, PopBuf, false, true><<<g, b, 0, st>>>(a, rg);
-                else stepRecordKernel<S, PopBuf, false, false><<<g, b, 0, st>>>(a, rg);
-                return lastError();
-            }
-        }
         if (fromL && wm) stepRecordKernel<S, PopPtr, true, true><<<g, b, 0, st>>>(a, rg);
         else if (fromL) stepRecordKernel<S, PopPtr, true, false><<<g, b, 0, st>>>(a, rg);
         else if (wm) stepRecordKernel<S, PopPtr, false, true><<<g, b, 0, st>>>(a, rg);

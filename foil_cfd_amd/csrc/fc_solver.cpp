@@ -16,10 +16,12 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/foilcfd.h"
+#include "fc_force_fold.hpp"
 #include "fc_geometry.hpp"
 #include "fc_kernels.hpp"
 #include "fc_units.hpp"
@@ -37,7 +39,6 @@ constexpr long long kCompactSettleSteps = 1000; // lbm_solver.cpp:34
 constexpr double kTimingEmaAlpha = 0.25;        // lbm_solver.cpp:41
 constexpr float kSpeckAmplitudeFrac = 1e-3f;    // lbm_solver.cpp:46
 constexpr unsigned kSpeckSeed = 0x5eedu;        // lbm_solver.cpp:49
-constexpr int kAvgRingSize = 256;               // lbm_solver.cpp:120
 
 // Phase of the speck perturbation from the seed (Weyl-style integer hash,
 // reference lbm_core.cu:763-767).
@@ -281,11 +282,9 @@ public:
         rampActive_ = true;
         rampStartStep_ = 0;
         gateOpenAtSteps_ = static_cast<long long>(kForceGateFlowThroughs * flowThroughSteps());
-        emaFx_ = emaFy_ = 0.0f;
-        emaSeeded_ = false;
+        fold_.clear();
         forcePending_ = false;
         stepsAtForceLaunch_ = stepsAtLastForceFold_ = 0;
-        ringHead_ = ringCount_ = 0;
         nanLatched_ = false;
         nanPending_ = false;
         nanKind_ = 0;
@@ -336,8 +335,7 @@ public:
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
     int setStepVariant(int v) {
-        if (v != kStepAuto && v != kStepSingle && v != kStepPair && v != kStepPipe && v != kStepStage &&
-            v != kStepBuf)
+        if (v != kStepAuto && v != kStepSingle && v != kStepBuf)
             return fail(FC_ERR_INVALID, "unknown step variant");
         stepVariant_ = v;
         return FC_OK;
@@ -806,45 +804,13 @@ public:
         *r = fc_force_readout{};
         if (!initialized_) return FC_OK;
         pollAsync();
-        r->flow_throughs = flowThroughsCompleted();
         // lbm_solver.cpp:1429-1435: fine scaling and fine span when the
         // momentum exchange runs on the fine grid.
         const bool fineForces = fine_.active && fine_.forcesFromFine;
         const Scaling& fs = fineForces ? fine_.scaling : scaling_;
         const int span = fineForces ? fine_.dims.nz : (zSlipWalls_ ? std::max(1, dims_.nz - 2) : dims_.nz);
-        r->cd = fs.forceToCoefficient(emaFx_, span);
-        r->cl = fs.forceToCoefficient(emaFy_, span);
-        r->lift_to_drag = (std::fabs(r->cd) > 1e-6f) ? r->cl / r->cd : 0.0f;
-        r->valid = (emaSeeded_ && steps_ >= gateOpenAtSteps_) ? 1 : 0;
-        if (r->valid && ringCount_ > 0) {
-            const long long oneFT = static_cast<long long>(flowThroughSteps());
-            std::vector<float> cl, cd, ld;
-            for (int i = 0; i < ringCount_; ++i) {
-                const int idx = (ringHead_ - 1 - i + kAvgRingSize) % kAvgRingSize;
-                const Sample& smp = ring_[idx];
-                if (smp.step < oneFT) continue;
-                const float c1 = fs.forceToCoefficient(smp.fy, span);
-                const float c2 = fs.forceToCoefficient(smp.fx, span);
-                cl.push_back(c1);
-                cd.push_back(c2);
-                ld.push_back((std::fabs(c2) > 1e-6f) ? c1 / c2 : 0.0f);
-            }
-            if (!cl.empty()) {
-                auto stats = [](std::vector<float> v, float& avg, float& mn, float& mx, float& med) {
-                    float sum = 0.0f;
-                    for (float x : v) sum += x;
-                    avg = sum / static_cast<float>(v.size());
-                    mn = *std::min_element(v.begin(), v.end());
-                    mx = *std::max_element(v.begin(), v.end());
-                    std::sort(v.begin(), v.end());
-                    const std::size_t m = v.size() / 2;
-                    med = (v.size() % 2 == 0) ? 0.5f * (v[m - 1] + v[m]) : v[m];
-                };
-                stats(cl, r->cl_avg, r->cl_min, r->cl_max, r->cl_median);
-                stats(cd, r->cd_avg, r->cd_min, r->cd_max, r->cd_median);
-                stats(ld, r->ld_avg, r->ld_min, r->ld_max, r->ld_median);
-            }
-        }
+        fold_.readout(fs, span, static_cast<long long>(flowThroughSteps()), steps_ >= gateOpenAtSteps_, r);
+        r->flow_throughs = flowThroughsCompleted();
         return FC_OK;
     }
 
@@ -938,7 +904,7 @@ public:
     void notifySnapshotRestored(bool fullState) {
         if (!initialized_) return;
         rampActive_ = false;
-        emaSeeded_ = false;
+        fold_.unseed();
         forcePending_ = false;
         stepsAtLastForceFold_ = steps_;
         nanLatched_ = false;
@@ -1007,7 +973,7 @@ public:
             if (rc != FC_OK) return rc;
         }
         rampActive_ = false;
-        emaSeeded_ = false;
+        fold_.unseed();
         forcePending_ = false;
         stepsAtLastForceFold_ = steps_;
         gateOpenAtSteps_ = steps_ + static_cast<long long>(kWarmGateFlowThroughs * flowThroughSteps());
@@ -1388,11 +1354,6 @@ public:
     }
 
 private:
-    struct Sample {
-        long long step = 0;
-        float fx = 0, fy = 0;
-    };
-
     int fail(int code, const std::string& why) {
         error = why;
         return code;
@@ -2224,22 +2185,9 @@ private:
             timingPending_ = false;
         }
         if (forcePending_ && eventDone(evForce_)) {
-            const float windowSteps = std::max(1.0f, emaWindowFlowThroughs_ * flowThroughSteps());
-            const float dSteps = static_cast<float>(stepsAtForceLaunch_ - stepsAtLastForceFold_);
-            const float alpha = 1.0f - std::exp(-dSteps / windowSteps);
-            const float fx = static_cast<float>(hForce_[0]);
-            const float fy = static_cast<float>(hForce_[1]);
-            if (!emaSeeded_) {
-                emaFx_ = fx;
-                emaFy_ = fy;
-                emaSeeded_ = true;
-            } else {
-                emaFx_ += alpha * (fx - emaFx_);
-                emaFy_ += alpha * (fy - emaFy_);
-            }
-            ring_[ringHead_] = Sample{stepsAtForceLaunch_, fx, fy};
-            ringHead_ = (ringHead_ + 1) % kAvgRingSize;
-            if (ringCount_ < kAvgRingSize) ++ringCount_;
+            fold_.fold(stepsAtForceLaunch_, static_cast<float>(stepsAtForceLaunch_ - stepsAtLastForceFold_),
+                       emaWindowFlowThroughs_ * flowThroughSteps(), static_cast<float>(hForce_[0]),
+                       static_cast<float>(hForce_[1]));
             lastRaw_[0] = hForce_[0];
             lastRaw_[1] = hForce_[1];
             lastRaw_[2] = hForce_[2];
@@ -2347,11 +2295,9 @@ private:
     int stepVariant_ = kStepAuto;
     long long gateOpenAtSteps_ = 0;
     float emaWindowFlowThroughs_ = 1.0f;
-    float emaFx_ = 0, emaFy_ = 0;
-    bool emaSeeded_ = false, forcePending_ = false;
+    ForceFold fold_;
+    bool forcePending_ = false;
     long long stepsAtForceLaunch_ = 0, stepsAtLastForceFold_ = 0;
-    Sample ring_[kAvgRingSize]{};
-    int ringHead_ = 0, ringCount_ = 0;
     double lastRaw_[3] = {0, 0, 0};
     long long lastRawStep_ = -1;
     bool nanLatched_ = false, nanPending_ = false;
@@ -2660,6 +2606,35 @@ int fc_force_raw(fc_solver* s, double* out3, long long* at_step) {
 }
 
 int fc_force_now(fc_solver* s, double* out3) { return s ? s->impl.forceNow(out3) : FC_ERR_INVALID; }
+
+struct fc_force_fold {
+    fc::ForceFold fold;
+};
+
+fc_force_fold* fc_force_fold_create(void) { return new (std::nothrow) fc_force_fold(); }
+
+void fc_force_fold_destroy(fc_force_fold* f) { delete f; }
+
+int fc_force_fold_clear(fc_force_fold* f) {
+    if (!f) return FC_ERR_INVALID;
+    f->fold.clear();
+    return FC_OK;
+}
+
+int fc_force_fold_add(fc_force_fold* f, long long at_step, float d_steps, float window_steps, double fx,
+                      double fy) {
+    if (!f || !(d_steps >= 0.0f)) return FC_ERR_INVALID;
+    f->fold.fold(at_step, d_steps, window_steps, static_cast<float>(fx), static_cast<float>(fy));
+    return FC_OK;
+}
+
+int fc_force_fold_readout(const fc_force_fold* f, const fc_scaling* scaling, int span_cells,
+                          long long one_flow_through_steps, int gate_open, fc_force_readout* out) {
+    if (!f || !scaling || !out || span_cells < 1) return FC_ERR_INVALID;
+    *out = fc_force_readout{};
+    f->fold.readout(fc::fromC(*scaling), span_cells, one_flow_through_steps, gate_open != 0, out);
+    return FC_OK;
+}
 int fc_mass(fc_solver* s, double* out) { return s ? s->impl.mass(out) : FC_ERR_INVALID; }
 
 int fc_read_macro(fc_solver* s, float* rho, float* u, float* v, float* w) {

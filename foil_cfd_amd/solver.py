@@ -76,6 +76,10 @@ def force_to_coefficient(s: FcScaling, force: float, span: int) -> float:
     return load().fc_force_to_coefficient(C.byref(s), force, span)
 
 
+def flow_through_steps(s: FcScaling, nx: int) -> float:
+    return load().fc_flow_through_steps(C.byref(s), nx)
+
+
 # ------------------------------------------------------------- geometry ----
 def generate_naca4(digits: str, points_per_surface: int = 200):
     """generateNACA4 (airfoil.cpp:177-266) -> (points (n,2) float32, le_index)."""
@@ -240,6 +244,53 @@ class ForceReadout:
     flow_throughs: float
     cl_avg: float
     cd_avg: float
+    cl_min: float = 0.0
+    cl_max: float = 0.0
+    cl_median: float = 0.0
+    cd_min: float = 0.0
+    cd_max: float = 0.0
+    cd_median: float = 0.0
+    ld_avg: float = 0.0
+    ld_min: float = 0.0
+    ld_max: float = 0.0
+    ld_median: float = 0.0
+
+
+class ForceFold:
+    """The solver's force EMA / trailing-average fold on caller-supplied
+    samples (fc_force_fold_*; reference lbm_solver.cpp:633-654, :1414-1509)."""
+
+    def __init__(self):
+        self._lib = load()
+        self._h = self._lib.fc_force_fold_create()
+        if not self._h:
+            raise MemoryError("fc_force_fold_create failed")
+
+    def close(self):
+        if self._h:
+            self._lib.fc_force_fold_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        check(self._lib.fc_force_fold_clear(self._h), "fc_force_fold_clear")
+
+    def add(self, at_step: int, d_steps: float, window_steps: float, fx: float, fy: float):
+        check(self._lib.fc_force_fold_add(self._h, int(at_step), float(d_steps), float(window_steps),
+                                          float(fx), float(fy)), "fc_force_fold_add")
+
+    def readout(self, scaling: FcScaling, span: int, one_ft_steps: int, gate_open: bool) -> ForceReadout:
+        r = FcForceReadout()
+        check(self._lib.fc_force_fold_readout(self._h, C.byref(scaling), int(span), int(one_ft_steps),
+                                              1 if gate_open else 0, C.byref(r)), "fc_force_fold_readout")
+        return ForceReadout(r.cl, r.cd, r.lift_to_drag, bool(r.valid), r.flow_throughs,
+                            r.cl_avg, r.cd_avg, r.cl_min, r.cl_max, r.cl_median, r.cd_min, r.cd_max,
+                            r.cd_median, r.ld_avg, r.ld_min, r.ld_max, r.ld_median)
 
 
 class LBMSolver:
@@ -336,8 +387,8 @@ class LBMSolver:
         return {"active": bool(r.active), "factor": r.factor,
                 "box": (r.box.x0, r.box.y0, r.box.x1, r.box.y1),
                 "fine_dims": (r.fine_dims.nx, r.fine_dims.ny, r.fine_dims.nz),
-                "fine_tau": r.fine_scaling.tau, "device_bytes": r.device_bytes,
-                "forces_from_fine": bool(r.forces_from_fine)}
+                "fine_tau": r.fine_scaling.tau, "fine_scaling": r.fine_scaling,
+                "device_bytes": r.device_bytes, "forces_from_fine": bool(r.forces_from_fine)}
 
     def download_fine_f(self, f64: bool = True) -> np.ndarray:
         info = self.refinement_info()
@@ -389,12 +440,10 @@ class LBMSolver:
               "fc_upload_finer_f")
 
     def set_step_variant(self, variant: str):
-        """"auto" (library default), "single" (one cell per lane), "pair"
-        (two x-adjacent cells per lane where the grid allows it), "pipe"
-        (persistent, software-pipelined one-cell-per-lane kernel), "stage"
-        (persistent, LDS-staged) or "buf" (one cell per lane with buffer
-        addressing)."""
-        v = {"auto": 0, "single": 1, "pair": 2, "pipe": 3, "stage": 4, "buf": 5}[variant]
+        """"auto" (library default), "single" (the default cell kernel on every
+        sub-step, restriction records from the separate pass) or "buf" (one
+        cell per lane with buffer addressing)."""
+        v = {"auto": 0, "single": 1, "buf": 2}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):
@@ -480,7 +529,8 @@ class LBMSolver:
         r = FcForceReadout()
         check(self._lib.fc_forces(self._h, C.byref(r)), "fc_forces")
         return ForceReadout(r.cl, r.cd, r.lift_to_drag, bool(r.valid), r.flow_throughs,
-                            r.cl_avg, r.cd_avg)
+                            r.cl_avg, r.cd_avg, r.cl_min, r.cl_max, r.cl_median, r.cd_min, r.cd_max,
+                            r.cd_median, r.ld_avg, r.ld_min, r.ld_max, r.ld_median)
 
     def force_raw(self):
         out = (C.c_double * 3)()

@@ -195,20 +195,16 @@ int fc_set_force_ema_window(fc_solver* s, float flow_throughs);  /* setForceEmaW
 int fc_set_smagorinsky_cs(fc_solver* s, float cs);
 
 /* Step kernel selection (no reference counterpart: the reference has one
- * kernel). FC_STEP_AUTO (default) is the library's choice, currently one
- * cell per lane with 64-bit population addressing (the fastest measured on
- * MI355X); FC_STEP_SINGLE forces it; FC_STEP_PAIR runs two x-adjacent cells
- * per lane with packed population accesses where the grid allows it (even
- * nx, x / y not periodic, population array under 2 GiB); FC_STEP_PIPE runs a
- * persistent, software-pipelined kernel that keeps the next row segment's
- * population loads in flight while it collides the current one;
- * FC_STEP_STAGE runs a persistent kernel that stages the next row segment's
- * populations in LDS; FC_STEP_BUF runs the one-cell kernel with 32-bit buffer
- * addressing and eight waves per SIMD. PIPE, STAGE and BUF need a population
- * array under 2 GiB and otherwise fall back to the default. All give the same
- * cell update, bit for bit. */
-enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_PAIR = 2, FC_STEP_PIPE = 3, FC_STEP_STAGE = 4,
-       FC_STEP_BUF = 5 };
+ * kernel). FC_STEP_AUTO (default) is the library's choice: one cell per
+ * lane with 64-bit population addressing (the fastest measured on MI355X);
+ * on a factor-2 patch level without a nested level, the last fine sub-step
+ * of each coarse step also writes the restriction's child-average records.
+ * FC_STEP_SINGLE runs the same cell kernel on every sub-step and leaves the
+ * records to the separate restriction pass. FC_STEP_BUF runs the one-cell
+ * kernel with 32-bit buffer addressing (population arrays under 2 GiB, the
+ * default kernel beyond) and the separate restriction pass. All three give
+ * the same cell update, bit for bit. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_BUF = 2 };
 int fc_set_step_variant(fc_solver* s, int variant);
 
 /* ---- stepping ------------------------------------------------------------ */
@@ -226,6 +222,24 @@ int fc_forces(fc_solver* s, fc_force_readout* out);   /* LBMSolver::forces, :371
 int fc_force_raw(fc_solver* s, double* out3, long long* at_step);
 /* Synchronous momentum-exchange force of the current state (launchForceReduction). */
 int fc_force_now(fc_solver* s, double* out3);
+/* The force fold of fc_forces on caller-supplied samples (the reference
+ * keeps it inside LBMSolver: EMA fold lbm_solver.cpp:633-654, statistics
+ * lbm_solver.cpp:1414-1509, 256-sample ring lbm_solver.cpp:120). The solver
+ * runs the same fold on its own samples; callers that take theirs with
+ * fc_force_now (or from another solver) fold them here. add: the EMA weight
+ * is 1 - exp(-d_steps / max(1, window_steps)), the first sample after a
+ * create / clear seeds it, every sample enters the ring. readout: the EMA
+ * coefficients (scaling, span_cells) and, when gate_open, mean / min / max /
+ * median of Cl, Cd and L/D over the ring samples taken at step >=
+ * one_flow_through_steps; flow_throughs is left 0. */
+typedef struct fc_force_fold fc_force_fold;
+fc_force_fold* fc_force_fold_create(void);
+void fc_force_fold_destroy(fc_force_fold* f);
+int fc_force_fold_clear(fc_force_fold* f);
+int fc_force_fold_add(fc_force_fold* f, long long at_step, float d_steps, float window_steps, double fx,
+                      double fy);
+int fc_force_fold_readout(const fc_force_fold* f, const fc_scaling* scaling, int span_cells,
+                          long long one_flow_through_steps, int gate_open, fc_force_readout* out);
 /* Synchronous FP64 global mass sum over Fluid cells of sum_q f_q. */
 int fc_mass(fc_solver* s, double* out);
 /* Host copies of the macroscopic fields (velocityField/deviceRho, :425-429);
