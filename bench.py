@@ -87,26 +87,13 @@ def run_storage(case, storage, steps, warmup, per_call, kernel_steps, variant="a
     return res
 
 
-PATCH_MARGINS_C = (0.20, 0.50, 0.10, 0.20)  # upstream, wake, below, above (src/app/ui.h:159-162)
-PARTICLES = 1_000_000
-
-
 def full_setup(case, storage, aoa, wall="newton"):
-    """The default-tunnel configuration on top of the coarse wing: factor-2
-    patch derived from the solid bbox with the app's margins, clean surface
-    references, wall model, particles."""
+    """The default-tunnel configuration on top of the coarse wing
+    (cases.default_tunnel): factor-2 patch with the app's margins, clean
+    surface references, wall model on both levels, 1 M particles."""
     from foil_cfd_amd import cases
     t0 = time.perf_counter()
-    up, wake, below, above = PATCH_MARGINS_C
-    box, fine = cases.refinement_patch(case, 2, upstream_c=up, wake_c=wake, below_c=below, above_c=above,
-                                       aoa=aoa)
-    s = cases.make_solver(case, storage)
-    s.set_surface_reference(case.flags)
-    s.init_refinement(box, 2, fine)
-    s.set_refined_surface_reference(fine)
-    s.set_wall_model(wall)
-    s.particles_seed(PARTICLES, 0xF01DCAFE)
-    info = s.refinement_info()
+    s, box, info = cases.default_tunnel(case, storage, aoa, wall)
     log(f"[bench] {storage}: patch {box} fine grid {info['fine_dims']} "
         f"({info['device_bytes'] / 1e9:.2f} GB), wall cells coarse {s.wall_cell_count_level(0)} "
         f"fine {s.wall_cell_count_level(1)}, setup {time.perf_counter() - t0:.1f} s")
@@ -232,6 +219,87 @@ def accuracy_run(args):
         if status != "ok":
             break
     s.close()
+
+
+ACCURACY_PROFILE = os.path.join(ROOT, "profiles", "wing_cl_cd_accuracy.json")
+
+
+def accuracy_summary(paths, out_path, batches=6):
+    """Folds --accuracy records into the committed summary: per angle of
+    attack and storage the time-averaged Cl / Cd over the samples taken from
+    one flow-through on (the reference readout's window), the standard error
+    of each average from `batches` batch means, the error against the FP64
+    run of the same angle with its one-sigma uncertainty, and how long the
+    sample traces of a storage and the FP64 run stay within 1 % of each other
+    (the start-up transient is deterministic; the developed separated flow is
+    chaotic, so traces then decorrelate and only averages compare)."""
+    import numpy as np
+    runs = {}
+    for p in paths:
+        with open(p) as fh:
+            r = json.load(fh)
+        runs.setdefault(f"{r['aoa']:g}", {})[r["storage"]] = r
+    out = {"workload": "naca2412_wing_512x192x240_chord256_trt_les_imem_patch2x_particles1M",
+           "protocol": "one momentum-exchange sample per 100 coarse steps; average over samples at or after "
+                       "1 flow-through (reference readout), 4 flow-throughs run; standard error from batch means",
+           "batches": batches, "aoa": {}}
+    for aoa, by in sorted(runs.items(), key=lambda kv: float(kv[0])):
+        ent = {}
+        for st, r in by.items():
+            s = np.asarray(r["samples"], dtype=np.float64)
+            ft = r["flow_through_steps"]
+            m = s[:, 0] >= ft
+            e = {"status": r.get("status"), "steps": int(r.get("steps_done", 0)), "wall_s": r.get("wall_s"),
+                 "samples": int(m.sum())}
+            for name, col in (("cl", 3), ("cd", 4)):
+                v = s[m, col]
+                bm = np.array([b.mean() for b in np.array_split(v, batches)])
+                e[name] = round(float(v.mean()), 6)
+                e[name + "_se"] = round(float(bm.std(ddof=1) / np.sqrt(batches)), 6)
+                e[name + "_batch_means"] = [round(float(x), 5) for x in bm]
+            e["readout_cl_avg"] = r["readout"]["cl_avg"]
+            e["readout_cd_avg"] = r["readout"]["cd_avg"]
+            ent[st] = e
+        if "f64" in by:
+            ref = np.asarray(by["f64"]["samples"], dtype=np.float64)
+            for st, e in ent.items():
+                if st == "f64":
+                    continue
+                for name in ("cl", "cd"):
+                    r64, s64 = ent["f64"][name], ent["f64"][name + "_se"]
+                    e[name + "_err_vs_f64"] = round((e[name] - r64) / abs(r64), 5)
+                    e[name + "_err_sigma"] = round(float(np.hypot(e[name + "_se"], s64)) / abs(r64), 5)
+                # trace difference per sample, relative to the FP64 run's mean
+                s = np.asarray(by[st]["samples"], dtype=np.float64)
+                k = min(len(s), len(ref))
+                rel = np.maximum(np.abs(s[:k, 3] - ref[:k, 3]) / abs(ent["f64"]["cl"]),
+                                 np.abs(s[:k, 4] - ref[:k, 4]) / abs(ent["f64"]["cd"]))
+                off = np.nonzero(rel > 0.01)[0]
+                e["traces_within_1pct_until_step"] = int(ref[k - 1, 0]) if not len(off) else (
+                    int(ref[off[0] - 1, 0]) if off[0] > 0 else 0)
+                early = ref[:k, 0] <= 1100
+                e["trace_max_rel_diff_to_step_1100"] = round(float(rel[early].max()), 6) if early.any() else None
+        out["aoa"][aoa] = ent
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps({k: {st: {kk: v for kk, v in e.items() if "batch" not in kk} for st, e in ent.items()}
+                      for k, ent in out["aoa"].items()}), flush=True)
+
+
+def accuracy_line():
+    """The committed accuracy summary, reduced to the bench line's figures."""
+    try:
+        with open(ACCURACY_PROFILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    res = {}
+    for aoa, ent in rec.get("aoa", {}).items():
+        for st, e in ent.items():
+            if "cl_err_vs_f64" in e:
+                res.setdefault(f"aoa{aoa}", {})[st] = {k: e[k] for k in ("cl_err_vs_f64", "cd_err_vs_f64",
+                                                                        "cl_err_sigma", "cd_err_sigma")}
+    return {"per_aoa": res, "source": "profiles/wing_cl_cd_accuracy.json"} if res else None
 
 
 def cpu_baseline(case, seconds=12.0):
@@ -417,6 +485,8 @@ def main():
     ap.add_argument("--flow-throughs", type=float, default=4.0)
     ap.add_argument("--wall", default="newton", choices=["off", "newton", "mlp"])
     ap.add_argument("--out", default="accuracy.json")
+    ap.add_argument("--accuracy-summary", nargs="+", metavar="RECORD",
+                    help="fold --accuracy records into the committed accuracy summary (no GPU)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -428,6 +498,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.accuracy_summary:
+        accuracy_summary(args.accuracy_summary, ACCURACY_PROFILE)
         return
     if args.polar:
         polar_bench(args)
@@ -472,10 +545,10 @@ def main():
                     f"particles1M_{head}store")
         config = {"workload": workload, "cells": ncells, "fine_cells": fd[0] * fd[1] * fd[2],
                   "fine_grid": fd, "patch_box_coarse": r["patch_box"], "refine_factor": 2,
-                  "patch_margins_chords": dict(zip(("upstream", "wake", "below", "above"), PATCH_MARGINS_C)),
+                  "patch_margins_chords": dict(zip(("upstream", "wake", "below", "above"), cases.APP_PATCH_MARGINS_C)),
                   "cell_updates_per_step": r["cell_updates_per_step"], "storage": store,
                   "wall_model": "iMEM Reichardt (Newton), coarse + fine wall cells",
-                  "particles": PARTICLES}
+                  "particles": cases.DEFAULT_PARTICLES}
         kernel = ("fine-level stream-collide launches (dominant; one of the two per coarse step also writes "
                   "the restriction's child-average records)")
         traffic = measured_traffic(head, "fine")
@@ -514,6 +587,7 @@ def main():
                             for kk, vv in v.items()} for k, v in results.items()},
         "reference_published_glups_fp32": REFERENCE_PUBLISHED_GLUPS_FP32,
     }
+    out["cl_cd_err_vs_fp64"] = accuracy_line()
     if head in ("f16c", "f16") and "f32" in results:
         out["fp16_vs_fp32_speedup"] = round(results[head]["glups"] / results["f32"]["glups"], 3)
     print(json.dumps(out), flush=True)

@@ -210,6 +210,7 @@ SIGNATURES = {
     "fc_forces": (C.c_int, [_vp, C.POINTER(FcForceReadout)]),
     "fc_force_raw": (C.c_int, [_vp, _dp, _llp]),
     "fc_force_now": (C.c_int, [_vp, _dp]),
+    "fc_force_now_noalloc": (C.c_int, [_vp, _dp]),
     "fc_force_fold_create": (_vp, []),
     "fc_force_fold_destroy": (None, [_vp]),
     "fc_force_fold_clear": (C.c_int, [_vp]),

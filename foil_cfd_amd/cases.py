@@ -139,6 +139,78 @@ def presolve_case(case: Case, chord: int | None = None, aoa: float = 8.0, digits
     return Case(f"{case.name}_pre{nc}", dims, flags, pre, cs=case.cs, ramp=True)
 
 
+APP_PATCH_MARGINS_C = (0.20, 0.50, 0.10, 0.20)  # upstream, wake, below, above (src/app/ui.h:159-162)
+DEFAULT_PARTICLES = 1_000_000                    # src/render/viz.h:28
+PARTICLE_SEED = 0xF01DCAFE                       # src/render/viz.cpp:676
+
+
+def default_patch(case: Case, aoa: float):
+    """Patch box of the app's margins derived from the case's (possibly
+    vane-merged) flags, and the clean foil flags of the fine level."""
+    up, wake, below, above = APP_PATCH_MARGINS_C
+    return refinement_patch(case, 2, upstream_c=up, wake_c=wake, below_c=below, above_c=above, aoa=aoa)
+
+
+def default_tunnel(case: Case, storage: str, aoa: float, wall: str = "newton", particles: int = DEFAULT_PARTICLES,
+                   fine_flags=None, coarse_flags=None, box=None):
+    """The reference app's default tunnel on top of a wing case: a factor-2
+    patch derived from the solid box with the app's margins (main.cpp:590-
+    598), the clean foil flags as the surface references of both levels, the
+    iMEM wall model on the wall cells of both levels and the tracer
+    particles. fine_flags / coarse_flags replace the plain foil flags of a
+    level (e.g. foil + vanes; the box then comes from the coarse ones, as
+    the app derives it from the vane-merged flags, main.cpp:560-600).
+    A given box (coarse cells) needs fine_flags. Returns (solver, box,
+    refinement info)."""
+    live = case if coarse_flags is None else Case(case.name, case.dims, coarse_flags, case.scaling,
+                                                   case.periodic_xy, case.cs, case.ramp, case.f0)
+    if box is None:
+        box, fine = default_patch(live, aoa)
+    else:  # a given box comes with its fine flags
+        fine = fine_flags
+    s = make_solver(live, storage)
+    s.set_surface_reference(case.flags)
+    s.init_refinement(box, 2, fine if fine_flags is None else fine_flags)
+    s.set_refined_surface_reference(fine)
+    s.set_wall_model(wall)
+    if particles:
+        s.particles_seed(particles, PARTICLE_SEED)
+    return s, box, s.refinement_info()
+
+
+def lowbias32(x):
+    """The reference's integer hash (src/render/particles.cu:48-55) on uint32 arrays."""
+    x = np.asarray(x, dtype=np.uint32).copy()
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x7FEB352D)
+    x ^= x >> np.uint32(15)
+    x *= np.uint32(0x846CA68B)
+    x ^= x >> np.uint32(16)
+    return x
+
+
+def synthetic_vane_q(qfrac, ffmask, seed=0xC4A11E5, q_min=0.05):
+    """Replaces the cut fraction of every vane link (nonzero bytes of the
+    dense qfrac[cell*19 + q], ascending) by q ~ U[0, 1) from a fixed-seed
+    lowbias32 stream, 24-bit uniform as particles.cu:57-60 draws it, then
+    applies the densify rule of vg.cpp:380-392: q < q_min drops the link
+    (byte 0, two-node bit cleared), else byte = lround(255 q), 0 -> 1.
+    Returns (qfrac, ffmask, kept links)."""
+    qf = np.array(qfrac, dtype=np.uint8, copy=True).reshape(-1)
+    ff = np.array(ffmask, dtype=np.uint32, copy=True).reshape(-1)
+    idx = np.nonzero(qf)[0]
+    u = (lowbias32(np.uint32(seed) + np.arange(idx.size, dtype=np.uint32)) & np.uint32(0xFFFFFF)).astype(
+        np.float32) * np.float32(1.0 / 16777216.0)
+    keep = u >= np.float32(q_min)
+    b = np.floor(u.astype(np.float64) * 255.0 + 0.5).astype(np.int64)  # lround of a positive value
+    b[b == 0] = 1
+    qf[idx] = np.where(keep, b, 0).astype(np.uint8)
+    drop = idx[~keep]
+    cell, q = drop // 19, drop % 19
+    np.bitwise_and.at(ff, cell, ~(np.uint32(1) << q.astype(np.uint32)))
+    return qf, ff, int(keep.sum())
+
+
 def make_solver(case: Case, storage: str = "f32") -> S.LBMSolver:
     s = S.LBMSolver()
     s.set_startup_ramp_enabled(case.ramp)

@@ -64,6 +64,16 @@ int launchStep(const LatticeView& lat, int layout, const StepArgs& args, void* s
 int launchForce(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks,
                 int count, double* partials, int partialSlots, double* out3, void* stream);
 int forcePartialSlots(int count);
+// The same reduction with the partials and the result in the library's
+// static device scratch (no allocation): at most kNoAllocForceSlots blocks
+// of force cells (kNoAllocForceSlots * 256 cells); the result is copied into
+// host3 and the stream synchronized. noAllocScratch returns the scratch
+// address of the current device (looked up once per device).
+inline constexpr int kNoAllocForceSlots = 8192;
+inline constexpr int kMaxScratchDevices = 64;
+int noAllocScratch(double** out);
+int launchForceNoAlloc(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks, int count,
+                       double* host3, void* stream);
 // Global FP64 mass over Fluid cells; partials sized by massPartialSlots().
 int launchMass(const LatticeView& lat, int layout, double* partials, int partialSlots,
                double* out1, void* stream);

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "fc_kcommon.hpp"
 #include "fc_refine_math.hpp"
@@ -500,11 +501,7 @@ __global__ void __launch_bounds__(kBlock) initKernel(KArgs a, float u0) {
     for (int q = 0; q < kQ; ++q) {
         const C h = heqOf<C>(q, K, C(0), C(1), ux, C(0), C(0), usq);
         S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
-        if (k & kKindFeed) {
-            T* fd = static_cast<T*>(a.feed);
-            S::store(fd + q * plane + yz, h);
-            S::store(fd + (kQ + q) * plane + yz, h);
-        }
+        if (k & kKindFeed) S::store(static_cast<T*>(a.feed) + q * plane + yz, h);  // read from layout L
     }
 }
 
@@ -545,7 +542,12 @@ __global__ void __launch_bounds__(kBlock) speckKernel(KArgs a, float amp, float 
     }
 }
 
-// Canonical unpadded SoA (FP32 or FP64) -> layout L, and the feed planes.
+// Canonical unpadded SoA (FP32 or FP64) -> layout L, and the half of the
+// outlet feed planes the next step reads. The other half keeps what it held,
+// as the reference's uploadF replaces only the active ping-pong buffer
+// (tests/lbm_test_util.h:116-147): a feed cell that never collides (a slip
+// cell beside an outlet corner) then still alternates between the uploaded
+// state and the one before it, like the reference's two buffers.
 template <class S, class CT>
 __global__ void __launch_bounds__(kBlock) uploadKernel(KArgs a, const CT* canon) {
     using T = typename S::T;
@@ -1157,6 +1159,61 @@ int launchWatchdog(const LatticeView& lat, int layout, long long sampleOffset, i
                    int* flag, void* stream) {
     return byStorage<WatchdogOp>(lat.storage, lat, layout, sampleOffset, flagBase, flag,
                                  static_cast<hipStream_t>(stream));
+}
+
+// ---- allocation-free force reduction ----------------------------------------
+// File-scope device scratch of the allocation-free force entry point: the
+// force kernel's per-block partial sums (3 doubles per block) and the final
+// sum. Every GPU holds its own copy for the life of the process; the host
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the symbol's name is not a device address in host code).
+// Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[3 * kNoAllocForceSlots + 4];
+
+namespace {
+std::mutex gScratchMutex;
+double* gScratchPtr[kMaxScratchDevices] = {};
+}  // namespace
+
+int noAllocScratch(double** out) {
+    *out = nullptr;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+        (void)hipGetLastError();
+        return static_cast<int>(hipErrorInvalidDevice);
+    }
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchPtr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return static_cast<int>(e != hipSuccess ? e : hipErrorInvalidSymbol);
+        }
+        gScratchPtr[dev] = static_cast<double*>(p);
+    }
+    *out = gScratchPtr[dev];
+    return 0;
+}
+
+int launchForceNoAlloc(const LatticeView& lat, int layout, const int* cells, const std::uint32_t* masks, int count,
+                       double* host3, void* stream) {
+    if (count < 0 || !host3 || (count > 0 && (!cells || !masks))) return static_cast<int>(hipErrorInvalidValue);
+    if ((count + kBlock - 1) / kBlock > kNoAllocForceSlots) return static_cast<int>(hipErrorInvalidValue);
+    double* d = nullptr;
+    if (const int e = noAllocScratch(&d); e != 0) return e;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    double* out3 = d + 3 * kNoAllocForceSlots;
+    // every partial a launch reads is written by the same launch, and the
+    // final sum overwrites the result slot: nothing needs clearing
+    if (const int e = byStorage<ForceOp>(lat.storage, lat, layout, cells, masks, count, d, kNoAllocForceSlots, out3,
+                                         st);
+        e != 0)
+        return e;
+    if (hipMemcpyAsync(host3, out3, 3 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return lastError();
+    return 0;
 }
 
 int launchFillFloat(float* dst, long long count, float value, void* stream) {

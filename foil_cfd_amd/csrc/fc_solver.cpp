@@ -222,6 +222,11 @@ public:
             freeAll();
             return fail(FC_ERR_DEVICE, "initial field setup failed");
         }
+        // Device address of the static scratch of fc_force_now_noalloc on
+        // this device (looked up once per device; a failure only disables
+        // that entry point, which then reports it).
+        double* scratch = nullptr;
+        noAllocReady_ = noAllocScratch(&scratch) == 0;
         return FC_OK;
     }
 
@@ -836,6 +841,23 @@ public:
             rc = fail(FC_ERR_DEVICE, "force readback failed");
         hipRelease(d);
         return rc;
+    }
+
+    // forceNow without allocation: partials and result in the library's
+    // static device scratch (at most kNoAllocForceSlots * 256 force cells).
+    int forceNowNoAlloc(double* out3) {
+        if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
+        if (!out3) return fail(FC_ERR_INVALID, "null output");
+        if (!noAllocReady_) return fail(FC_ERR_DEVICE, "static scratch address not available on this device");
+        const bool fineForces = fine_.active && fine_.forcesFromFine;
+        const int count = fineForces ? fine_.forceCount : forceCount_;
+        if (count > kNoAllocForceSlots * 256)
+            return fail(FC_ERR_INVALID, "force list of " + std::to_string(count) + " cells exceeds the static scratch (" +
+                                            std::to_string(kNoAllocForceSlots * 256) + " cells)");
+        const int e = fineForces ? launchForceNoAlloc(fineView(), fine_.layout, fine_.dForceCells, fine_.dForceMasks,
+                                                      count, out3, stream_)
+                                 : launchForceNoAlloc(view(), layout_, dForceCells_, dForceMasks_, count, out3, stream_);
+        return e == 0 ? FC_OK : fail(FC_ERR_DEVICE, "allocation-free force reduction failed");
     }
 
     int mass(double* out) {
@@ -2296,6 +2318,7 @@ private:
     long long gateOpenAtSteps_ = 0;
     float emaWindowFlowThroughs_ = 1.0f;
     ForceFold fold_;
+    bool noAllocReady_ = false;
     bool forcePending_ = false;
     long long stepsAtForceLaunch_ = 0, stepsAtLastForceFold_ = 0;
     double lastRaw_[3] = {0, 0, 0};
@@ -2606,6 +2629,10 @@ int fc_force_raw(fc_solver* s, double* out3, long long* at_step) {
 }
 
 int fc_force_now(fc_solver* s, double* out3) { return s ? s->impl.forceNow(out3) : FC_ERR_INVALID; }
+
+int fc_force_now_noalloc(fc_solver* s, double* out3) {
+    return s ? s->impl.forceNowNoAlloc(out3) : FC_ERR_INVALID;
+}
 
 struct fc_force_fold {
     fc::ForceFold fold;

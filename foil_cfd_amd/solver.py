@@ -543,6 +543,12 @@ class LBMSolver:
         check(self._lib.fc_force_now(self._h, out), "fc_force_now")
         return np.array(out[:])
 
+    def force_now_noalloc(self) -> np.ndarray:
+        """fc_force_now through the library's static device scratch (no allocation)."""
+        out = (C.c_double * 3)()
+        check(self._lib.fc_force_now_noalloc(self._h, out), "fc_force_now_noalloc")
+        return np.array(out[:])
+
     def mass(self) -> float:
         out = C.c_double(0)
         check(self._lib.fc_mass(self._h, C.byref(out)), "fc_mass")

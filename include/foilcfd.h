@@ -222,6 +222,15 @@ int fc_forces(fc_solver* s, fc_force_readout* out);   /* LBMSolver::forces, :371
 int fc_force_raw(fc_solver* s, double* out3, long long* at_step);
 /* Synchronous momentum-exchange force of the current state (launchForceReduction). */
 int fc_force_now(fc_solver* s, double* out3);
+/* fc_force_now without any device allocation (no reference counterpart;
+ * for callers that cannot allocate: code around HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The reduction's per-block
+ * partials and its result live in the library's static device scratch
+ * (192 KiB per GPU, held until the process ends); the same kernels as
+ * fc_force_now, so the result is bit-identical. At most 8192 * 256 force
+ * cells (FC_ERR_INVALID beyond). Calls that overlap on one GPU (solvers on
+ * different streams) share the scratch and race. */
+int fc_force_now_noalloc(fc_solver* s, double* out3);
 /* The force fold of fc_forces on caller-supplied samples (the reference
  * keeps it inside LBMSolver: EMA fold lbm_solver.cpp:633-654, statistics
  * lbm_solver.cpp:1414-1509, 256-sample ring lbm_solver.cpp:120). The solver

@@ -1095,7 +1095,9 @@ void orc_stream_collide_f64(const double* src, double* dst, const uint8_t* flags
 // with per-step tau and inlet speed supplied by the caller (the solver's
 // startup ramps, lbm_solver.cpp:1144-1154). The macro arrays are written on
 // the last step (writeMacro = i == n-1). fA holds the input state and, on
-// return, the output state (the swap count is handled internally).
+// return, the output state; fB holds the second ping-pong buffer and, on
+// return, the state before the last step (the swap count is handled
+// internally).
 void orc_run_f32(float* fA, float* fB, const uint8_t* flags, int nx, int ny, int nz,
                  int periodicX, int periodicY, int nsteps, const float* tauPerStep,
                  const float* uInPerStep, float magicLambda, float smagPrefactor,
@@ -1111,7 +1113,7 @@ void orc_run_f32(float* fA, float* fB, const uint8_t* flags, int nx, int ny, int
                                     ffMask);
         std::swap(src, dst);
     }
-    if (src != fA) std::memcpy(fA, src, sizeof(float) * (size_t)kQ * (size_t)g.ncells());
+    if (src != fA) std::swap_ranges(fA, fA + (size_t)kQ * (size_t)g.ncells(), fB);
 }
 
 void orc_run_f64(double* fA, double* fB, const uint8_t* flags, int nx, int ny, int nz,
@@ -1129,7 +1131,7 @@ void orc_run_f64(double* fA, double* fB, const uint8_t* flags, int nx, int ny, i
                                       qFrac, ffMask);
         std::swap(src, dst);
     }
-    if (src != fA) std::memcpy(fA, src, sizeof(double) * (size_t)kQ * (size_t)g.ncells());
+    if (src != fA) std::swap_ranges(fA, fA + (size_t)kQ * (size_t)g.ncells(), fB);
 }
 
 // Storage-format emulation (test infrastructure, no reference counterpart):
@@ -1190,13 +1192,16 @@ void orc_run_stored_f64(double* fA, double* fB, const uint8_t* flags, int nx, in
                         int periodicX, int periodicY, int nsteps, const double* tauPerStep,
                         const double* uInPerStep, double magicLambda, double smagPrefactor,
                         double* rho, double* u, double* v, double* w, int mant, int emin,
-                        double scale) {
+                        double scale, int keepOther) {
     const Grid g = makeGrid(nx, ny, nz, periodicX, periodicY);
     const long long n = g.ncells();
     double* src = fA;
     double* dst = fB;
     roundStorage(src, n, mant, emin, scale);
-    std::memcpy(dst, src, sizeof(double) * (size_t)kQ * (size_t)n);
+    if (keepOther)
+        roundStorage(dst, n, mant, emin, scale);
+    else
+        std::memcpy(dst, src, sizeof(double) * (size_t)kQ * (size_t)n);
     for (int i = 0; i < nsteps; ++i) {
         StepArgs a{tauPerStep[i], magicLambda, smagPrefactor, uInPerStep[i], i == nsteps - 1};
         streamCollide<double, double>(src, dst, flags, g, a, rho, u, v, w, nullptr, nullptr,

@@ -73,7 +73,7 @@ def lib():
             [_dp] * 4 + [_u16p] * 3 + [_u8p, _u32p]
         L.orc_run_stored_f64.argtypes = [_dp, _dp] + common + [C.c_int, _dp, _dp, C.c_double,
                                                                C.c_double] + \
-            [_dp] * 4 + [C.c_int, C.c_int, C.c_double]
+            [_dp] * 4 + [C.c_int, C.c_int, C.c_double, C.c_int]
         L.orc_f16c_encode.restype = C.c_uint16
         L.orc_f16c_encode.argtypes = [C.c_float]
         L.orc_f16c_decode.restype = C.c_float
@@ -203,14 +203,18 @@ def smag_prefactor(cs):
 
 
 def run(f, flags, dims, taus, uins, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False,
-        slip=None, qfrac=None, ffmask=None, macro=True):
+        slip=None, qfrac=None, ffmask=None, macro=True, other=None):
     """Advance ``f`` (canonical SoA, modified in place) by len(taus) steps of the
-    reference step sequence; returns (rho, u, v, w) of the last step."""
+    reference step sequence; returns (rho, u, v, w) of the last step.
+    ``other`` is the initial content of the second ping-pong buffer (default:
+    a copy of ``f``); only cells that never get written (solid, slip) keep
+    it, and it is read only where an outlet copies such a cell. It is
+    updated in place with the final other buffer when given."""
     nx, ny, nz = dims
     n = nx * ny * nz
     f64 = f.dtype == np.float64
     dt = np.float64 if f64 else np.float32
-    tmp = np.empty_like(f)
+    tmp = f.copy() if other is None else other
     fl = np.ascontiguousarray(flags, dtype=np.uint8)
     taus = np.ascontiguousarray(taus, dtype=dt)
     uins = np.ascontiguousarray(uins, dtype=dt)
@@ -235,15 +239,19 @@ STORAGE_FORMATS = {  # (stored mantissa bits, min normal exponent, deviation sca
 }
 
 
-def run_stored(f, flags, dims, taus, uins, fmt, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False):
+def run_stored(f, flags, dims, taus, uins, fmt, lam=3.0 / 16.0, cs=0.1733, periodic_xy=False,
+               other=None):
     """FP64 steps with every population rounded after each step as storage
     format ``fmt`` (a STORAGE_FORMATS key or a (mant, emin, scale) tuple)
-    keeps it. ``f`` (FP64 canonical SoA) is advanced in place."""
+    keeps it. ``f`` (FP64 canonical SoA) is advanced in place. ``other`` is
+    the second buffer's initial content, as in run() (default: a copy of
+    ``f``); it is rounded to the format too and updated in place."""
     nx, ny, nz = dims
     n = nx * ny * nz
     mant, emin, scale = STORAGE_FORMATS[fmt] if isinstance(fmt, str) else fmt
     assert f.dtype == np.float64
-    tmp = np.empty_like(f)
+    tmp = np.empty_like(f) if other is None else other
+    assert tmp.dtype == np.float64 and tmp.size == f.size
     fl = np.ascontiguousarray(flags, dtype=np.uint8)
     taus = np.ascontiguousarray(taus, dtype=np.float64)
     uins = np.ascontiguousarray(uins, dtype=np.float64)
@@ -251,7 +259,7 @@ def run_stored(f, flags, dims, taus, uins, fmt, lam=3.0 / 16.0, cs=0.1733, perio
     lib().orc_run_stored_f64(_p(f, _dp), _p(tmp, _dp), _p(fl, _u8p), nx, ny, nz, int(periodic_xy),
                              int(periodic_xy), len(taus), _p(taus, _dp), _p(uins, _dp), lam,
                              float(smag_prefactor(cs)), *[_p(a, _dp) for a in m], int(mant),
-                             int(emin), float(scale))
+                             int(emin), float(scale), int(other is not None))
     return tuple(m)
 
 

@@ -1,0 +1,208 @@
+"""The 23.6 M-cell wing in the configurations it is quoted on, through the
+HIP path: the full default tunnel (factor-2 patch, iMEM wall model on both
+levels, 1 M particles) in the three storages, and the vortex-generator array
+on the same wing with the learned wall function. Long time averages of the
+coefficients (several flow-throughs) are measured by bench.py --accuracy;
+these tests run the configurations for a few hundred coarse steps, inside
+the start-up transient, where the flow is still deterministic enough for
+the storage formats to be compared field by field against FP64."""
+import sys
+
+import numpy as np
+import pytest
+
+import foil_cfd_amd as fc
+from foil_cfd_amd import cases, mlp
+from foil_cfd_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run_default_tunnel(case, storage, aoa, calls, per_call=100, wall="newton", coarse_flags=None,
+                       fine_flags=None, before_steps=None, box=None):
+    s, box, info = cases.default_tunnel(case, storage, aoa, wall, coarse_flags=coarse_flags,
+                                        fine_flags=fine_flags, box=box)
+    if before_steps is not None:
+        before_steps(s)
+    for k in range(calls):
+        s.step_n(per_call)
+        s.particles_advect(float(per_call), k + 1)
+    s.synchronize()
+    return s, box, info
+
+
+def test_c3_full_configuration_storages_against_fp64(gpu):
+    """Config 3 at AoA 8: 512 x 192 x 240 wing, factor-2 patch (864 x 248 x
+    480 fine cells), wall model on both levels, 1 M particles, 400 coarse
+    steps (800 fine sub-steps) in FP64, FP32 and FP16C. Every run stays
+    finite with the watchdog clear; the coarse velocity field and the fine-
+    grid force of FP32 and FP16C are compared with the FP64 run."""
+    case = cases.config3_wing(8.0)
+    out = {}
+    for st in ("f64", "f32", "f16c"):
+        say(f"case: config 3 full configuration AoA 8, {st}, 400 coarse steps")
+        s, box, info = run_default_tunnel(case, st, 8.0, 4)
+        assert not s.nan_detected(), s.nan_kind()
+        assert info["active"] and info["forces_from_fine"]
+        _, u, v, w = s.read_macro()
+        F = s.force_now()
+        wr = s.wall_model_readout()
+        p = s.particles_read()
+        assert np.isfinite(p).all()
+        out[st] = (np.stack([u, v, w]).astype(np.float64), F)
+        say(f"  {st}: force {F[:2]}, wall cells {wr.cells} (mean y+ {wr.mean_yplus:.2f}), "
+            f"fine {info['fine_dims']}")
+        s.close()
+    fluid = case.flags == fc.FLUID
+    ref_u, ref_f = out["f64"]
+    bounds = {"f32": (1e-4, 1e-4), "f16c": (2e-2, 2e-2)}
+    for st, (ub, fb) in bounds.items():
+        uu, F = out[st]
+        eu = rel_l2(uu[:, fluid], ref_u[:, fluid])
+        ef = float(np.abs(F[:2] - ref_f[:2]).max() / np.abs(ref_f[:2]).max())
+        say(f"{st} vs f64 after 400 coarse steps: velocity rel L2 {eu:.3e}, force {ef:.3e}")
+        assert eu < ub and ef < fb
+
+
+def harvest_config2_samples(target=1_000_000, per_call=20, start=400, max_calls=400):
+    """Wall-cell samples (y_s, |u_t|, nu, Newton u_tau) from the config-2
+    tunnel (384 x 128 x 64, chord 128, AoA 8, Cs 0.1, the physical scaling),
+    one harvest per call of per_call steps after `start` steps."""
+    case = cases.config2_tunnel(8.0)
+    s = cases.make_solver(case, "f32")
+    s.set_wall_model("newton")
+    s.step_n(start)
+    s.set_wall_harvest(True)
+    got, n = [], 0
+    for _ in range(max_calls):
+        s.step_n(per_call)
+        if s.nan_detected():
+            break
+        h = s.wall_harvest()
+        h = h[h[:, 3] > 0]
+        got.append(h)
+        n += len(h)
+        if len(got) % 10 == 0:
+            say(f"  harvest call {len(got)}: {n} samples")
+        if n >= target:
+            break
+    s.close()
+    return np.concatenate(got)
+
+
+def test_c4_vg_array_wing_learned_wall_fp16c(gpu):
+    """Config 4: the 23.6 M-cell wing at 10 deg with 24 vanes (12 counter-
+    rotating pairs) at x/c = 0.2, height 0.008 c, length 2.5 h, +-16 deg,
+    pair spacing 6 h, voxelized into both levels (vg.cpp:246-261); the fine
+    level's vane links with synthetic cut fractions q ~ U[0, 1) from a
+    fixed-seed lowbias32 stream, kept by the densify rule; the learned wall
+    function trained on about 1e6 samples harvested from a config-2 tunnel;
+    FP16C storage. The run stays finite, every kept link is active, the
+    network serves wall cells and the vanes change the fine flow."""
+    aoa = 10.0
+    case = cases.config3_wing(aoa)
+    smp = harvest_config2_samples()
+    say(f"case: config 4, network trained on {len(smp)} harvested config-2 samples")
+    assert len(smp) >= 1_000_000
+    tr = mlp.MlpTrainer(smp, seed=11)
+    losses = tr.fit(2000, 1024, 3e-3, 1e-4)
+    prm = tr.export()
+    tr.close()
+    say(f"  training loss {float(np.mean(losses[:20])):.3e} -> {float(np.mean(losses[-20:])):.3e}")
+    assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:20])
+
+    nx, ny, nz = case.dims
+    pts, le = S.generate_naca4("2412", 200)
+    h = 0.008
+    vg = S.vg_params(S.VG_PAIR, x_c=0.2, height_c=h, length_h=2.5, beta_deg=16.0, pitch_c=6 * h, gap_h=2.5,
+                     count=12)
+    layout = S.make_layout(nx, ny, nz, case.scaling.chord_cells)
+    coarse_vg, coarse_slabs = S.build_flags_with_vgs(pts, le, aoa, layout, [vg], case.flags)
+    box, clean_fine = cases.default_patch(cases.Case(case.name, case.dims, coarse_vg, case.scaling), aoa)
+    fl = S.make_fine_layout(layout, S.FcPatchBox(*box), 2)
+    fine_vg, slabs = S.build_flags_with_vgs(pts, le, aoa, fl, [vg], clean_fine)
+    assert len(slabs) == 24 and len(coarse_slabs) == 24
+    fd = (fl.dims.nx, fl.dims.ny, fl.dims.nz)
+    added = int(((fine_vg == fc.SOLID) & (clean_fine != fc.SOLID)).sum())
+    qf, ff, links = S.build_vane_qlinks(fd, fine_vg, clean_fine, slabs)
+    qs, fs, kept = cases.synthetic_vane_q(qf, ff)
+    say(f"  {added} fine vane cells, {links} vane links, {kept} kept with synthetic q")
+    assert added > 24 * 20 and kept > 0.9 * links
+    b = qs[qs != 0].astype(np.float64) / 255.0
+    assert b.min() >= 0.05 - 1.0 / 255 and abs(b.mean() - 0.525) < 0.02  # U[0.05, 1) after the rule
+
+    def arm(s):
+        s.set_wall_mlp(prm)
+        s.set_wall_model("mlp")
+        s.set_qlibb_enabled(True)
+        s.set_fine_qlinks(qs, fs, kept, 0)
+
+    s, box2, info = run_default_tunnel(case, "f16c", aoa, 3, coarse_flags=coarse_vg, fine_flags=fine_vg,
+                                       before_steps=arm)
+    assert tuple(box2) == tuple(box)
+    assert not s.nan_detected(), s.nan_kind()
+    r = s.qlibb_readout()
+    wr = s.wall_model_readout()
+    F = s.force_now()
+    say(f"  q-LIBB links {r['links']} active {r['active']}; wall readout mode {wr.mode} cells {wr.cells} "
+        f"network-served {wr.mlp_cells} excluded by vanes {wr.excluded_vg}; force {F[:2]}")
+    assert r["links"] == kept and r["active"] > 0.9 * kept
+    assert wr.mode == S.WALL_MLP and wr.mlp_cells > 0 and wr.excluded_vg > 0
+    assert np.isfinite(F).all()
+    vg_fine = s.download_fine_f(f64=False).reshape(S.Q, -1)
+    s.close()
+    p, _, _ = run_default_tunnel(case, "f16c", aoa, 3, fine_flags=clean_fine, box=box,
+                                 before_steps=lambda s: (s.set_wall_mlp(prm), s.set_wall_model("mlp")))
+    plain_fine = p.download_fine_f(f64=False).reshape(S.Q, -1)
+    p.close()
+    both = (fine_vg == fc.FLUID) & (clean_fine == fc.FLUID)
+    d = rel_l2(vg_fine[:, both].astype(np.float64), plain_fine[:, both].astype(np.float64))
+    say(f"  vanes change the fine populations by {d:.3e} (rel L2 over fluid cells)")
+    assert d > 1e-5
+
+
+def test_c5_polar_group_equals_individual_stepping(gpu):
+    """Config 5 at its stated size: 16 NACA 2412 cases, AoA -4..16 deg,
+    chords from the fixed-seed draw in [96, 192], grids 2c x 0.75c x 0.9c,
+    FP16C, Newton wall model. The batched group step (one launch per step,
+    graph replay) leaves every member exactly where stepping it alone does."""
+    from foil_cfd_amd import group
+    cs = group.polar_sweep(16)
+    cells = sum(c.flags.size for c in cs)
+    say(f"case: config 5 polar, 16 cases, {cells / 1e6:.1f} M cells, chords "
+        f"{sorted(c.scaling.chord_cells for c in cs)}")
+    assert all(96 <= c.scaling.chord_cells <= 192 for c in cs)
+    grouped = [cases.make_solver(c, "f16c") for c in cs]
+    for s in grouped:
+        s.set_wall_model("newton")
+    g = group.SolverGroup(grouped)
+    g.step(60)
+    g.step(41)
+    for s in grouped:
+        s.synchronize()
+    g.close()
+    for k, (c, sg) in enumerate(zip(cs, grouped)):
+        alone = cases.make_solver(c, "f16c")
+        alone.set_wall_model("newton")
+        alone.step_n(60)
+        alone.step_n(41)
+        alone.synchronize()
+        for a, b in zip(sg.read_macro(), alone.read_macro()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(sg.force_raw()[0], alone.force_raw()[0])
+        if k in (0, 15):
+            fa, fb = sg.download_f(), alone.download_f()
+            same = np.isnan(fa) & np.isnan(fb)
+            assert not np.count_nonzero((fa != fb) & ~same)
+        assert not sg.nan_detected()
+        alone.close()
+        sg.close()
