@@ -481,7 +481,8 @@ stepKernelGroupMixed(const GroupEntry* __restrict__ tab, GroupLaunch g) {
 // ---------------------------------------------------------------------------
 // Init: f = feq(1, (u0,0,0)) (solid cells at rest) written in layout L, plus
 // both halves of the outlet feed planes (reference initEquilibriumKernel,
-// lbm_core.cu:102-118).
+// lbm_core.cu:102-118, run on both ping-pong buffers, lbm_solver.cpp:931-935):
+// a feed cell that never collides keeps this state in both halves.
 // ---------------------------------------------------------------------------
 template <class S>
 __global__ void __launch_bounds__(kBlock) initKernel(KArgs a, float u0) {
@@ -501,7 +502,11 @@ __global__ void __launch_bounds__(kBlock) initKernel(KArgs a, float u0) {
     for (int q = 0; q < kQ; ++q) {
         const C h = heqOf<C>(q, K, C(0), C(1), ux, C(0), C(0), usq);
         S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
-        if (k & kKindFeed) S::store(static_cast<T*>(a.feed) + q * plane + yz, h);  // read from layout L
+        if (k & kKindFeed) {
+            T* fd = static_cast<T*>(a.feed);
+            S::store(fd + q * plane + yz, h);
+            S::store(fd + (kQ + q) * plane + yz, h);
+        }
     }
 }
 
@@ -564,11 +569,7 @@ __global__ void __launch_bounds__(kBlock) uploadKernel(KArgs a, const CT* canon)
     for (int q = 0; q < kQ; ++q) {
         const C h = K * (C(canon[static_cast<long long>(q) * a.ncells + c]) - wOf<C>(q));
         S::store(f + static_cast<long long>(kOpp[q]) * a.ncells + c, h);
-        if (k & kKindFeed) {
-            T* fd = static_cast<T*>(a.feed);
-            S::store(fd + q * plane + yz, h);
-            S::store(fd + (kQ + q) * plane + yz, h);
-        }
+        if (k & kKindFeed) S::store(static_cast<T*>(a.feed) + q * plane + yz, h);  // read from layout L
     }
 }
 

@@ -116,11 +116,12 @@ def test_m0_taylor_green_against_the_oracle(gpu):
     s.synchronize()
     got = s.download_f()
     s.close()
-    # the startup ramps run from the first step, as after the reference's
-    # uploadF; FP64 oracle: the GPU's FP32 store of f - w carries far less
-    # rounding than the FP32 oracle's store of f, so the exact arithmetic is
-    # the bar (the FP32 oracle drifts 2e-4 from it here)
-    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, n, 1000)
+    # the upload ends the startup ramps, as the reference's uploadF does
+    # (notifySnapshotRestored, tests/lbm_test_util.h:145); FP64 oracle: the
+    # GPU's FP32 store of f - w carries far less rounding than the FP32
+    # oracle's store of f, so the exact arithmetic is the bar (the FP32
+    # oracle drifts 2e-4 from it here)
+    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, n, 1000, ramp=False)
     f = f0.astype(np.float64)
     other = np.repeat(np.asarray(S.W, np.float64)[:, None], n ** 3, axis=1).reshape(-1).copy()
     orc.run(f, flags, (n, n, n), taus.astype(np.float64), uins.astype(np.float64), cs=0.1733, macro=False,
@@ -131,7 +132,10 @@ def test_m0_taylor_green_against_the_oracle(gpu):
               np.linalg.norm(np.stack(mo[1:])[:, fl]))
     kg, ko = kinetic_energy(got, flags, n ** 3), kinetic_energy(f, flags, n ** 3)
     say(f"  KE(1000) gpu {kg:.6f} oracle {ko:.6f}; velocity rel L2 {e:.2e}")
-    assert e < 1e-5 and abs(kg / ko - 1) < 1e-5
+    # FP32 arithmetic leaves ~1e-5; running the ramps, or holding the copied
+    # upload in the never-written corner storage instead of the rest state,
+    # moves KE(1000) by 7e-2 and 2.5e-3
+    assert e < 1e-4 and abs(kg / ko - 1) < 1e-4
 
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])
@@ -210,9 +214,9 @@ def test_m1_cavity_against_the_oracle(gpu):
     s.synchronize()
     _, u, v, _ = s.read_macro()
     s.close()
-    # ramps on from the first step (lid speed and viscosity), as in the
-    # reference's run; the FP32 oracle drifts 8e-4 from the FP64 one here
-    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, dims[0], 4000)
+    # no ramps after the upload (lid at full speed from the first step); the
+    # FP32 oracle drifts 8e-4 from the FP64 one here
+    taus, uins = orc.ramp_schedule(sc.tau, sc.nu_lat, sc.u_lat, dims[0], 4000, ramp=False)
     f = rest.copy()
     _, uo, vo, _ = orc.run(f, flags, dims, taus.astype(np.float64), uins.astype(np.float64), cs=0.1733,
                            other=rest.copy())
@@ -220,7 +224,7 @@ def test_m1_cavity_against_the_oracle(gpu):
     e = float(np.linalg.norm(np.stack([u, v])[:, fl] - np.stack([uo, vo])[:, fl]) /
               np.linalg.norm(np.stack([uo, vo])[:, fl]))
     say(f"  velocity rel L2 vs the FP64 oracle {e:.2e}")
-    assert e < 1e-5
+    assert e < 1e-4   # the ramped lid would give 2.4e-2
 
 
 def test_m1_lid_driven_cavity_ghia(gpu):
