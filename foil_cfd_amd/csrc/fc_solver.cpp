@@ -665,26 +665,37 @@ public:
         return rebuildCoarseWall();
     }
 
+    // Every level with a wall list harvests while enabled; the samples read
+    // back are the readout level's (the finest level with wall physics).
     int setWallHarvest(bool on) {
         harvestOn_ = on;
         if (!initialized_) return FC_OK;
-        hipRelease(wm_.dHarvest);
-        if (on && wm_.count > 0 &&
-            hipAlloc(&wm_.dHarvest, static_cast<std::size_t>(wm_.count) * 4 * sizeof(float)) != FC_OK)
-            return fail(FC_ERR_ALLOC, "harvest buffer");
+        WMLevel* levels[3] = {&wm_, fine_.active ? &fine_.wm : nullptr, finer_.active ? &finer_.wm : nullptr};
+        for (WMLevel* w : levels) {
+            if (!w) continue;
+            hipRelease(w->dHarvest);
+            if (on && w->active && w->count > 0 &&
+                hipAlloc(&w->dHarvest, static_cast<std::size_t>(w->count) * 4 * sizeof(float)) != FC_OK)
+                return fail(FC_ERR_ALLOC, "harvest buffer");
+        }
         return FC_OK;
+    }
+
+    const WMLevel& readoutLevel() const {
+        return (finer_.active && finer_.wm.active) ? finer_.wm : (fine_.active && fine_.wm.active) ? fine_.wm : wm_;
     }
 
     int wallHarvest(float* out, int cap) {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
-        if (!wm_.dHarvest) return 0;
-        const int n = std::min(cap, wm_.count);
+        const WMLevel& wm = readoutLevel();
+        if (!wm.dHarvest) return 0;
+        const int n = std::min(cap, wm.count);
         if (out && n > 0 &&
-            (hipMemcpyAsync(out, wm_.dHarvest, static_cast<std::size_t>(n) * 4 * sizeof(float),
+            (hipMemcpyAsync(out, wm.dHarvest, static_cast<std::size_t>(n) * 4 * sizeof(float),
                             hipMemcpyDeviceToHost, stream_) != hipSuccess ||
              hipStreamSynchronize(stream_) != hipSuccess))
             return -fail(FC_ERR_DEVICE, "harvest readback failed");
-        return out ? n : wm_.count;
+        return out ? n : wm.count;
     }
 
     int setMlp(const MlpParams* host) {
@@ -707,9 +718,7 @@ public:
         r->mode = wmMode_;
         r->enabled = wmMode_ != kWallOff ? 1 : 0;
         // the level with the finest wall physics (lbm_solver.cpp:994-1010)
-        const WMLevel& wm = (finer_.active && finer_.wm.active) ? finer_.wm
-                            : (fine_.active && fine_.wm.active) ? fine_.wm
-                                                                : wm_;
+        const WMLevel& wm = readoutLevel();
         r->from_fine = &wm != &wm_ ? 1 : 0;
         r->excluded_vg = wm.stats.excludedVG;
         r->degenerate = wm.stats.degenerate;
@@ -1491,7 +1500,7 @@ private:
     }
     int rebuildLevelWall(FineLevel& lv) {
         return buildWallLevel(lv.wm, lv.dims, lv.hostFlags, lv.surfaceRef, kWallSampleCellsFine, lv.hostKind, lv.kind,
-                              false);
+                              harvestOn_);
     }
     // Every level (setWallModelEnabled: coarse, and the patch levels when active).
     int rebuildWallModel() {
@@ -1527,14 +1536,14 @@ private:
                 p.nuLat = (tauF - 0.5f) / 3.0f;
                 WMLevel& w = fine_.wm;
                 if (launchWallUpdate(viewOf(fine_), fine_.layout, w.view(), w.dUwx, w.dUwy, w.dUwz, p, w.dStats,
-                                     nullptr, stream_) != 0)
+                                     w.dHarvest, stream_) != 0)
                     return fail(FC_ERR_DEVICE, "fine wall-model update failed");
             }
             if (finer_.active && finer_.wm.active) {
                 p.nuLat = (fineTauFor(tauF, finer_.factor) - 0.5f) / 3.0f;
                 WMLevel& w = finer_.wm;
                 if (launchWallUpdate(viewOf(finer_), finer_.layout, w.view(), w.dUwx, w.dUwy, w.dUwz, p, w.dStats,
-                                     nullptr, stream_) != 0)
+                                     w.dHarvest, stream_) != 0)
                     return fail(FC_ERR_DEVICE, "finer wall-model update failed");
             }
         }

@@ -400,7 +400,10 @@ int fc_set_refined_surface_reference(fc_solver* s, const uint8_t* clean_fine_fla
 int fc_set_refined_finer_surface_reference(fc_solver* s, const uint8_t* clean_finer_flags);
 /* Harvest training samples (y_s, |u_t|, nu, Newton u_tau; u_tau < 0 = none)
  * from every update while enabled; fc_wall_harvest copies the last update's
- * samples (4 floats each) and returns the count (out NULL: count only). */
+ * samples (4 floats each) of the readout level (the finest level with wall
+ * physics, as fc_get_wall_model_readout) and returns the count (out NULL:
+ * count only). No reference counterpart: training data of the learned wall
+ * function. */
 int fc_set_wall_harvest(fc_solver* s, int enabled);
 int fc_wall_harvest(fc_solver* s, float* out, int cap);
 /* Load (or with NULL, drop) the learned u_tau network used by FC_WALL_MLP. */
