@@ -835,6 +835,8 @@ __global__ void __launch_bounds__(kBlock) watchdogKernel(KArgs a, int layout, lo
     const long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long cell = t * 4093 + offset;
     if (cell >= a.ncells) return;
+    // a refined level's shell holds mixed-time slots that are never read
+    if ((a.kind[cell] & kKindFlagMask) == kInterface) return;
     const int z = static_cast<int>(cell / (static_cast<long long>(a.nx) * a.ny));
     const int rem = static_cast<int>(cell - static_cast<long long>(z) * a.nx * a.ny);
     const int y = rem / a.nx;

@@ -185,8 +185,9 @@ __device__ inline bool shellCell(long long t, const RArgs& r, int& x, int& y, in
 // links whose target is Solid. So the shell fill stores f*_q(x) only when
 // x + c_q lies inside the shell: the 5 inward populations of the inner ring
 // (x or y = 1, n - 2), nothing of the outer ring (x or y = 0, n - 1). The
-// other shell slots keep older values; a download shows Interface cells as
-// NaN either way.
+// other shell slots keep older, finite values: a level download returns them
+// as they are (compare levels on their Fluid cells), and the NaN watchdog
+// skips Interface cells.
 // ---------------------------------------------------------------------------
 __device__ inline bool insideShell(int x, int y, const RArgs& r) {
     return x >= 2 && y >= 2 && x < r.fnx - 2 && y < r.fny - 2;

@@ -380,3 +380,30 @@ def test_particle_respawn_rules():
     assert not (20 <= pos[1, 0] < 21 and 10 <= pos[1, 1] < 11 and 5 <= pos[1, 2] < 6)
     assert 0.6 <= pos[1, 3] <= 1.0 and 0.6 <= pos[2, 3] <= 1.0
     assert pos[3, 0] == 30.0 and pos[3, 3] == pytest.approx(0.498)
+
+
+@pytest.mark.parametrize("tau,caps", [(0.5005, True), (0.52, False)])
+def test_low_viscosity_tunnel_reaches_speed_cap(tau, caps):
+    """The reference step at the physical relaxation time of the wing cases
+    (tau 0.5005, the clamp of units.h) on the 128 x 48 x 16 tunnel (chord 40,
+    AoA 8, Cs 0.1, u_lat 0.08, startup ramps) drives the fastest fluid cell
+    to the 0.30 speed cap of the collision (lbm_core.cu:398) within 400
+    steps; at tau 0.52 it stays near the free stream. This is why the
+    time-averaged coefficient tests of the 0.1-Smagorinsky tunnels run at
+    tau 0.52."""
+    from foil_cfd_amd import cases
+    import foil_cfd_amd as fc
+    case = cases.naca_tunnel(128, 48, 16, 40, aoa=8.0, cs=0.1, u_lat=0.08, tau=tau)
+    f = orc.init_equilibrium(case.flags, case.dims, 0.0)
+    orc.spanwise_perturb(f, case.flags, case.dims, 1e-3 * case.scaling.u_lat)
+    fl = case.flags == fc.FLUID
+    top = 0.0
+    for c in range(4):
+        taus, uins = orc.ramp_schedule(case.scaling.tau, case.scaling.nu_lat, case.scaling.u_lat, 128, 100,
+                                       start=100 * c)
+        _, u, v, w = orc.run(f, case.flags, case.dims, taus, uins, cs=0.1)
+        top = max(top, float(np.sqrt(u * u + v * v + w * w)[fl].max()))
+    if caps:
+        assert top >= np.float32(0.30) * (1 - 1e-6)
+    else:
+        assert top < 0.1
