@@ -28,7 +28,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"f16": 76.0, "f16c": 76.0, "bf16": 76.0, "f32": 152.0, "f64": 304.0}
-RECORD_BYTES = 96.0  # restriction record per coarse cell: 19 fneq, valid mark, 4 moments (FP32)
+# restriction record per coarse cell: 19 fneq, valid mark, 4 moments; FP32 for FP32 / FP64 storage,
+# fneq and the mark as IEEE halves beside FP32 moments for the 16-bit storages
+RECORD_BYTES = {"f16": 64.0, "f16c": 64.0, "bf16": 64.0, "f32": 96.0, "f64": 96.0}
 METRIC = "GLUPS & % HBM peak, 23.6M-cell wing, FP16-store vs FP32 at ≤1% Cl/Cd err"
 REFERENCE_PUBLISHED_GLUPS_FP32 = 8.151  # RTX 5090, 23.6M cells, FP32 (notes/PERF.md:10-20)
 
@@ -138,7 +140,7 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
     # cell of the box shrunk by 2): counted as half a record set per launch.
     x0, y0, x1, y1 = box
     n_rec = (x1 - x0 - 4) * (y1 - y0 - 4) * case.dims[2]
-    rec_bytes = RECORD_BYTES * n_rec / 2 if variant == "auto" else 0.0
+    rec_bytes = RECORD_BYTES[storage] * n_rec / 2 if variant == "auto" else 0.0
     per_level = {}
     for name, (launches, ms), cells, extra in (("coarse", lv[0], n_c, 0.0), ("fine", lv[1], n_f, rec_bytes)):
         avg = ms / launches if launches else float("nan")
@@ -459,6 +461,64 @@ def polar_bench(args):
     print(json.dumps(out), flush=True)
 
 
+PROBE_PROFILE = os.path.join(ROOT, "profiles", "aa_traffic_probe.json")
+
+
+def traffic_probe(reps_coarse=200, reps_fine=50):
+    """Traffic-only ceiling of the in-place step's access pattern on this GPU
+    (fc_probe_aa_traffic: the step kernels' decomposition moving 19 reads and
+    19 writes per cell, no collision) for the coarse wing and the fine patch
+    grid, 2- and 4-byte populations, the local (S->L) and shifted (L->S)
+    step, with and without 20 FP32 FMAs per population in between; plus a
+    plain device copy. One JSON line; rates in TB/s of algorithmic bytes."""
+    import ctypes as C
+
+    import foil_cfd_amd as fc
+    from foil_cfd_amd._abi import FcDims
+    lib = fc.load()
+    rows = []
+    for dims, reps in (((512, 192, 240), reps_coarse), ((864, 248, 480), reps_fine)):
+        n = dims[0] * dims[1] * dims[2]
+        for eb in (2, 4):
+            for shifted in (0, 1):
+                for work in (0, 20):
+                    ms = C.c_double(0.0)
+                    rc = lib.fc_probe_aa_traffic(FcDims(*dims), eb, shifted, work, reps, C.byref(ms))
+                    if rc != 0:
+                        raise RuntimeError(f"fc_probe_aa_traffic failed ({rc})")
+                    tbs = 2.0 * 19 * eb * n / (ms.value * 1e-3) / 1e12
+                    rows.append({"dims": list(dims), "bytes_per_pop": eb, "pattern": "L->S" if shifted else "S->L",
+                                 "fma_per_pop": work, "ms": round(ms.value, 5), "tb_s": round(tbs, 3)})
+                    log(f"[probe] {dims} {eb} B {'L->S' if shifted else 'S->L'} fma {work}: "
+                        f"{ms.value * 1e3:.1f} us {tbs:.3f} TB/s")
+    copies = []
+    for gib in (1, 4):
+        ms = C.c_double(0.0)
+        if lib.fc_probe_copy(gib << 30, 50, C.byref(ms)) != 0:
+            raise RuntimeError("fc_probe_copy failed")
+        tbs = 2.0 * (gib << 30) / (ms.value * 1e-3) / 1e12
+        copies.append({"gib": gib, "ms": round(ms.value, 5), "tb_s": round(tbs, 3)})
+        log(f"[probe] copy {gib} GiB: {ms.value * 1e3:.1f} us {tbs:.3f} TB/s")
+    print(json.dumps({"probe": "aa_traffic", "peak_tb_s": HBM_PEAK_GBS / 1e3, "aa": rows, "copy": copies}),
+          flush=True)
+
+
+def probe_line():
+    """The committed traffic-only probe, reduced to the 16-bit ceilings of the
+    two step patterns on the coarse and the fine grid (TB/s)."""
+    try:
+        with open(PROBE_PROFILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    out = {}
+    for r in rec.get("aa", []):
+        if r["fma_per_pop"] == 0:
+            key = f"{'coarse' if r['dims'][0] == 512 else 'fine'}_{r['bytes_per_pop']}B_{r['pattern']}"
+            out[key] = r["tb_s"]
+    return {"tb_s": out, "source": "profiles/aa_traffic_probe.json"} if out else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,6 +533,8 @@ def main():
                     help="step kernel: auto (library default), single or buf")
     ap.add_argument("--coarse-only", action="store_true",
                     help="bare coarse wing (no refinement patch, wall model or particles)")
+    ap.add_argument("--periodic-box", action="store_true",
+                    help="with --coarse-only: an all-fluid periodic box of the wing's size (bulk step cost)")
     ap.add_argument("--polar", action="store_true",
                     help="batched 16-case polar sweep instead of the wing (separate JSON line)")
     ap.add_argument("--polar-storage", default="f16c")
@@ -489,6 +551,8 @@ def main():
                     help="fold --accuracy records into the committed accuracy summary (no GPU)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--probe", action="store_true",
+                    help="traffic-only ceiling of the step's access pattern (separate JSON line)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -498,6 +562,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         copy_calibration()
+        return
+    if args.probe:
+        traffic_probe()
         return
     if args.accuracy_summary:
         accuracy_summary(args.accuracy_summary, ACCURACY_PROFILE)
@@ -514,6 +581,8 @@ def main():
     fc.load()
     t_setup = time.perf_counter()
     case = cases.config3_wing(args.aoa)
+    if args.coarse_only and args.periodic_box:
+        case = cases.periodic_box(case)
     log(f"[bench] case {case.name}: flags built in {time.perf_counter() - t_setup:.1f} s")
 
     storages = [s.strip() for s in args.storages.split(",") if s.strip()]
@@ -534,6 +603,8 @@ def main():
              "f64": "fp64"}[head]
     if args.coarse_only:
         workload = f"naca2412_wing_512x192x240_chord256_aoa{args.aoa:g}_trt_les_{head}store_coarse_only"
+        if args.periodic_box:
+            workload = f"periodic_box_512x192x240_trt_les_{head}store"
         config = {"workload": workload, "cells": ncells, "storage": store,
                   "not_included": "refinement patch, wall model, particles (coarse grid only)"}
         kernel = "stepKernel (fused stream-collide), coarse wing"
@@ -588,6 +659,7 @@ def main():
         "reference_published_glups_fp32": REFERENCE_PUBLISHED_GLUPS_FP32,
     }
     out["cl_cd_err_vs_fp64"] = accuracy_line()
+    out["traffic_probe_ceiling"] = probe_line()
     if head in ("f16c", "f16") and "f32" in results:
         out["fp16_vs_fp32_speedup"] = round(results[head]["glups"] / results["f32"]["glups"], 3)
     print(json.dumps(out), flush=True)

@@ -344,6 +344,8 @@ SIGNATURES = {
     "fc_kernel_timing": (C.c_int, [_vp, _ip, _dp]),
     "fc_kernel_timing_level": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "fc_probe_aa_traffic": (C.c_int, [FcDims, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "fc_probe_copy": (C.c_int, [C.c_size_t, C.c_int, _dp]),
 }
 
 _LIB = None

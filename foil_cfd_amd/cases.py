@@ -94,6 +94,15 @@ def config3_wing(aoa: float = 8.0) -> Case:
     return naca_tunnel(512, 192, 240, 256, aoa=aoa, u_lat=0.08, cs=0.1733)
 
 
+def periodic_box(like: Case) -> Case:
+    """An all-fluid, fully periodic box with the dimensions, scaling and LES
+    constant of another case (no boundaries, no walls): the bulk cost of the
+    step on that grid."""
+    nx, ny, nz = like.dims
+    return Case(f"periodic_box_{nx}x{ny}x{nz}", like.dims, np.zeros(nx * ny * nz, dtype=np.uint8), like.scaling,
+                periodic_xy=True, cs=like.cs, ramp=False)
+
+
 def refinement_patch(case: Case, factor: int = 2, upstream_c: float = 0.35, wake_c: float = 1.0,
                      below_c: float = 0.45, above_c: float = 0.45, digits: str = "2412",
                      aoa: float = 8.0):

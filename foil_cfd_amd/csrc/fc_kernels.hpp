@@ -241,9 +241,12 @@ struct ParticleParams {
 
 // pos4: 4 floats per particle (x, y, z in lattice-cell space, age in [0, 1]).
 int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, void* stream);
-// kind: the solver's kind bytes (low nibble = cell flag) or null.
+// kind: the solver's kind bytes (low nibble = cell flag) or null. vel4: null,
+// or device scratch of 4 floats per cell that first receives (u, v, w, 0) of
+// every cell so each trilinear corner is one 16-byte read (same result).
 int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
-                         const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, void* stream);
+                         const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, float* vel4,
+                         void* stream);
 
 // ---- q-LIBB pre-pass ----------------------------------------------------------
 // Links: cell index + packed (q | byte << 8 | ff << 16); scratch: count doubles.

@@ -320,7 +320,7 @@ stepKernelBuf(KArgs a) {
 // therefore bit-identical to that pass, which this launch replaces.
 // ---------------------------------------------------------------------------
 struct RecArgs {
-    float4* rec;  // kRecordPlanes planes of nR records
+    float4* rec;  // nR records (full or compact format, by storage)
     long long nR;
     int cx0, cx1;  // restricted coarse columns, relative to the box
     int cy0, cy1;  // restricted coarse rows, relative to the box
@@ -399,9 +399,7 @@ __global__ void __launch_bounds__(kBlock) stepRecordKernel(KArgs a, RecArgs rg) 
         (cx - rg.cx0) + static_cast<long long>(rg.rw) * ((yp - rg.cy0) + static_cast<long long>(rg.rh) * zp);
     float rec[4 * kRecordPlanes];
     childRecord(h, nf, rec);
-#pragma unroll
-    for (int c = 0; c < kRecordPlanes; ++c)
-        rg.rec[c * rg.nR + ri] = make_float4(rec[4 * c], rec[4 * c + 1], rec[4 * c + 2], rec[4 * c + 3]);
+    storeRecord<RecordFormat<S>::kCompact>(rg.rec, rg.nR, ri, rec);
 }
 
 // ---------------------------------------------------------------------------

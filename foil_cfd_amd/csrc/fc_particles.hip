@@ -42,17 +42,34 @@ __device__ inline V3 lerp3(V3 a, V3 b, float t) {
     return V3{a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t};
 }
 
-__device__ inline V3 velAt(const float* u, const float* v, const float* w, const Dims& d, int x, int y,
-                           int z) {
+// Velocity sources of the sampler: the three macro arrays, or one packed
+// (u, v, w, 0) vector per cell so that a corner is a single 16-byte read
+// instead of three scattered 4-byte reads. Same values either way.
+struct VelSoA {
+    const float* u;
+    const float* v;
+    const float* w;
+    __device__ V3 at(long long c) const { return V3{u[c], v[c], w[c]}; }
+};
+struct VelPacked {
+    const float4* vel;
+    __device__ V3 at(long long c) const {
+        const float4 t = vel[c];
+        return V3{t.x, t.y, t.z};
+    }
+};
+
+template <class VS>
+__device__ inline V3 velAt(const VS& vs, const Dims& d, int x, int y, int z) {
     int zz = z;
     if (zz < 0) zz += d.nz;
     else if (zz >= d.nz) zz -= d.nz;
-    const long long c = cellAt(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), zz);
-    return V3{u[c], v[c], w[c]};
+    return vs.at(cellAt(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), zz));
 }
 
 // Trilinear sample at p - 0.5 (cell centres at +0.5), x / y clamped, z periodic.
-__device__ inline V3 sampleVel(const float* u, const float* v, const float* w, const Dims& d, V3 p) {
+template <class VS>
+__device__ inline V3 sampleVel(const VS& vs, const Dims& d, V3 p) {
     const float gx = fminf(fmaxf(p.x - 0.5f, 0.0f), static_cast<float>(d.nx) - 1.0001f);
     const float gy = fminf(fmaxf(p.y - 0.5f, 0.0f), static_cast<float>(d.ny) - 1.0001f);
     float gz = p.z - 0.5f;
@@ -61,10 +78,10 @@ __device__ inline V3 sampleVel(const float* u, const float* v, const float* w, c
     const float fx = gx - static_cast<float>(x0);
     const float fy = gy - static_cast<float>(y0);
     const float fz = gz - static_cast<float>(z0);
-    const V3 c000 = velAt(u, v, w, d, x0, y0, z0), c100 = velAt(u, v, w, d, x0 + 1, y0, z0);
-    const V3 c010 = velAt(u, v, w, d, x0, y0 + 1, z0), c110 = velAt(u, v, w, d, x0 + 1, y0 + 1, z0);
-    const V3 c001 = velAt(u, v, w, d, x0, y0, z0 + 1), c101 = velAt(u, v, w, d, x0 + 1, y0, z0 + 1);
-    const V3 c011 = velAt(u, v, w, d, x0, y0 + 1, z0 + 1), c111 = velAt(u, v, w, d, x0 + 1, y0 + 1, z0 + 1);
+    const V3 c000 = velAt(vs, d, x0, y0, z0), c100 = velAt(vs, d, x0 + 1, y0, z0);
+    const V3 c010 = velAt(vs, d, x0, y0 + 1, z0), c110 = velAt(vs, d, x0 + 1, y0 + 1, z0);
+    const V3 c001 = velAt(vs, d, x0, y0, z0 + 1), c101 = velAt(vs, d, x0 + 1, y0, z0 + 1);
+    const V3 c011 = velAt(vs, d, x0, y0 + 1, z0 + 1), c111 = velAt(vs, d, x0 + 1, y0 + 1, z0 + 1);
     const V3 x00 = lerp3(c000, c100, fx), x10 = lerp3(c010, c110, fx);
     const V3 x01 = lerp3(c001, c101, fx), x11 = lerp3(c011, c111, fx);
     return lerp3(lerp3(x00, x10, fy), lerp3(x01, x11, fy), fz);
@@ -110,8 +127,15 @@ __global__ void __launch_bounds__(256) seedKernel(float4* pos, int count, Dims d
     pos[i] = make_float4(a, b, c, e);
 }
 
-__global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, const float* u, const float* v,
-                                                    const float* w, const std::uint8_t* kind, Dims d,
+// (u, v, w) of every cell into one 16-byte vector (read once per cell).
+__global__ void __launch_bounds__(256) packVelocityKernel(const float* u, const float* v, const float* w,
+                                                          float4* vel, long long n) {
+    const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c < n) vel[c] = make_float4(u[c], v[c], w[c], 0.0f);
+}
+
+template <class VS>
+__global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, VS vs, const std::uint8_t* kind, Dims d,
                                                     ParticleParams prm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -122,11 +146,11 @@ __global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, cons
     const float nzf = static_cast<float>(d.nz);
     bool blew = !finite3(p);
     if (!blew && prm.dtSteps > 0.0f) {
-        const V3 u1 = sampleVel(u, v, w, d, p);
+        const V3 u1 = sampleVel(vs, d, p);
         V3 pm{p.x + 0.5f * prm.dtSteps * u1.x, p.y + 0.5f * prm.dtSteps * u1.y, p.z + 0.5f * prm.dtSteps * u1.z};
         if (pm.z < 0.0f) pm.z += nzf;
         else if (pm.z >= nzf) pm.z -= nzf;
-        const V3 u2 = sampleVel(u, v, w, d, pm);
+        const V3 u2 = sampleVel(vs, d, pm);
         p = V3{p.x + prm.dtSteps * u2.x, p.y + prm.dtSteps * u2.y, p.z + prm.dtSteps * u2.z};
         blew = !finite3(p) || !finite3(u2);
         age -= prm.ageRate * prm.dtSteps;
@@ -159,11 +183,19 @@ int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, voi
 }
 
 int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
-                         const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, void* stream) {
+                         const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, float* vel4,
+                         void* stream) {
     if (!pos4 || count <= 0 || !u || !v || !w || d.cells() <= 0) return static_cast<int>(hipErrorInvalidValue);
     float4* pos = reinterpret_cast<float4*>(pos4);
-    advectKernel<<<(count + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(pos, count, u, v, w, kind, d,
-                                                                                     prm);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (vel4) {
+        float4* vel = reinterpret_cast<float4*>(vel4);
+        const long long n = d.cells();
+        packVelocityKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(u, v, w, vel, n);
+        advectKernel<VelPacked><<<(count + 255) / 256, 256, 0, st>>>(pos, count, VelPacked{vel}, kind, d, prm);
+    } else {
+        advectKernel<VelSoA><<<(count + 255) / 256, 256, 0, st>>>(pos, count, VelSoA{u, v, w}, kind, d, prm);
+    }
     return static_cast<int>(hipGetLastError());
 }
 

@@ -480,11 +480,10 @@ constexpr int kStagedTX = kStagedTile[0], kStagedTY = kStagedTile[1], kStagedTZ 
 // cell, lane l on fine column m (xc - x0) + l % m, so every population load
 // of the wave is one contiguous run of fine x. Each lane sums its m x m
 // children in (y, z), the m lanes of a cell are combined by cross-lane reads,
-// and the cell's first lane stores its record as six float4 planes over the
-// restricted box, plane c at scratch4[c nR + ri], ri = (xc - rx0) + rw ((yc -
-// ry0) + rh zc): planes 0..3 fneq[4c..4c+3] of fneq = h_avg - heq(h_avg),
-// plane 4 fneq[16..18] and the valid mark (1 when the cell has a non-solid
-// child), plane 5 the moments rho, ux, uy, uz.
+// and the cell's first lane stores its record (fneq = h_avg - heq(h_avg), the
+// valid mark -- 1 when the cell has a non-solid child -- and the moments rho,
+// ux, uy, uz) at ri = (xc - rx0) + rw ((yc - ry0) + rh zc) in the full or the
+// compact format of the storage (storeRecord).
 template <class S>
 __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const std::uint8_t* fkind, int fLayout,
                                                          RArgs r, float* scratch) {
@@ -534,10 +533,7 @@ __global__ void __launch_bounds__(kBlock) childFneqKernel(const void* ff, const 
     const long long ri = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
     float rec[4 * kRecordPlanes];
     childRecord(h, nf, rec);
-    float4* dst = reinterpret_cast<float4*>(scratch);
-#pragma unroll
-    for (int k = 0; k < kRecordPlanes; ++k)
-        dst[k * nR + ri] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    storeRecord<RecordFormat<S>::kCompact>(reinterpret_cast<float4*>(scratch), nR, ri, rec);
 }
 
 // Restriction of one coarse cell from its filtered fneq (already divided by
@@ -683,6 +679,129 @@ __global__ void __launch_bounds__(TX * TY * TZ) restrictStagedKernel(void* cf, c
                       mRho, mU, mV, mW);
 }
 
+// The staged filter over compact records (16-bit storages): the three
+// planes of eight halves are fetched into registers one at a time (the next
+// one in flight while the current is summed) and each is decoded once into
+// both LDS buffers, four values per buffer; the plane holding the valid mark
+// (fneq[16..18] + valid) first. Per value the neighbour order and the
+// arithmetic are those of the full-format filter, on the records' rounded
+// values; the sums are unscaled once at the end (an exact power of two).
+template <class S, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX * TY * TZ) restrictStagedCompactKernel(void* cf, const std::uint8_t* ckind,
+                                                                           int cLayout, RArgs r, float neqScaleInv,
+                                                                           float* mRho, float* mU, float* mV,
+                                                                           float* mW, const float4* scratch4) {
+    constexpr int kT = TX * TY * TZ;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, kH = HX * HY * HZ;
+    constexpr int kPer = (kH + kT - 1) / kT;
+    __shared__ float4 sRec[2][kH];
+    const int rx0 = r.x0 + kRestrictBand, ry0 = r.y0 + kRestrictBand;
+    const int rx1 = r.x1 - kRestrictBand, ry1 = r.y1 - kRestrictBand;
+    const int rw = rx1 - rx0, rh = ry1 - ry0;
+    const long long nR = static_cast<long long>(rw) * rh * r.cnz;
+    const int ntx = (rw + TX - 1) / TX, nty = (rh + TY - 1) / TY;
+    const long long nb = static_cast<long long>(ntx) * nty * ((r.cnz + TZ - 1) / TZ);
+    const long long lbk = xcdBlock(nb);
+    if (lbk >= nb) return;  // workgroup-uniform
+    const int b = static_cast<int>(lbk);
+    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    const int ox = rx0 + bx * TX, oy = ry0 + by * TY, oz = bz * TZ;
+    const int t = static_cast<int>(threadIdx.x);
+
+    // record index of each halo slot this thread stages (-1: outside the box)
+    long long src[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = t + j * kT;
+        src[j] = -1;
+        if (i < kH) {
+            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+            const int xc = ox + hx - 1, yc = oy + hy - 1;
+            const int zc = ((oz + hz - 1) % r.cnz + r.cnz) % r.cnz;
+            if (xc >= rx0 && xc < rx1 && yc >= ry0 && yc < ry1)
+                src[j] = (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc);
+        }
+    }
+    const uint4* const h4 = reinterpret_cast<const uint4*>(scratch4 + nR);
+    uint4 pre[kPer];
+    auto fetch = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) pre[j] = src[j] >= 0 ? h4[c * nR + src[j]] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            if (t + j * kT >= kH) continue;
+            const uint4 v = pre[j];
+            sRec[0][t + j * kT] = make_float4(recUnpack(v, 0), recUnpack(v, 1), recUnpack(v, 2), recUnpack(v, 3));
+            sRec[1][t + j * kT] = make_float4(recUnpack(v, 4), recUnpack(v, 5), recUnpack(v, 6), recUnpack(v, 7));
+        }
+    };
+
+    const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+    const int xc = ox + lx, yc = oy + ly, zc = oz + lz;
+    const int hc = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
+    const bool inBox = xc < rx1 && yc < ry1 && zc < r.cnz;
+    const long long cc = inBox ? cellOf(xc, yc, zc, r.cnx, r.cny) : 0;
+    const long long own = inBox ? (xc - rx0) + static_cast<long long>(rw) * ((yc - ry0) + static_cast<long long>(rh) * zc) : 0;
+    bool active = inBox && (ckind[cc] & kKindFlagMask) == kFluid;
+    const float4 mom = active ? scratch4[own] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    fetch(2);
+    stage();
+    __syncthreads();
+    unsigned nmask = 0;
+    if (active) {
+#pragma unroll
+        for (int n = 0; n < kQ; ++n)
+            if (sRec[0][hc + kCx[n] + HX * (kCy[n] + HY * kCz[n])].w != 0.0f) nmask |= 1u << n;
+        active = (nmask & 1u) != 0;
+    }
+    float fneq[kQ];
+    constexpr int kOrder[3] = {2, 0, 1};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k < 2) fetch(kOrder[k + 1]);
+        if (active) {
+#pragma unroll
+            for (int hs = 0; hs < 2; ++hs) {
+                if (kOrder[k] == 2 && hs == 1) continue;  // halves 4..7 of the valid plane are padding
+                const float4* sp = sRec[hs];
+                float4 a = sp[hc];
+#pragma unroll
+                for (int n = 1; n < kQ; ++n) {
+                    if (!(nmask & (1u << n))) continue;
+                    const float4 v = sp[hc + kCx[n] + HX * (kCy[n] + HY * kCz[n])];
+                    a.x += v.x;
+                    a.y += v.y;
+                    a.z += v.z;
+                    a.w += v.w;
+                }
+                const int q0 = 8 * kOrder[k] + 4 * hs;
+                fneq[q0] = a.x;
+                fneq[q0 + 1] = a.y;
+                fneq[q0 + 2] = a.z;
+                if (q0 + 3 < kQ) fneq[q0 + 3] = a.w;
+            }
+        }
+        if (k < 2) {
+            __syncthreads();
+            stage();
+            __syncthreads();
+        }
+    }
+    if (!active) return;
+    const float invS = 1.0f / static_cast<float>(__popc(nmask));
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        fneq[q] *= invS;
+        fneq[q] *= kRecInvScale;
+    }
+    const int dEdge = min(min(xc - rx0, rx1 - 1 - xc), min(yc - ry0, ry1 - 1 - yc));
+    restrictFinish<S>(cf, cLayout, r, xc, yc, zc, cc, dEdge, fneq, Mom{mom.x, mom.y, mom.z, mom.w}, neqScaleInv,
+                      mRho, mU, mV, mW);
+}
+
 // Unstaged restriction (no scratch): a workgroup covers kTX x kTY x kTZ
 // coarse cells; phase 1 averages the children of the tile plus a one-cell
 // halo into LDS (fneq, and the moments of the tile's own cells), phase 2
@@ -816,9 +935,14 @@ struct RefineOps {
             constexpr int TX = kStagedTX, TY = kStagedTY, TZ = kStagedTZ;
             const long long blocks = static_cast<long long>((rw + TX - 1) / TX) * ((rh + TY - 1) / TY) *
                                      ((r.cnz + TZ - 1) / TZ);
-            restrictStagedKernel<S, TX, TY, TZ><<<xcdGrid(blocks), TX * TY * TZ, 0, st>>>(
-                c.f, c.kind, cLayout, r, neqScaleInv, macro ? rho : nullptr, u, v, w,
-                reinterpret_cast<const float4*>(scratch));
+            if constexpr (RecordFormat<S>::kCompact)
+                restrictStagedCompactKernel<S, TX, TY, TZ><<<xcdGrid(blocks), TX * TY * TZ, 0, st>>>(
+                    c.f, c.kind, cLayout, r, neqScaleInv, macro ? rho : nullptr, u, v, w,
+                    reinterpret_cast<const float4*>(scratch));
+            else
+                restrictStagedKernel<S, TX, TY, TZ><<<xcdGrid(blocks), TX * TY * TZ, 0, st>>>(
+                    c.f, c.kind, cLayout, r, neqScaleInv, macro ? rho : nullptr, u, v, w,
+                    reinterpret_cast<const float4*>(scratch));
         } else {
             const unsigned blocks = static_cast<unsigned>(((rw + kTX - 1) / kTX) * ((rh + kTY - 1) / kTY) *
                                                           ((r.cnz + kTZ - 1) / kTZ));

@@ -255,6 +255,7 @@ public:
         wm_.free();
         hipRelease(dMlp_);
         hipRelease(dParticles_);
+        hipRelease(dVel4_);
         qc_.free();
         particleCount_ = 0;
         if (hForce_) (void)hipHostFree(hForce_);
@@ -613,9 +614,17 @@ public:
         ParticleParams p = particleParams_;
         p.dtSteps = dtSteps;
         p.seed = seed;
-        if (launchParticleAdvect(dParticles_, particleCount_, u_, v_, w_, kind_, dims_, p, stream_) != 0)
+        if (packedVelocity() != FC_OK) return fail(FC_ERR_ALLOC, "packed velocity buffer");
+        if (launchParticleAdvect(dParticles_, particleCount_, u_, v_, w_, kind_, dims_, p, dVel4_, stream_) != 0)
             return fail(FC_ERR_DEVICE, "particle advection failed");
         return FC_OK;
+    }
+
+    // Packed (u, v, w, 0) velocity scratch of the advection (one 16-byte
+    // corner read instead of three), allocated on first use.
+    int packedVelocity() {
+        if (dVel4_) return FC_OK;
+        return hipAlloc(&dVel4_, static_cast<std::size_t>(ncells_) * 4 * sizeof(float));
     }
 
     int particlesAdvectExternal(float* devPos4, int count, float dtSteps, unsigned seed) {
@@ -623,7 +632,8 @@ public:
         ParticleParams p = particleParams_;
         p.dtSteps = dtSteps;
         p.seed = seed;
-        if (launchParticleAdvect(devPos4, count, u_, v_, w_, kind_, dims_, p, stream_) != 0)
+        if (packedVelocity() != FC_OK) return fail(FC_ERR_ALLOC, "packed velocity buffer");
+        if (launchParticleAdvect(devPos4, count, u_, v_, w_, kind_, dims_, p, dVel4_, stream_) != 0)
             return fail(FC_ERR_DEVICE, "particle advection failed");
         return FC_OK;
     }
@@ -2286,6 +2296,7 @@ private:
     bool harvestOn_ = false;
     MlpParams* dMlp_ = nullptr;
     float* dParticles_ = nullptr;
+    float* dVel4_ = nullptr;
     bool qlibbEnabled_ = false;
     QLevel qc_;  // coarse-level q-LIBB links
     int particleCount_ = 0;

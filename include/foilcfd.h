@@ -679,6 +679,19 @@ int fc_kernel_timing_level(fc_solver* s, int level, int* launches, double* total
  * the solver's stream (ms). */
 int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms);
 
+/* Traffic-only probes of the HBM ceiling (no counterpart in the reference;
+ * measurement infrastructure for the stream-collide roofline). The AA probe
+ * allocates a 19-slab array of dims cells with elem_bytes (2 or 4) per
+ * population, then times `reps` launches of a kernel with the step kernels'
+ * decomposition that moves exactly one in-place step's bytes (19 reads and 19
+ * writes per cell): shifted = 0 is the local S->L pattern, 1 the L->S pattern
+ * with neighbour reads and writes; `work` FP32 FMAs per population (19
+ * independent chains) sit between the read and the write (0: none). The
+ * copy probe times a plain device copy of `bytes`. Both free what they
+ * allocate; ms per launch. */
+int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int shifted, int work, int reps, double* ms_per_launch);
+int fc_probe_copy(size_t bytes, int reps, double* ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif
