@@ -102,7 +102,40 @@ def full_setup(case, storage, aoa, wall="newton"):
     return s, box, info
 
 
-def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant="auto"):
+def train_wall_network(s, calls, per_call, seed=[1]):
+    """The learned wall function for a solver: wall-cell samples (y_s, |u_t|,
+    nu, Newton u_tau) harvested from its own wall cells over `calls` calls,
+    a 3-32-32-1 network trained on them with the library's BF16 MFMA kernels
+    and Adam, then installed (Newton stays the fallback outside the
+    network's envelope). Untimed set-up; returns what was trained."""
+    import numpy as np
+
+    from foil_cfd_amd import mlp
+    s.set_wall_harvest(True)
+    got = []
+    for _ in range(calls):
+        s.step_n(per_call)
+        h = s.wall_harvest()
+        got.append(h[h[:, 3] > 0])
+    s.set_wall_harvest(False)
+    smp = np.concatenate(got)
+    if len(smp) < 1024:
+        raise RuntimeError(f"wall harvest gave {len(smp)} samples; the learned wall function needs a flow")
+    tr = mlp.MlpTrainer(smp, seed=7)
+    t0 = time.perf_counter()
+    losses = tr.fit(2000, 1024, 3e-3, 1e-4)
+    prm = tr.export()
+    tr.close()
+    s.set_wall_mlp(prm)
+    s.set_wall_model("mlp")
+    out = {"samples": int(len(smp)), "train_s": round(time.perf_counter() - t0, 3),
+           "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:]))}
+    log(f"[bench] learned wall function: {out['samples']} harvested samples, loss {out['loss_first']:.3e} -> "
+        f"{out['loss_last']:.3e} in {out['train_s']} s")
+    return out
+
+
+def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant="auto", wall_fn="newton"):
     n_c = case.dims[0] * case.dims[1] * case.dims[2]
     s, box, info = full_setup(case, storage, aoa)
     s.set_step_variant(variant)
@@ -121,6 +154,15 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
             done += k
 
     run(warmup)
+    # the learned wall function is trained (untimed) on samples harvested from
+    # this solver's own wall cells after the startup ramps (2 nx steps; the
+    # ramped viscosity would leave the later cells outside its envelope)
+    network = None
+    if wall_fn == "mlp":
+        ramp_end = 2 * case.dims[0]
+        while s.step_count() < ramp_end:
+            s.step_n(min(100, ramp_end - s.step_count()))
+        network = train_wall_network(s, 3, 100)
     s.synchronize()
     t0 = time.perf_counter()
     run(steps)
@@ -162,6 +204,9 @@ def run_full(case, storage, steps, warmup, per_call, kernel_steps, aoa, variant=
         "cl_ema": fr.cl,
         "cd_ema": fr.cd,
         "wall_cells_active_fine": wr.cells,
+        "wall_function": wall_fn,
+        "wall_network_cells": int(wr.mlp_cells),
+        "wall_network": network,
         "patch_box": list(box),
         "fine_dims": list(fd),
     }
@@ -184,6 +229,8 @@ def accuracy_run(args):
     fc.load()
     storage = args.storages.split(",")[0].strip()
     case = cases.config3_wing(args.aoa)
+    if args.tau:  # same grid, patch and models at a lower Reynolds number
+        case = cases.naca_tunnel(512, 192, 240, 256, aoa=args.aoa, u_lat=0.08, cs=0.1733, tau=args.tau)
     s, box, info = full_setup(case, storage, args.aoa, wall=args.wall)
     ft = fc.flow_through_steps(case.scaling, case.dims[0])
     total = int(round(args.flow_throughs * ft / args.per_call)) * args.per_call
@@ -192,7 +239,8 @@ def accuracy_run(args):
            "storage": storage, "aoa": args.aoa, "wall_model": args.wall, "per_call": args.per_call,
            "flow_through_steps": ft, "steps_planned": total, "patch_box": list(box),
            "fine_dims": list(info["fine_dims"]), "forces_from_fine": info["forces_from_fine"],
-           "tau": case.scaling.tau, "fine_tau": fscale.tau, "samples": []}
+           "tau": case.scaling.tau, "fine_tau": fscale.tau, "re_chord": case.scaling.re_effective,
+           "samples": []}
     t0 = time.perf_counter()
     seed = 1
     done = 0
@@ -245,6 +293,11 @@ def accuracy_summary(paths, out_path, batches=6):
            "protocol": "one momentum-exchange sample per 100 coarse steps; average over samples at or after "
                        "1 flow-through (reference readout), 4 flow-throughs run; standard error from batch means",
            "batches": batches, "aoa": {}}
+    try:  # angles not among the new records keep their committed entries
+        with open(out_path) as fh:
+            out["aoa"].update({k: v for k, v in json.load(fh).get("aoa", {}).items() if k not in runs})
+    except (OSError, ValueError):
+        pass
     for aoa, by in sorted(runs.items(), key=lambda kv: float(kv[0])):
         ent = {}
         for st, r in by.items():
@@ -282,6 +335,7 @@ def accuracy_summary(paths, out_path, batches=6):
                 early = ref[:k, 0] <= 1100
                 e["trace_max_rel_diff_to_step_1100"] = round(float(rel[early].max()), 6) if early.any() else None
         out["aoa"][aoa] = ent
+    out["aoa"] = dict(sorted(out["aoa"].items(), key=lambda kv: float(kv[0])))
     with open(out_path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps({k: {st: {kk: v for kk, v in e.items() if "batch" not in kk} for st, e in ent.items()}
@@ -545,12 +599,18 @@ def main():
                     help="time-averaged Cl/Cd run of the full configuration for the first of --storages "
                          "(record written to --out)")
     ap.add_argument("--flow-throughs", type=float, default=4.0)
+    ap.add_argument("--tau", type=float, default=None,
+                    help="--accuracy at this relaxation time instead of the physical (clamped) one")
+    ap.add_argument("--summary-out", default=None,
+                    help="--accuracy-summary target (default profiles/wing_cl_cd_accuracy.json)")
     ap.add_argument("--wall", default="newton", choices=["off", "newton", "mlp"])
     ap.add_argument("--out", default="accuracy.json")
     ap.add_argument("--accuracy-summary", nargs="+", metavar="RECORD",
                     help="fold --accuracy records into the committed accuracy summary (no GPU)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--headline-wall", default="mlp", choices=["newton", "mlp"],
+                    help="wall model of the 16-bit storages in the full configuration (FP32 / FP64 use Newton)")
     ap.add_argument("--probe", action="store_true",
                     help="traffic-only ceiling of the step's access pattern (separate JSON line)")
     args = ap.parse_args()
@@ -567,7 +627,7 @@ def main():
         traffic_probe()
         return
     if args.accuracy_summary:
-        accuracy_summary(args.accuracy_summary, ACCURACY_PROFILE)
+        accuracy_summary(args.accuracy_summary, args.summary_out or ACCURACY_PROFILE)
         return
     if args.polar:
         polar_bench(args)
@@ -592,8 +652,9 @@ def main():
             results[st] = run_storage(case, st, args.steps, args.warmup, args.per_call, args.kernel_steps,
                                       args.variant)
         else:
+            wall = args.headline_wall if st in ("f16c", "f16", "bf16") else "newton"
             results[st] = run_full(case, st, args.steps, args.warmup, args.per_call, args.kernel_steps, args.aoa,
-                                   args.variant)
+                                   args.variant, wall)
     head = next((h for h in ("f16c", "f16") if h in results), storages[0])
     r = results[head]
     cpu = None if args.no_cpu else cpu_baseline(case)
@@ -618,14 +679,17 @@ def main():
                   "fine_grid": fd, "patch_box_coarse": r["patch_box"], "refine_factor": 2,
                   "patch_margins_chords": dict(zip(("upstream", "wake", "below", "above"), cases.APP_PATCH_MARGINS_C)),
                   "cell_updates_per_step": r["cell_updates_per_step"], "storage": store,
-                  "wall_model": "iMEM Reichardt (Newton), coarse + fine wall cells",
+                  "wall_model": ("iMEM Reichardt, coarse + fine wall cells; " +
+                                 ("learned u_tau network (Newton fallback)" if r.get("wall_function") == "mlp"
+                                  else "Newton u_tau") + " for the headline storage, Newton for FP32"),
                   "particles": cases.DEFAULT_PARTICLES}
         kernel = ("fine-level stream-collide launches (dominant; one of the two per coarse step also writes "
                   "the restriction's child-average records)")
         traffic = measured_traffic(head, "fine")
         bytes_per_launch = r["kernel_bytes"]
     config.update({"collision": "TRT(3/16) + Smagorinsky(0.1733)", "streaming": "in-place AA",
-                   "step_variant": args.variant, "steps_per_call": args.per_call, "parallelism": "single_gpu"})
+                   "step_variant": args.variant, "steps_per_call": min(args.per_call, args.steps),
+                   "parallelism": "single_gpu"})
     out = {
         "metric": METRIC,
         "value": round(r["glups"], 4),

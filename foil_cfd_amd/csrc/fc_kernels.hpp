@@ -4,6 +4,7 @@
 // hipError_t of its launch as an int (0 = success).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include "fc_layout.hpp"
@@ -243,10 +244,14 @@ struct ParticleParams {
 int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, void* stream);
 // kind: the solver's kind bytes (low nibble = cell flag) or null. vel4: null,
 // or device scratch of 4 floats per cell that first receives (u, v, w, 0) of
-// every cell so each trilinear corner is one 16-byte read (same result).
+// every cell so each trilinear corner is one 16-byte read. orderScratch
+// (with vel4; particleOrderScratchBytes(count, d) bytes): the particles are
+// first sorted by cell and advanced in that order, so neighbouring lanes
+// share the gathered lines. Every option gives the same particles.
+std::size_t particleOrderScratchBytes(int count, const Dims& d);
 int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
                          const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, float* vel4,
-                         void* stream);
+                         void* orderScratch, std::size_t orderBytes, void* stream);
 
 // ---- q-LIBB pre-pass ----------------------------------------------------------
 // Links: cell index + packed (q | byte << 8 | ff << 16); scratch: count doubles.

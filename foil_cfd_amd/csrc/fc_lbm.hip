@@ -134,6 +134,23 @@ __device__ inline Macro<C> collide(C H[kQ], C K, C tau0, C lam, C pre) {
     return Macro<C>{rho, ux, uy, uz};
 }
 
+// An empty asm that takes v as input and output: the load that produced v
+// has completed at this point, and later loads cannot be hoisted above it.
+template <class R>
+__device__ inline void pinLoaded(R& v) {
+    asm volatile("" : "+v"(v));
+}
+
+// Slot of the incoming population q of this cell in the current layout: from
+// L, f*_opp(q) of the upstream cell; from S, the cell's own slot q.
+template <bool FROM_L, int q>
+struct PullSlot {
+    static constexpr int dir = FROM_L ? kOpp[q] : q;
+    static constexpr int ex = FROM_L ? -kCx[q] : 0;
+    static constexpr int ey = FROM_L ? -kCy[q] : 0;
+    static constexpr int ez = FROM_L ? -kCz[q] : 0;
+};
+
 // ---------------------------------------------------------------------------
 // The fused step. FROM_L: the array is in layout L and this step writes
 // layout S (scatter writes); otherwise it reads layout S and writes L (fully
@@ -180,11 +197,11 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
                 a.w[c] = 0.0f;
             }
         } else if (flag == kOutlet) {  // zero-gradient copy, lbm_core.cu:254-272
-            const T* src = static_cast<const T*>(a.feed) + (FROM_L ? 0 : 1) * kQ * plane;
+            // H holds the feed plane's values (loaded with the pulls)
             C r = 0, jx = 0, jy = 0, jz = 0;
             staticFor<kQ>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const C h = S::load(src + q * plane + yz);
+                const C h = H[q];
                 storeOut(qc, h, true);
                 r += h;
                 jx += C(kCx[q]) * h;
@@ -218,7 +235,55 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
             uwy = C(halfBits(a.uwy[c]));
             uwz = C(halfBits(a.uwz[c]));
         }
-        staticFor<kQ - 1>([&](auto qm) {
+        if constexpr (std::is_same_v<M<S>, PopPtr<S>>) {
+            // The 18 upstream kinds in one batch, then the 18 source slots in
+            // one batch (the slot picked per link without a branch: bounce-
+            // back, a mirror, or the plain pull again), so a wave holding a
+            // wall-adjacent cell waits for two more memory round trips, not
+            // for one or two per link.
+            // upstream kinds folded into three link masks as they arrive
+            unsigned solidM = 0, slipYM = 0, slipZM = 0;
+            staticFor<kQ - 1>([&](auto qm) {
+                constexpr int q = decltype(qm)::value + 1;
+                const std::uint8_t nf = a.kind[nb(p, -kCx[q], -kCy[q], -kCz[q])] & kKindFlagMask;
+                solidM |= static_cast<unsigned>(nf == kSolid) << q;
+                slipYM |= static_cast<unsigned>(nf == kSlipTop || nf == kSlipBottom) << q;
+                slipZM |= static_cast<unsigned>(nf == kSlipFront || nf == kSlipBack) << q;
+            });
+            // in three groups of six links, each completed before the next
+            // (the empty asm) so at most six selected addresses are live
+            staticFor<3>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                staticFor<6>([&](auto jc) {
+                    constexpr int q = 1 + 6 * g + decltype(jc)::value;
+                    constexpr int cx = kCx[q], cy = kCy[q], cz = kCz[q];
+                    constexpr int my = kMirY[q], mz = kMirZ[q];
+                    using L = PullSlot<FROM_L, q>;
+                    const bool solid = (solidM >> q) & 1u;
+                    const bool slipY = (slipYM >> q) & 1u;
+                    const bool slipZ = (slipZM >> q) & 1u;
+                    // (direction slab, cell) of bounce-back, y mirror, z mirror, plain pull
+                    constexpr int dBB = FROM_L ? q : kOpp[q], dSY = FROM_L ? kOpp[my] : my;
+                    constexpr int dSZ = FROM_L ? kOpp[mz] : mz;
+                    const unsigned cBB = FROM_L ? nb(p, 0, 0, 0) : nb(p, -cx, -cy, -cz);
+                    const unsigned cSY = FROM_L ? nb(p, -cx, 0, -cz) : nb(p, 0, -cy, 0);
+                    const unsigned cSZ = FROM_L ? nb(p, -cx, -cy, 0) : nb(p, 0, 0, -cz);
+                    const unsigned cPL = nb(p, L::ex, L::ey, L::ez);
+                    const int d = solid ? dBB : (slipY ? dSY : (slipZ ? dSZ : L::dir));
+                    const unsigned cc = solid ? cBB : (slipY ? cSY : (slipZ ? cSZ : cPL));
+                    H[q] = S::load(pop.f + static_cast<long long>(d) * pop.n + cc);
+                });
+                staticFor<6>([&](auto jc) { pinLoaded(H[1 + 6 * g + decltype(jc)::value]); });
+            });
+            if (WM) {
+                staticFor<kQ - 1>([&](auto qm) {
+                    constexpr int q = decltype(qm)::value + 1;
+                    const C cuw = C(kCx[q]) * uwx + C(kCy[q]) * uwy + C(kCz[q]) * uwz;
+                    const C withSlip = fmac(K * C(6) * wOf<C>(q), cuw, H[q]);
+                    H[q] = ((solidM >> q) & 1u) ? withSlip : H[q];
+                });
+            }
+        } else staticFor<kQ - 1>([&](auto qm) {
             constexpr int q = decltype(qm)::value + 1;
             constexpr int cx = kCx[q], cy = kCy[q], cz = kCz[q];
             const std::uint8_t nf = a.kind[nb(p, -cx, -cy, -cz)] & kKindFlagMask;
@@ -264,15 +329,33 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
     }
 }
 
-// Slot of the incoming population q of this cell in the current layout: from
-// L, f*_opp(q) of the upstream cell; from S, the cell's own slot q.
-template <bool FROM_L, int q>
-struct PullSlot {
-    static constexpr int dir = FROM_L ? kOpp[q] : q;
-    static constexpr int ex = FROM_L ? -kCx[q] : 0;
-    static constexpr int ey = FROM_L ? -kCy[q] : 0;
-    static constexpr int ez = FROM_L ? -kCz[q] : 0;
-};
+// The loads of a cell after its kind byte: a fluid cell pulls its 19
+// populations; an outlet cell loads the 19 values the x-1 column published
+// in the previous step (the zero-gradient copy's source). Both go into H and
+// are issued before any wait, so a wave holding an outlet lane waits for
+// one round trip, not two.
+template <class S, template <class> class M, bool FROM_L>
+__device__ inline void loadCell(const KArgs& a, const RowPos& p, const M<S>& pop, std::uint8_t k,
+                                typename S::C (&H)[kQ]);
+
+
+template <class S, template <class> class M, bool FROM_L>
+__device__ inline void loadCell(const KArgs& a, const RowPos& p, const M<S>& pop, std::uint8_t k,
+                                typename S::C (&H)[kQ]) {
+    const std::uint8_t flag = k & kKindFlagMask;
+    if (flag == kFluid) {
+        staticFor<kQ>([&](auto qc) {
+            using L = PullSlot<FROM_L, decltype(qc)::value>;
+            H[decltype(qc)::value] = pop.ld(L::dir, L::ex, L::ey, L::ez);
+        });
+    } else if (flag == kOutlet) {
+        using T = typename S::T;
+        const long long plane = static_cast<long long>(a.ny) * a.nz;
+        const T* src = static_cast<const T*>(a.feed) + (FROM_L ? 0 : 1) * kQ * plane + (p.y + a.ny * p.z);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) H[q] = S::load(src + q * plane);
+    }
+}
 
 template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
 __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
@@ -280,16 +363,23 @@ __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
     const M<S> pop(a, p);
     const std::uint8_t k = a.kind[nb(p, 0, 0, 0)];
     C H[kQ];
-    if ((k & kKindFlagMask) == kFluid)
-        staticFor<kQ>([&](auto qc) {
-            using L = PullSlot<FROM_L, decltype(qc)::value>;
-            H[decltype(qc)::value] = pop.ld(L::dir, L::ex, L::ey, L::ez);
-        });
+    loadCell<S, M, FROM_L>(a, p, pop, k, H);
     cellBody<S, M, FROM_L, WM, PX, PY>(a, p, pop, k, H);
 }
 
+// Resident waves per SIMD a step kernel asks the register allocator for.
+// Seven (at most 72 VGPRs) for the 16-bit storages' wall-model steps and
+// record-writing steps: they then fit with no hot-path spills (73-107 VGPRs
+// and 4-6 waves without the request) and run faster on the refined wing. The
+// other variants keep the allocator's own choice (a request for one wave is
+// the default): the same cap spills the 32-bit and plain-wall variants and
+// slows them (both measured on MI355X).
+template <class S, bool WM_OR_RECORD>
+inline constexpr int kStepWaves = sizeof(typename S::T) == 2 && WM_OR_RECORD ? 7 : 1;
+
 template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
-__global__ void __launch_bounds__(kBlock) stepKernel(KArgs a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, WM>, 8)))
+stepKernel(KArgs a) {
     RowPos p;
     if (!rowPos<PX, PY>(a, p)) return;
     stepCell<S, M, FROM_L, WM, PX, PY>(a, p);
@@ -336,7 +426,8 @@ __device__ inline float storedDeviation(typename S::C v) {
 }
 
 template <class S, template <class> class M, bool FROM_L, bool WM>
-__global__ void __launch_bounds__(kBlock) stepRecordKernel(KArgs a, RecArgs rg) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, true>, 8)))
+stepRecordKernel(KArgs a, RecArgs rg) {
     static_assert(kWavesPerBlock == 4, "one wave per child row of a 2 x 2 row group");
     __shared__ float sH[kWavesPerBlock][kQ][kWave];
     __shared__ unsigned char sLive[kWavesPerBlock][kWave];
@@ -357,11 +448,7 @@ __global__ void __launch_bounds__(kBlock) stepRecordKernel(KArgs a, RecArgs rg) 
     if (live) {
         const M<S> pop(a, p);
         k = a.kind[nb(p, 0, 0, 0)];
-        if ((k & kKindFlagMask) == kFluid)
-            staticFor<kQ>([&](auto qc) {
-                using L = PullSlot<FROM_L, decltype(qc)::value>;
-                H[decltype(qc)::value] = pop.ld(L::dir, L::ex, L::ey, L::ez);
-            });
+        loadCell<S, M, FROM_L>(a, p, pop, k, H);
         cellBody<S, M, FROM_L, WM, false, false>(a, p, pop, k, H);
     }
     if (yp < rg.cy0 || yp >= rg.cy1) return;  // workgroup-uniform: rows outside the restricted box
@@ -963,11 +1050,15 @@ struct StepRecordOp {
         const bool wm = lat.uwx && lat.uwy && lat.uwz;
         const bool fromL = layout == kLayoutL;
         const dim3 g(rowGrid(a)), b(kBlock);
-        if (fromL && wm) stepRecordKernel<S, PopPtr, true, true><<<g, b, 0, st>>>(a, rg);
-        else if (fromL) stepRecordKernel<S, PopPtr, true, false><<<g, b, 0, st>>>(a, rg);
-        else if (wm) stepRecordKernel<S, PopPtr, false, true><<<g, b, 0, st>>>(a, rg);
-        else stepRecordKernel<S, PopPtr, false, false><<<g, b, 0, st>>>(a, rg);
+        if (fromL && wm) launch<true, true>(g, b, st, a, rg);
+        else if (fromL) launch<true, false>(g, b, st, a, rg);
+        else if (wm) launch<false, true>(g, b, st, a, rg);
+        else launch<false, false>(g, b, st, a, rg);
         return lastError();
+    }
+    template <bool FROM_L, bool WM>
+    static void launch(dim3 g, dim3 b, hipStream_t st, const KArgs& a, const RecArgs& rg) {
+        stepRecordKernel<S, PopPtr, FROM_L, WM><<<g, b, 0, st>>>(a, rg);
     }
 };
 

@@ -6,7 +6,12 @@
 // switched off so every particle is bit-identical to the CPU restatement.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "fc_kcommon.hpp"
+#include "fc_refine_math.hpp"
 
 #pragma clang fp contract(off)
 
@@ -49,14 +54,19 @@ struct VelSoA {
     const float* u;
     const float* v;
     const float* w;
+    const std::uint8_t* kind;
     __device__ V3 at(long long c) const { return V3{u[c], v[c], w[c]}; }
+    __device__ bool solid(long long c) const { return kind && (kind[c] & kKindFlagMask) == kSolid; }
 };
+// The packed vector's fourth component marks solid cells (1), so the final
+// solid-entry test reads a vector the two samples most often just fetched.
 struct VelPacked {
     const float4* vel;
     __device__ V3 at(long long c) const {
         const float4 t = vel[c];
         return V3{t.x, t.y, t.z};
     }
+    __device__ bool solid(long long c) const { return vel[c].w != 0.0f; }
 };
 
 template <class VS>
@@ -127,18 +137,63 @@ __global__ void __launch_bounds__(256) seedKernel(float4* pos, int count, Dims d
     pos[i] = make_float4(a, b, c, e);
 }
 
-// (u, v, w) of every cell into one 16-byte vector (read once per cell).
+// (u, v, w, solid mark) of every cell into one 16-byte vector.
 __global__ void __launch_bounds__(256) packVelocityKernel(const float* u, const float* v, const float* w,
-                                                          float4* vel, long long n) {
+                                                          const std::uint8_t* kind, float4* vel, long long n) {
     const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c < n) vel[c] = make_float4(u[c], v[c], w[c], 0.0f);
+    if (c < n) vel[c] = make_float4(u[c], v[c], w[c], kind && isSolid(kind, c) ? 1.0f : 0.0f);
 }
 
-template <class VS>
-__global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, VS vs, const std::uint8_t* kind, Dims d,
-                                                    ParticleParams prm) {
+// Sort key of a particle: the Morton (Z-order) index of the cell it is in
+// (clamped into the grid; non-finite positions sort to cell 0), so particles
+// that share trilinear corners in x, y or z sort close together and a run of
+// the sorted list covers a compact block of cells. Grids up to 1024 cells
+// per axis (30-bit keys); larger grids fall back to the plain cell index.
+__host__ __device__ inline unsigned spread3(unsigned v) {  // bit i -> bit 3i, 10 bits
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__host__ __device__ inline bool mortonKeys(const Dims& d) { return d.nx <= 1024 && d.ny <= 1024 && d.nz <= 1024; }
+
+__global__ void __launch_bounds__(256) particleKeyKernel(const float4* pos, int count, Dims d, unsigned* key,
+                                                         int* slot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
+    const float4 p = pos[i];
+    const bool ok = __builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z);
+    const int x = ok ? clampi(static_cast<int>(p.x), 0, d.nx - 1) : 0;
+    const int y = ok ? clampi(static_cast<int>(p.y), 0, d.ny - 1) : 0;
+    const int z = ok ? clampi(static_cast<int>(p.z), 0, d.nz - 1) : 0;
+    key[i] = mortonKeys(d) ? spread3(static_cast<unsigned>(x)) | (spread3(static_cast<unsigned>(y)) << 1) |
+                                 (spread3(static_cast<unsigned>(z)) << 2)
+                           : static_cast<unsigned>(cellAt(d, x, y, z));
+    slot[i] = i;
+}
+
+// One particle per lane. With `order` (slots sorted by cell), lane t of the
+// XCD-contiguous block range advances slot order[t]: neighbouring lanes and
+// the blocks of one XCD then sample neighbouring cells, so the trilinear
+// gathers hit lines already in L2 instead of a new line per corner. The
+// update of a slot (and its random stream, seeded by the slot) does not
+// depend on the order, so both orders give identical particles.
+template <class VS>
+__global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, VS vs, const std::uint8_t* kind, Dims d,
+                                                    ParticleParams prm, const int* order) {
+    int i;
+    if (order) {
+        const long long blocks = (static_cast<long long>(count) + 255) / 256;
+        const long long t = xcdBlock(blocks) * 256 + threadIdx.x;
+        if (t >= count) return;
+        i = order[t];
+    } else {
+        i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= count) return;
+    }
     unsigned r = hashU32(prm.seed ^ (static_cast<unsigned>(i) * 0x9E3779B9u));
     const float4 pk = pos[i];
     V3 p{pk.x, pk.y, pk.z};
@@ -164,7 +219,7 @@ __global__ void __launch_bounds__(256) advectKernel(float4* pos, int count, VS v
     const int cx = clampi(static_cast<int>(p.x), 0, d.nx - 1);
     const int cy = clampi(static_cast<int>(p.y), 0, d.ny - 1);
     const int cz = clampi(static_cast<int>(p.z), 0, d.nz - 1);
-    const bool solid = !outflow && !oob && kind && isSolid(kind, cellAt(d, cx, cy, cz));
+    const bool solid = !outflow && !oob && vs.solid(cellAt(d, cx, cy, cz));
     if (outflow || oob || solid || age <= 0.0f) {
         const bool toInlet = outflow || oob || rand01(r) < prm.inletFraction;
         p = toInlet ? inletSpawn(d, r) : volumeSpawn(d, kind, r);
@@ -182,20 +237,91 @@ int launchParticleSeed(float* pos4, int count, const Dims& d, unsigned seed, voi
     return static_cast<int>(hipGetLastError());
 }
 
+namespace {
+
+int keyBits(const Dims& d) {
+    if (mortonKeys(d)) {  // 3 x the bits of the longest axis
+        int b = 1;
+        const int m = d.nx > d.ny ? (d.nx > d.nz ? d.nx : d.nz) : (d.ny > d.nz ? d.ny : d.nz);
+        while ((1 << b) < m) ++b;
+        return 3 * b;
+    }
+    int b = 1;
+    while (b < 32 && (1ll << b) < d.cells()) ++b;
+    return b;
+}
+
+// Layout of the order scratch: keys, alternate keys, slots, alternate slots
+// (count each, 256-byte aligned), then rocPRIM's radix-sort storage.
+struct OrderScratch {
+    unsigned *key, *keyAlt;
+    int *slot, *slotAlt;
+    void* sortTmp;
+    std::size_t sortBytes;
+};
+
+std::size_t alignUp(std::size_t b) { return (b + 255) & ~static_cast<std::size_t>(255); }
+
+std::size_t sortTempBytes(int count, int bits) {
+    std::size_t bytes = 0;
+    rocprim::double_buffer<unsigned> k(nullptr, nullptr);
+    rocprim::double_buffer<int> v(nullptr, nullptr);
+    if (rocprim::radix_sort_pairs(nullptr, bytes, k, v, count, 0, bits) != hipSuccess) return 0;
+    return bytes;
+}
+
+OrderScratch carve(void* base, int count, std::size_t sortBytes) {
+    char* p = static_cast<char*>(base);
+    const std::size_t a = alignUp(static_cast<std::size_t>(count) * 4);
+    OrderScratch o;
+    o.key = reinterpret_cast<unsigned*>(p);
+    o.keyAlt = reinterpret_cast<unsigned*>(p + a);
+    o.slot = reinterpret_cast<int*>(p + 2 * a);
+    o.slotAlt = reinterpret_cast<int*>(p + 3 * a);
+    o.sortTmp = p + 4 * a;
+    o.sortBytes = sortBytes;
+    return o;
+}
+
+}  // namespace
+
+std::size_t particleOrderScratchBytes(int count, const Dims& d) {
+    if (count <= 0 || d.cells() <= 0) return 0;
+    const std::size_t t = sortTempBytes(count, keyBits(d));
+    return t ? 4 * alignUp(static_cast<std::size_t>(count) * 4) + alignUp(t) : 0;
+}
+
 int launchParticleAdvect(float* pos4, int count, const float* u, const float* v, const float* w,
                          const std::uint8_t* kind, const Dims& d, const ParticleParams& prm, float* vel4,
-                         void* stream) {
+                         void* orderScratch, std::size_t orderBytes, void* stream) {
     if (!pos4 || count <= 0 || !u || !v || !w || d.cells() <= 0) return static_cast<int>(hipErrorInvalidValue);
     float4* pos = reinterpret_cast<float4*>(pos4);
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (vel4) {
-        float4* vel = reinterpret_cast<float4*>(vel4);
-        const long long n = d.cells();
-        packVelocityKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(u, v, w, vel, n);
-        advectKernel<VelPacked><<<(count + 255) / 256, 256, 0, st>>>(pos, count, VelPacked{vel}, kind, d, prm);
-    } else {
-        advectKernel<VelSoA><<<(count + 255) / 256, 256, 0, st>>>(pos, count, VelSoA{u, v, w}, kind, d, prm);
+    const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+    if (!vel4) {
+        advectKernel<VelSoA><<<grid, 256, 0, st>>>(pos, count, VelSoA{u, v, w, kind}, kind, d, prm, nullptr);
+        return static_cast<int>(hipGetLastError());
     }
+    float4* vel = reinterpret_cast<float4*>(vel4);
+    const long long n = d.cells();
+    packVelocityKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(u, v, w, kind, vel, n);
+    const int* order = nullptr;
+    if (orderScratch) {
+        const int bits = keyBits(d);
+        const std::size_t tb = sortTempBytes(count, bits);
+        if (!tb || orderBytes < 4 * alignUp(static_cast<std::size_t>(count) * 4) + tb)
+            return static_cast<int>(hipErrorInvalidValue);
+        OrderScratch o = carve(orderScratch, count, tb);
+        particleKeyKernel<<<grid, 256, 0, st>>>(pos, count, d, o.key, o.slot);
+        rocprim::double_buffer<unsigned> keys(o.key, o.keyAlt);
+        rocprim::double_buffer<int> slots(o.slot, o.slotAlt);
+        std::size_t bytes = o.sortBytes;
+        if (rocprim::radix_sort_pairs(o.sortTmp, bytes, keys, slots, count, 0, bits, st) != hipSuccess)
+            return static_cast<int>(hipErrorUnknown);
+        order = slots.current();
+    }
+    const unsigned g = order ? xcdGrid(grid) : grid;
+    advectKernel<VelPacked><<<g, 256, 0, st>>>(pos, count, VelPacked{vel}, kind, d, prm, order);
     return static_cast<int>(hipGetLastError());
 }
 

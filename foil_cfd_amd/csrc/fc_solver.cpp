@@ -256,6 +256,8 @@ public:
         hipRelease(dMlp_);
         hipRelease(dParticles_);
         hipRelease(dVel4_);
+        hipRelease(dOrder_);
+        orderBytes_ = 0;
         qc_.free();
         particleCount_ = 0;
         if (hForce_) (void)hipHostFree(hForce_);
@@ -598,6 +600,8 @@ public:
         if (count <= 0) return fail(FC_ERR_INVALID, "particle count must be positive");
         if (count != particleCount_) {
             hipRelease(dParticles_);
+            hipRelease(dOrder_);
+            orderBytes_ = 0;
             particleCount_ = 0;
             if (hipAlloc(&dParticles_, static_cast<std::size_t>(count) * 4 * sizeof(float)) != FC_OK)
                 return fail(FC_ERR_ALLOC, "particle buffer");
@@ -615,7 +619,13 @@ public:
         p.dtSteps = dtSteps;
         p.seed = seed;
         if (packedVelocity() != FC_OK) return fail(FC_ERR_ALLOC, "packed velocity buffer");
-        if (launchParticleAdvect(dParticles_, particleCount_, u_, v_, w_, kind_, dims_, p, dVel4_, stream_) != 0)
+        if (!dOrder_) {
+            const std::size_t b = particleOrderScratchBytes(particleCount_, dims_);
+            if (!b || hipAlloc(&dOrder_, b) != FC_OK) return fail(FC_ERR_ALLOC, "particle order scratch");
+            orderBytes_ = b;
+        }
+        if (launchParticleAdvect(dParticles_, particleCount_, u_, v_, w_, kind_, dims_, p, dVel4_, dOrder_,
+                                 orderBytes_, stream_) != 0)
             return fail(FC_ERR_DEVICE, "particle advection failed");
         return FC_OK;
     }
@@ -633,7 +643,7 @@ public:
         p.dtSteps = dtSteps;
         p.seed = seed;
         if (packedVelocity() != FC_OK) return fail(FC_ERR_ALLOC, "packed velocity buffer");
-        if (launchParticleAdvect(devPos4, count, u_, v_, w_, kind_, dims_, p, dVel4_, stream_) != 0)
+        if (launchParticleAdvect(devPos4, count, u_, v_, w_, kind_, dims_, p, dVel4_, nullptr, 0, stream_) != 0)
             return fail(FC_ERR_DEVICE, "particle advection failed");
         return FC_OK;
     }
@@ -2297,6 +2307,8 @@ private:
     MlpParams* dMlp_ = nullptr;
     float* dParticles_ = nullptr;
     float* dVel4_ = nullptr;
+    char* dOrder_ = nullptr;      // sort scratch of the cell-ordered advection
+    std::size_t orderBytes_ = 0;
     bool qlibbEnabled_ = false;
     QLevel qc_;  // coarse-level q-LIBB links
     int particleCount_ = 0;
