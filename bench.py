@@ -523,8 +523,10 @@ def traffic_probe(reps_coarse=200, reps_fine=50):
     (fc_probe_aa_traffic: the step kernels' decomposition moving 19 reads and
     19 writes per cell, no collision) for the coarse wing and the fine patch
     grid, 2- and 4-byte populations, the local (S->L) and shifted (L->S)
-    step, with and without 20 FP32 FMAs per population in between; plus a
-    plain device copy. One JSON line; rates in TB/s of algorithmic bytes."""
+    step and the esoteric-twist pattern (every step reads and writes slot q
+    at cell + max(e_q, 0)), with and without 20 FP32 FMAs per population in
+    between; plus a plain device copy. One JSON line; rates in TB/s of
+    algorithmic bytes."""
     import ctypes as C
 
     import foil_cfd_amd as fc
@@ -534,17 +536,16 @@ def traffic_probe(reps_coarse=200, reps_fine=50):
     for dims, reps in (((512, 192, 240), reps_coarse), ((864, 248, 480), reps_fine)):
         n = dims[0] * dims[1] * dims[2]
         for eb in (2, 4):
-            for shifted in (0, 1):
+            for pattern, pname in ((0, "S->L"), (1, "L->S"), (2, "twist")):
                 for work in (0, 20):
                     ms = C.c_double(0.0)
-                    rc = lib.fc_probe_aa_traffic(FcDims(*dims), eb, shifted, work, reps, C.byref(ms))
+                    rc = lib.fc_probe_aa_traffic(FcDims(*dims), eb, pattern, work, reps, C.byref(ms))
                     if rc != 0:
                         raise RuntimeError(f"fc_probe_aa_traffic failed ({rc})")
                     tbs = 2.0 * 19 * eb * n / (ms.value * 1e-3) / 1e12
-                    rows.append({"dims": list(dims), "bytes_per_pop": eb, "pattern": "L->S" if shifted else "S->L",
+                    rows.append({"dims": list(dims), "bytes_per_pop": eb, "pattern": pname,
                                  "fma_per_pop": work, "ms": round(ms.value, 5), "tb_s": round(tbs, 3)})
-                    log(f"[probe] {dims} {eb} B {'L->S' if shifted else 'S->L'} fma {work}: "
-                        f"{ms.value * 1e3:.1f} us {tbs:.3f} TB/s")
+                    log(f"[probe] {dims} {eb} B {pname} fma {work}: {ms.value * 1e3:.1f} us {tbs:.3f} TB/s")
     copies = []
     for gib in (1, 4):
         ms = C.c_double(0.0)
@@ -558,8 +559,8 @@ def traffic_probe(reps_coarse=200, reps_fine=50):
 
 
 def probe_line():
-    """The committed traffic-only probe, reduced to the 16-bit ceilings of the
-    two step patterns on the coarse and the fine grid (TB/s)."""
+    """The committed traffic-only probe, reduced to the ceilings of the step
+    patterns on the coarse and the fine grid with no arithmetic (TB/s)."""
     try:
         with open(PROBE_PROFILE) as fh:
             rec = json.load(fh)
@@ -583,8 +584,8 @@ def main():
     ap.add_argument("--aoa", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f16c,f32")
-    ap.add_argument("--variant", default="auto", choices=["auto", "single", "buf"],
-                    help="step kernel: auto (library default), single or buf")
+    ap.add_argument("--variant", default="auto", choices=["auto", "single", "buf", "wide"],
+                    help="step kernel: auto (library default), single, buf or wide")
     ap.add_argument("--coarse-only", action="store_true",
                     help="bare coarse wing (no refinement patch, wall model or particles)")
     ap.add_argument("--periodic-box", action="store_true",

@@ -39,6 +39,10 @@ struct KArgs {
     unsigned long long segMagic, nyMagic;
     float tau0, lam, pre, uIn;
     int writeMacro;
+    // 1: population element indices need 64 bits (kQ * ncells >= 2^32); the
+    // plain-wall step then takes the 64-bit slot select. Also set on request
+    // (the wide step variant) so that path runs on small grids.
+    int wideIndex;
 };
 
 inline unsigned long long divMagic(long long d) {
@@ -79,6 +83,7 @@ KArgs makeArgs(const LatticeView& lat) {
     a.nwaves = a.nseg * a.ny * a.nz;
     a.nblocks = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
     a.chunk = (a.nblocks + kXcds - 1) / kXcds;
+    a.wideIndex = a.ncells * kQ >= (1ll << 32) ? 1 : 0;
     setDivMagic(a);
     return a;
 }

@@ -22,7 +22,9 @@ namespace fc {
 // on every sub-step and leaves the records to the separate restriction pass.
 // kStepBuf runs the one-cell kernel with 32-bit buffer addressing (arrays
 // under 2 GiB, the default beyond) and the separate restriction pass.
-enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepBuf = 2 };
+// kStepWide is kStepAuto with the 64-bit slot select of wall-adjacent cells
+// that grids of 2^32 / 19 cells or more take, on any grid (for testing).
+enum StepVariant : int { kStepAuto = 0, kStepSingle = 1, kStepBuf = 2, kStepWide = 3 };
 
 struct StepArgs {
     float tau0 = 0.6f;

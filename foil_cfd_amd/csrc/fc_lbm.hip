@@ -236,12 +236,13 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
             uwz = C(halfBits(a.uwz[c]));
         }
         if constexpr (std::is_same_v<M<S>, PopPtr<S>>) {
-            // The 18 upstream kinds in one batch, then the 18 source slots in
-            // one batch (the slot picked per link without a branch: bounce-
-            // back, a mirror, or the plain pull again), so a wave holding a
-            // wall-adjacent cell waits for two more memory round trips, not
-            // for one or two per link.
-            // upstream kinds folded into three link masks as they arrive
+            // The 18 upstream kinds in one batch, folded into three link
+            // masks as they arrive; then the 18 source slots (bounce-back, a
+            // mirror, or the plain pull again, picked without a branch) in
+            // three groups of six, each group completed before the next (the
+            // empty asm) so at most six selected addresses are live. A wave
+            // holding a wall-adjacent cell waits for four memory round trips
+            // instead of one or two per link.
             unsigned solidM = 0, slipYM = 0, slipZM = 0;
             staticFor<kQ - 1>([&](auto qm) {
                 constexpr int q = decltype(qm)::value + 1;
@@ -250,31 +251,51 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
                 slipYM |= static_cast<unsigned>(nf == kSlipTop || nf == kSlipBottom) << q;
                 slipZM |= static_cast<unsigned>(nf == kSlipFront || nf == kSlipBack) << q;
             });
-            // in three groups of six links, each completed before the next
-            // (the empty asm) so at most six selected addresses are live
-            staticFor<3>([&](auto gc) {
-                constexpr int g = decltype(gc)::value;
-                staticFor<6>([&](auto jc) {
-                    constexpr int q = 1 + 6 * g + decltype(jc)::value;
-                    constexpr int cx = kCx[q], cy = kCy[q], cz = kCz[q];
-                    constexpr int my = kMirY[q], mz = kMirZ[q];
-                    using L = PullSlot<FROM_L, q>;
-                    const bool solid = (solidM >> q) & 1u;
-                    const bool slipY = (slipYM >> q) & 1u;
-                    const bool slipZ = (slipZM >> q) & 1u;
-                    // (direction slab, cell) of bounce-back, y mirror, z mirror, plain pull
-                    constexpr int dBB = FROM_L ? q : kOpp[q], dSY = FROM_L ? kOpp[my] : my;
-                    constexpr int dSZ = FROM_L ? kOpp[mz] : mz;
-                    const unsigned cBB = FROM_L ? nb(p, 0, 0, 0) : nb(p, -cx, -cy, -cz);
-                    const unsigned cSY = FROM_L ? nb(p, -cx, 0, -cz) : nb(p, 0, -cy, 0);
-                    const unsigned cSZ = FROM_L ? nb(p, -cx, -cy, 0) : nb(p, 0, 0, -cz);
-                    const unsigned cPL = nb(p, L::ex, L::ey, L::ez);
-                    const int d = solid ? dBB : (slipY ? dSY : (slipZ ? dSZ : L::dir));
-                    const unsigned cc = solid ? cBB : (slipY ? cSY : (slipZ ? cSZ : cPL));
-                    H[q] = S::load(pop.f + static_cast<long long>(d) * pop.n + cc);
+            // NARROW: the slot's element index in 32 bits (kQ * ncells <
+            // 2^32), else (direction slab, cell) with a 64-bit product. The
+            // narrow form takes 73 instead of 83-90 VGPRs in the plain-wall
+            // L->S step and is faster there; in the other variants the wide
+            // form uses fewer registers (62-72) and is faster (measured on
+            // MI355X), so they take it on every grid.
+            auto fetchLinks = [&](auto narrowTag) {
+                constexpr bool NARROW = decltype(narrowTag)::value;
+                staticFor<3>([&](auto gc) {
+                    constexpr int g = decltype(gc)::value;
+                    staticFor<6>([&](auto jc) {
+                        constexpr int q = 1 + 6 * g + decltype(jc)::value;
+                        constexpr int cx = kCx[q], cy = kCy[q], cz = kCz[q];
+                        constexpr int my = kMirY[q], mz = kMirZ[q];
+                        using L = PullSlot<FROM_L, q>;
+                        const bool solid = (solidM >> q) & 1u;
+                        const bool slipY = (slipYM >> q) & 1u;
+                        const bool slipZ = (slipZM >> q) & 1u;
+                        // (direction slab, cell) of bounce-back, y mirror, z mirror, plain pull
+                        constexpr int dBB = FROM_L ? q : kOpp[q], dSY = FROM_L ? kOpp[my] : my;
+                        constexpr int dSZ = FROM_L ? kOpp[mz] : mz;
+                        const unsigned cBB = FROM_L ? nb(p, 0, 0, 0) : nb(p, -cx, -cy, -cz);
+                        const unsigned cSY = FROM_L ? nb(p, -cx, 0, -cz) : nb(p, 0, -cy, 0);
+                        const unsigned cSZ = FROM_L ? nb(p, -cx, -cy, 0) : nb(p, 0, 0, -cz);
+                        const unsigned cPL = nb(p, L::ex, L::ey, L::ez);
+                        if constexpr (NARROW) {
+                            const unsigned n32 = static_cast<unsigned>(pop.n);
+                            const unsigned iBB = dBB * n32 + cBB, iSY = dSY * n32 + cSY;
+                            const unsigned iSZ = dSZ * n32 + cSZ, iPL = L::dir * n32 + cPL;
+                            H[q] = S::load(pop.f + (solid ? iBB : (slipY ? iSY : (slipZ ? iSZ : iPL))));
+                        } else {
+                            const int d = solid ? dBB : (slipY ? dSY : (slipZ ? dSZ : L::dir));
+                            const unsigned cc = solid ? cBB : (slipY ? cSY : (slipZ ? cSZ : cPL));
+                            H[q] = S::load(pop.f + static_cast<long long>(d) * pop.n + cc);
+                        }
+                    });
+                    staticFor<6>([&](auto jc) { pinLoaded(H[1 + 6 * g + decltype(jc)::value]); });
                 });
-                staticFor<6>([&](auto jc) { pinLoaded(H[1 + 6 * g + decltype(jc)::value]); });
-            });
+            };
+            if constexpr (WM || !FROM_L)
+                fetchLinks(std::false_type{});
+            else if (!a.wideIndex)
+                fetchLinks(std::true_type{});
+            else
+                fetchLinks(std::false_type{});
             if (WM) {
                 staticFor<kQ - 1>([&](auto qm) {
                     constexpr int q = decltype(qm)::value + 1;
@@ -1034,6 +1055,7 @@ struct StepOp {
         a.pre = sa.smagPrefactor;
         a.uIn = sa.uInlet;
         a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+        if (sa.variant == kStepWide) a.wideIndex = 1;
         return dispatchStep<S>(a, lat, layout, sa.variant, st);
     }
 };
@@ -1047,6 +1069,7 @@ struct StepRecordOp {
         a.pre = sa.smagPrefactor;
         a.uIn = sa.uInlet;
         a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+        if (sa.variant == kStepWide) a.wideIndex = 1;
         const bool wm = lat.uwx && lat.uwy && lat.uwz;
         const bool fromL = layout == kLayoutL;
         const dim3 g(rowGrid(a)), b(kBlock);

@@ -5,9 +5,13 @@
 // step kernels' own decomposition (one wave per 64-cell row segment, the
 // XCD slab remap, 19 direction slabs): every lane reads its 19 slots and
 // writes 19 slots, either all in its own cell (the S->L step: read (q, c),
-// write (opp q, c)) or shifted to the 18 neighbours (the L->S step: read
-// (opp q, c - e_q), write (q, c + e_q)). All directions wrap periodically,
-// so every address lies inside the array. An optional `work` count adds that
+// write (opp q, c)), shifted to the 18 neighbours (the L->S step: read
+// (opp q, c - e_q), write (q, c + e_q)), or in the "esoteric twist" pattern
+// (Geier and Schoenherr, Computation 5 (2017) 19), where every step reads and
+// writes slot q at c + max(e_q, 0) componentwise: the cell itself and its
+// six positive neighbours x+1, y+1, z+1 and their pairs, each slot read and
+// written by exactly one lane. All directions wrap periodically, so every
+// address lies inside the array. An optional `work` count adds that
 // many FP32 fused multiply-adds per population (19 independent chains)
 // between the read and the write, to see where arithmetic starts to cost
 // bandwidth. The
@@ -37,7 +41,11 @@ __device__ inline float fromWork<float>(float x) {
     return x;
 }
 
-template <class T, bool SHIFTED>
+enum Pattern { kLocal = 0, kShifted = 1, kTwist = 2 };
+
+constexpr int posPart(int c) { return c > 0 ? c : 0; }
+
+template <class T, int PAT>
 __global__ void __launch_bounds__(kBlock) aaTrafficKernel(KArgs a, int work) {
     RowPos p;
     if (!rowPos<true, true>(a, p)) return;
@@ -45,8 +53,13 @@ __global__ void __launch_bounds__(kBlock) aaTrafficKernel(KArgs a, int work) {
     T v[kQ];
     staticFor<kQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const long long i = SHIFTED ? static_cast<long long>(kOpp[q]) * a.ncells + nb(p, -kCx[q], -kCy[q], -kCz[q])
-                                    : static_cast<long long>(q) * a.ncells + nb(p, 0, 0, 0);
+        long long i;
+        if constexpr (PAT == kShifted)
+            i = static_cast<long long>(kOpp[q]) * a.ncells + nb(p, -kCx[q], -kCy[q], -kCz[q]);
+        else if constexpr (PAT == kTwist)
+            i = static_cast<long long>(q) * a.ncells + nb(p, posPart(kCx[q]), posPart(kCy[q]), posPart(kCz[q]));
+        else
+            i = static_cast<long long>(q) * a.ncells + nb(p, 0, 0, 0);
         v[q] = f[i];
     });
     if (work > 0) {
@@ -67,8 +80,13 @@ __global__ void __launch_bounds__(kBlock) aaTrafficKernel(KArgs a, int work) {
     }
     staticFor<kQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const long long i = SHIFTED ? static_cast<long long>(q) * a.ncells + nb(p, kCx[q], kCy[q], kCz[q])
-                                    : static_cast<long long>(kOpp[q]) * a.ncells + nb(p, 0, 0, 0);
+        long long i;
+        if constexpr (PAT == kShifted)
+            i = static_cast<long long>(q) * a.ncells + nb(p, kCx[q], kCy[q], kCz[q]);
+        else if constexpr (PAT == kTwist)
+            i = static_cast<long long>(q) * a.ncells + nb(p, posPart(kCx[q]), posPart(kCy[q]), posPart(kCz[q]));
+        else
+            i = static_cast<long long>(kOpp[q]) * a.ncells + nb(p, 0, 0, 0);
         f[i] = v[q];
     });
 }
@@ -104,7 +122,7 @@ int timeLaunches(L launch, int reps, double* ms) {
 }
 
 template <class T>
-int runAA(fc_dims d, int shifted, int work, int reps, double* ms) {
+int runAA(fc_dims d, int pattern, int work, int reps, double* ms) {
     LatticeView lat;
     lat.dims = Dims{d.nx, d.ny, d.nz};
     const std::size_t bytes = static_cast<std::size_t>(kQ) * static_cast<std::size_t>(lat.dims.cells()) * sizeof(T);
@@ -115,10 +133,12 @@ int runAA(fc_dims d, int shifted, int work, int reps, double* ms) {
     KArgs a = makeArgs(lat);
     const dim3 g(static_cast<unsigned>(a.chunk) * kXcds), b(kBlock);
     if (rc == FC_OK) {
-        if (shifted)
-            rc = timeLaunches([&] { aaTrafficKernel<T, true><<<g, b>>>(a, work); }, reps, ms);
+        if (pattern == kShifted)
+            rc = timeLaunches([&] { aaTrafficKernel<T, kShifted><<<g, b>>>(a, work); }, reps, ms);
+        else if (pattern == kTwist)
+            rc = timeLaunches([&] { aaTrafficKernel<T, kTwist><<<g, b>>>(a, work); }, reps, ms);
         else
-            rc = timeLaunches([&] { aaTrafficKernel<T, false><<<g, b>>>(a, work); }, reps, ms);
+            rc = timeLaunches([&] { aaTrafficKernel<T, kLocal><<<g, b>>>(a, work); }, reps, ms);
     }
     (void)hipFree(f);
     return rc;
@@ -129,12 +149,13 @@ int runAA(fc_dims d, int shifted, int work, int reps, double* ms) {
 
 extern "C" {
 
-int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int shifted, int work, int reps, double* ms_per_launch) {
-    if (!ms_per_launch || dims.nx < 3 || dims.ny < 3 || dims.nz < 3 || reps < 1 || work < 0 ||
+int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int pattern, int work, int reps, double* ms_per_launch) {
+    if (!ms_per_launch || dims.nx < 3 || dims.ny < 3 || dims.nz < 3 || reps < 1 || work < 0 || pattern < 0 ||
+        pattern > fc::kTwist ||
         static_cast<long long>(dims.nx) * dims.ny * dims.nz >= (1ll << 31))
         return FC_ERR_INVALID;
-    if (elem_bytes == 2) return fc::runAA<unsigned short>(dims, shifted, work, reps, ms_per_launch);
-    if (elem_bytes == 4) return fc::runAA<float>(dims, shifted, work, reps, ms_per_launch);
+    if (elem_bytes == 2) return fc::runAA<unsigned short>(dims, pattern, work, reps, ms_per_launch);
+    if (elem_bytes == 4) return fc::runAA<float>(dims, pattern, work, reps, ms_per_launch);
     return FC_ERR_INVALID;
 }
 

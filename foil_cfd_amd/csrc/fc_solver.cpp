@@ -343,7 +343,7 @@ public:
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
     int setStepVariant(int v) {
-        if (v != kStepAuto && v != kStepSingle && v != kStepBuf)
+        if (v != kStepAuto && v != kStepSingle && v != kStepBuf && v != kStepWide)
             return fail(FC_ERR_INVALID, "unknown step variant");
         stepVariant_ = v;
         return FC_OK;
@@ -1938,7 +1938,8 @@ public:
         const LatticeView cv = view();
         const LatticeView fv = fineView();
         const bool records = m == 2 && !finer_.active && fine_.recordsOk && fine_.dRestrict &&
-                             fa.variant == kStepAuto && stepRecordsSupported(fv, g) != 0;
+                             (fa.variant == kStepAuto || fa.variant == kStepWide) &&
+                             stepRecordsSupported(fv, g) != 0;
         for (int k = 0; k < m; ++k) {
             if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
                                static_cast<float>(k) / static_cast<float>(m), neqScale, 0, fine_.dFillCache,

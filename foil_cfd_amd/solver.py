@@ -441,9 +441,10 @@ class LBMSolver:
 
     def set_step_variant(self, variant: str):
         """"auto" (library default), "single" (the default cell kernel on every
-        sub-step, restriction records from the separate pass) or "buf" (one
-        cell per lane with buffer addressing)."""
-        v = {"auto": 0, "single": 1, "buf": 2}[variant]
+        sub-step, restriction records from the separate pass), "buf" (one
+        cell per lane with buffer addressing) or "wide" ("auto" with the
+        64-bit population index of very large grids forced)."""
+        v = {"auto": 0, "single": 1, "buf": 2, "wide": 3}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
     def set_force_ema_window(self, ft: float):

@@ -202,9 +202,11 @@ int fc_set_smagorinsky_cs(fc_solver* s, float cs);
  * FC_STEP_SINGLE runs the same cell kernel on every sub-step and leaves the
  * records to the separate restriction pass. FC_STEP_BUF runs the one-cell
  * kernel with 32-bit buffer addressing (population arrays under 2 GiB, the
- * default kernel beyond) and the separate restriction pass. All three give
- * the same cell update, bit for bit. */
-enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_BUF = 2 };
+ * default kernel beyond) and the separate restriction pass. FC_STEP_WIDE is
+ * FC_STEP_AUTO with the 64-bit population index that grids of 2^32 / 19
+ * cells or more use for wall-adjacent cells, forced on any grid (to test
+ * that path). All four give the same cell update, bit for bit. */
+enum { FC_STEP_AUTO = 0, FC_STEP_SINGLE = 1, FC_STEP_BUF = 2, FC_STEP_WIDE = 3 };
 int fc_set_step_variant(fc_solver* s, int variant);
 
 /* ---- stepping ------------------------------------------------------------ */
@@ -684,12 +686,14 @@ int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms);
  * allocates a 19-slab array of dims cells with elem_bytes (2 or 4) per
  * population, then times `reps` launches of a kernel with the step kernels'
  * decomposition that moves exactly one in-place step's bytes (19 reads and 19
- * writes per cell): shifted = 0 is the local S->L pattern, 1 the L->S pattern
- * with neighbour reads and writes; `work` FP32 FMAs per population (19
+ * writes per cell): pattern = 0 is the local S->L pattern, 1 the L->S pattern
+ * with neighbour reads and writes, 2 the esoteric-twist pattern (slot q read
+ * and written at cell + max(e_q, 0), the same for every step); `work` FP32
+ * FMAs per population (19
  * independent chains) sit between the read and the write (0: none). The
  * copy probe times a plain device copy of `bytes`. Both free what they
  * allocate; ms per launch. */
-int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int shifted, int work, int reps, double* ms_per_launch);
+int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int pattern, int work, int reps, double* ms_per_launch);
 int fc_probe_copy(size_t bytes, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus

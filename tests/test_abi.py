@@ -63,3 +63,13 @@ def test_init_without_device_fails_cleanly(fclib):
     fclib.fc_destroy(h)
     assert rc == _abi.FC_ERR_DEVICE
     assert b"device" in err.value
+
+
+def test_probe_rejects_bad_arguments_before_touching_the_device(fclib):
+    # Argument checks come first, so these run without a GPU.
+    ms = ctypes.c_double(0.0)
+    d = _abi.FcDims(64, 8, 8)
+    for eb, pattern, work, reps in ((3, 0, 0, 1), (2, 3, 0, 1), (2, -1, 0, 1), (2, 0, -1, 1), (2, 0, 0, 0)):
+        assert fclib.fc_probe_aa_traffic(d, eb, pattern, work, reps, ctypes.byref(ms)) == _abi.FC_ERR_INVALID
+    assert fclib.fc_probe_aa_traffic(_abi.FcDims(2, 8, 8), 2, 0, 0, 1, ctypes.byref(ms)) == _abi.FC_ERR_INVALID
+    assert fclib.fc_probe_copy(8, 1, ctypes.byref(ms)) == _abi.FC_ERR_INVALID

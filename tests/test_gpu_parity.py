@@ -291,11 +291,12 @@ def test_reduced_storage_vs_fp64(gpu, storage, tol):
     ref.close()
 
 
-@pytest.mark.parametrize("variant", ["buf"])
+@pytest.mark.parametrize("variant", ["buf", "wide"])
 @pytest.mark.parametrize("storage", ["f16c", "f16", "bf16", "f32"])
 @pytest.mark.parametrize("flow", ["tunnel", "tunnel+wall", "periodic"])
 def test_step_variant_matches_default_kernel(gpu, storage, flow, variant):
-    """The buffer-addressed one-cell step ("buf") runs the same cell update
+    """The buffer-addressed one-cell step ("buf") and the step with the 64-bit
+    population index of very large grids ("wide") run the same cell update
     as the default one-cell-per-lane step: populations and macro fields
     bit-identical after
     an odd number of steps (both AA layouts), on a tunnel with boundary

@@ -172,17 +172,20 @@ def test_refined_run_stays_finite_and_counts_updates(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("storage,wall", [("f32", "off"), ("f16c", "off"), ("f16c", "newton"), ("f64", "off")])
-def test_record_writing_fine_step_matches_separate_pass(gpu, storage, wall):
+@pytest.mark.parametrize("storage,wall,variant", [("f32", "off", "auto"), ("f16c", "off", "auto"),
+                                                  ("f16c", "newton", "auto"), ("f64", "off", "auto"),
+                                                  ("f16c", "off", "wide"), ("f32", "off", "wide")])
+def test_record_writing_fine_step_matches_separate_pass(gpu, storage, wall, variant):
     """With the default step variant the last fine sub-step of each coarse
     step writes the restriction's child-average records from its registers;
     the single-cell variant keeps the separate child-average pass over the
     fine array. Coarse and fine states and the macro fields must agree bit
-    for bit after several coupled steps."""
-    say(f"case: refined tunnel 160x64x8 {storage} wall={wall}, record-writing fine step vs separate pass")
+    for bit after several coupled steps ("wide": the default with the 64-bit
+    population index of very large grids forced)."""
+    say(f"case: refined tunnel 160x64x8 {storage} wall={wall}, {variant} record-writing fine step vs separate pass")
     case, box, fine = refined_case()
     out = []
-    for variant in ("auto", "single"):
+    for variant in (variant, "single"):
         s = cases.make_solver(case, storage)
         s.init_refinement(box, 2, fine)
         if wall != "off":
