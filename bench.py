@@ -232,6 +232,7 @@ def accuracy_run(args):
     if args.tau:  # same grid, patch and models at a lower Reynolds number
         case = cases.naca_tunnel(512, 192, 240, 256, aoa=args.aoa, u_lat=0.08, cs=0.1733, tau=args.tau)
     s, box, info = full_setup(case, storage, args.aoa, wall=args.wall)
+    s.set_stochastic_rounding(args.rounding == "stochastic")
     ft = fc.flow_through_steps(case.scaling, case.dims[0])
     total = int(round(args.flow_throughs * ft / args.per_call)) * args.per_call
     span, fscale = info["fine_dims"][2], info["fine_scaling"]
@@ -240,6 +241,7 @@ def accuracy_run(args):
            "flow_through_steps": ft, "steps_planned": total, "patch_box": list(box),
            "fine_dims": list(info["fine_dims"]), "forces_from_fine": info["forces_from_fine"],
            "tau": case.scaling.tau, "fine_tau": fscale.tau, "re_chord": case.scaling.re_effective,
+           "rounding": args.rounding if storage in ("f16c", "f16", "bf16") else "exact",
            "samples": []}
     t0 = time.perf_counter()
     seed = 1
@@ -288,7 +290,10 @@ def accuracy_summary(paths, out_path, batches=6):
     for p in paths:
         with open(p) as fh:
             r = json.load(fh)
-        runs.setdefault(f"{r['aoa']:g}", {})[r["storage"]] = r
+        # 16-bit records without a rounding field predate stochastic rounding
+        rnd = r.get("rounding", "nearest" if r["storage"] in ("f16c", "f16", "bf16") else "exact")
+        name = r["storage"] + ("_nearest" if rnd == "nearest" else "")
+        runs.setdefault(f"{r['aoa']:g}", {})[name] = r
     out = {"workload": "naca2412_wing_512x192x240_chord256_trt_les_imem_patch2x_particles1M",
            "protocol": "one momentum-exchange sample per 100 coarse steps; average over samples at or after "
                        "1 flow-through (reference readout), 4 flow-throughs run; standard error from batch means",
@@ -305,7 +310,8 @@ def accuracy_summary(paths, out_path, batches=6):
             ft = r["flow_through_steps"]
             m = s[:, 0] >= ft
             e = {"status": r.get("status"), "steps": int(r.get("steps_done", 0)), "wall_s": r.get("wall_s"),
-                 "samples": int(m.sum())}
+                 "samples": int(m.sum()), "rounding": "nearest" if st.endswith("_nearest") else (
+                     "stochastic" if st in ("f16c", "f16", "bf16") else "exact")}
             for name, col in (("cl", 3), ("cd", 4)):
                 v = s[m, col]
                 bm = np.array([b.mean() for b in np.array_split(v, batches)])
@@ -342,20 +348,30 @@ def accuracy_summary(paths, out_path, batches=6):
                       for k, ent in out["aoa"].items()}), flush=True)
 
 
+ACCURACY_LAMINAR_PROFILE = os.path.join(ROOT, "profiles", "wing_cl_cd_accuracy_re1000.json")
+
+
 def accuracy_line():
-    """The committed accuracy summary, reduced to the bench line's figures."""
-    try:
-        with open(ACCURACY_PROFILE) as fh:
-            rec = json.load(fh)
-    except (OSError, ValueError):
-        return None
-    res = {}
-    for aoa, ent in rec.get("aoa", {}).items():
-        for st, e in ent.items():
-            if "cl_err_vs_f64" in e:
-                res.setdefault(f"aoa{aoa}", {})[st] = {k: e[k] for k in ("cl_err_vs_f64", "cd_err_vs_f64",
-                                                                        "cl_err_sigma", "cd_err_sigma")}
-    return {"per_aoa": res, "source": "profiles/wing_cl_cd_accuracy.json"} if res else None
+    """The committed accuracy summaries (the LES case at the reference's
+    Reynolds number and the laminar one at chord Re ~1000), reduced to the
+    bench line's figures: per angle and storage the Cl / Cd error against
+    FP64 and its one-sigma uncertainty from batch means."""
+    out = {}
+    for key, path in (("les", ACCURACY_PROFILE), ("laminar_re1000", ACCURACY_LAMINAR_PROFILE)):
+        try:
+            with open(path) as fh:
+                rec = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        res = {}
+        for aoa, ent in rec.get("aoa", {}).items():
+            for st, e in ent.items():
+                if "cl_err_vs_f64" in e:
+                    res.setdefault(f"aoa{aoa}", {})[st] = {k: e[k] for k in ("cl_err_vs_f64", "cd_err_vs_f64",
+                                                                            "cl_err_sigma", "cd_err_sigma")}
+        if res:
+            out[key] = {"per_aoa": res, "source": os.path.relpath(path, ROOT)}
+    return out or None
 
 
 def cpu_baseline(case, seconds=12.0):
@@ -605,6 +621,8 @@ def main():
     ap.add_argument("--summary-out", default=None,
                     help="--accuracy-summary target (default profiles/wing_cl_cd_accuracy.json)")
     ap.add_argument("--wall", default="newton", choices=["off", "newton", "mlp"])
+    ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest"],
+                    help="--accuracy: rounding of 16-bit population stores")
     ap.add_argument("--out", default="accuracy.json")
     ap.add_argument("--accuracy-summary", nargs="+", metavar="RECORD",
                     help="fold --accuracy records into the committed accuracy summary (no GPU)")
@@ -689,6 +707,7 @@ def main():
         traffic = measured_traffic(head, "fine")
         bytes_per_launch = r["kernel_bytes"]
     config.update({"collision": "TRT(3/16) + Smagorinsky(0.1733)", "streaming": "in-place AA",
+                   "rounding_16bit": "stochastic (hash of cell, direction, step)",
                    "step_variant": args.variant, "steps_per_call": min(args.per_call, args.steps),
                    "parallelism": "single_gpu"})
     out = {

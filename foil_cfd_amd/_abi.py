@@ -191,6 +191,8 @@ SIGNATURES = {
     "fc_set_force_ema_window": (C.c_int, [_vp, C.c_float]),
     "fc_set_smagorinsky_cs": (C.c_int, [_vp, C.c_float]),
     "fc_set_step_variant": (C.c_int, [_vp, C.c_int]),
+    "fc_set_stochastic_rounding": (C.c_int, [_vp, C.c_int]),
+    "fc_get_stochastic_rounding": (C.c_int, [_vp]),
     "fc_last_error": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "fc_init_refinement": (C.c_int, [_vp, C.POINTER(FcPatchBox), C.c_int, _u8p, C.c_char_p,
                                      C.c_int]),

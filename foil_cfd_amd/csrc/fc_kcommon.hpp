@@ -43,7 +43,18 @@ struct KArgs {
     // plain-wall step then takes the 64-bit slot select. Also set on request
     // (the wide step variant) so that path runs on small grids.
     int wideIndex;
+    // Stochastic rounding of the 16-bit population stores (1) or round to
+    // nearest (0), and this launch's seed for the random bits.
+    int stochastic;
+    unsigned roundSeed;
 };
+
+// Random bits for the stochastic rounding of one cell's stores in one step:
+// roundHash (fc_layout.hpp) of the cell index and the launch seed, then a
+// Weyl sequence over the 19 directions (the storage codecs use the top
+// bits). A pure function of (cell, direction, seed), so every kernel variant
+// that stores a cell's populations draws the same bits.
+__device__ inline unsigned roundBits(unsigned h, int q) { return h + static_cast<unsigned>(q) * 0x9E3779B9u; }
 
 inline unsigned long long divMagic(long long d) {
     return d > 0 && d < (1 << 16) ? ((1ull << 40) + static_cast<unsigned long long>(d) - 1) /
@@ -186,6 +197,22 @@ __device__ inline bool inDomain(const KArgs& a, const RowPos& p, int ex, int ey)
 //   scalar add and no per-lane address arithmetic. Every offset must stay
 //   below 2^31 bytes, so the host only selects it for arrays under 2 GiB.
 // ---------------------------------------------------------------------------
+// Storage bits of a value: the 16-bit formats round stochastically with the
+// random bits r when sto is set (else, and for 32 / 64 bits, to nearest).
+template <class S>
+__device__ inline typename S::Bits roundedBits(typename S::C v, bool sto, unsigned r) {
+    if constexpr (sizeof(typename S::T) == 2)
+        return sto ? S::toBitsSR(v, r) : S::toBits(v);
+    else
+        return S::toBits(v);
+}
+
+// A value as it reads back after a rounded store.
+template <class S>
+__device__ inline typename S::C roundTrip(typename S::C v, bool sto, unsigned r) {
+    return S::fromBits(roundedBits<S>(v, sto, r));
+}
+
 template <class S>
 struct PopPtr {
     using T = typename S::T;
@@ -199,6 +226,9 @@ struct PopPtr {
     }
     __device__ void st(int q, int ex, int ey, int ez, C v) const {
         S::store(f + static_cast<long long>(q) * n + nb(*p, ex, ey, ez), v);
+    }
+    __device__ void stBits(int q, int ex, int ey, int ez, typename S::Bits b) const {
+        reinterpret_cast<typename S::Bits*>(f)[static_cast<long long>(q) * n + nb(*p, ex, ey, ez)] = b;
     }
 };
 
@@ -254,13 +284,14 @@ struct PopBuf {
         else
             return static_cast<Bits>(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, kPopLoadPolicy));
     }
-    __device__ void st(int q, int ex, int ey, int ez, C v) const {
+    __device__ void st(int q, int ex, int ey, int ez, C v) const { stBits(q, ex, ey, ez, S::toBits(v)); }
+    __device__ void stBits(int q, int ex, int ey, int ez, Bits b) const {
         const int vo = static_cast<int>(vx[ex + 1]);
         const int so = static_cast<int>(static_cast<unsigned>(q) * slab + row[ey + 1][ez + 1]);
         if constexpr (sizeof(Bits) == 2)
-            __builtin_amdgcn_raw_buffer_store_b16(S::toBits(v), rs, vo, so, kPopStorePolicy);
+            __builtin_amdgcn_raw_buffer_store_b16(b, rs, vo, so, kPopStorePolicy);
         else
-            __builtin_amdgcn_raw_buffer_store_b32(S::toBits(v), rs, vo, so, kPopStorePolicy);
+            __builtin_amdgcn_raw_buffer_store_b32(b, rs, vo, so, kPopStorePolicy);
     }
 
 };

@@ -33,6 +33,10 @@ struct StepArgs {
     float uInlet = 0.08f;
     int writeMacro = 0;
     int variant = kStepAuto;
+    // 16-bit storages: stochastic rounding of the population stores (1) or
+    // round to nearest (0); the seed of this launch's random bits.
+    int stochastic = 0;
+    unsigned roundSeed = 0;
 };
 
 // Device view of one lattice level (non-owning).

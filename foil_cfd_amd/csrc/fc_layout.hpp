@@ -87,4 +87,17 @@ inline constexpr int kWavesPerBlock = 4;
 inline constexpr int kBlock = kWave * kWavesPerBlock;
 inline constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin
 
+// Integer hash of the stochastic rounding's random bits (lowbias32:
+// xorshift-multiply rounds with the constants of C. Wellons' hash-prospector
+// search) of a value and a seed; constexpr, so host and device share it.
+constexpr unsigned roundHash(unsigned c, unsigned seed) {
+    unsigned x = c ^ seed;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
 }  // namespace fc

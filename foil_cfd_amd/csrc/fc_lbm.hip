@@ -172,9 +172,22 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
     // Store f*_q of this cell into the output layout (scatter writes are
     // bounds checked only for boundary cells; fluid cells never leave the
     // domain because every non-periodic face plane is boundary-flagged).
+    // 16-bit storages round stochastically when the launch asks for it, with
+    // the cell's random bits for this step.
+    constexpr bool kNarrow = sizeof(T) == 2;
+    const bool sto = kNarrow && a.stochastic;
+    const unsigned rh = kNarrow ? roundHash(c, a.roundSeed) : 0u;
     auto storeOut = [&](auto qc, C val, bool check) {
         constexpr int q = decltype(qc)::value;
-        if (FROM_L) {
+        if constexpr (kNarrow) {
+            const typename S::Bits b = roundedBits<S>(val, sto, roundBits(rh, q));
+            if (FROM_L) {
+                if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
+                pop.stBits(q, kCx[q], kCy[q], kCz[q], b);
+            } else {
+                pop.stBits(kOpp[q], 0, 0, 0, b);
+            }
+        } else if (FROM_L) {
             if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
             pop.st(q, kCx[q], kCy[q], kCz[q], val);
         } else {
@@ -337,10 +350,17 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
     const Macro<C> mac = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
 
     staticFor<kQ>([&](auto qc) { storeOut(qc, H[decltype(qc)::value], false); });
-    if (k & kKindFeed) {
+    if (k & kKindFeed) {  // the outlet's copy, rounded exactly as the slot store
         T* dstFeed = static_cast<T*>(a.feed) + newLayout * kQ * plane;
+        if constexpr (kNarrow) {
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) S::store(dstFeed + q * plane + yz, H[q]);
+            for (int q = 0; q < kQ; ++q)
+                reinterpret_cast<typename S::Bits*>(dstFeed)[q * plane + yz] =
+                    roundedBits<S>(H[q], sto, roundBits(rh, q));
+        } else {
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) S::store(dstFeed + q * plane + yz, H[q]);
+        }
     }
     if (a.writeMacro) {
         a.rho[c] = static_cast<float>(mac.rho);
@@ -438,12 +458,17 @@ struct RecArgs {
     int rw, rh;
 };
 
-// A value as it reads back from the fine array: stored, loaded, unscaled.
+// A value as it reads back from the fine array: stored (rounded as the step
+// stored it), loaded, unscaled.
 template <class S>
-__device__ inline float storedDeviation(typename S::C v) {
-    typename S::T t;
-    S::store(&t, v);
-    return static_cast<float>(S::load(&t)) / S::K;
+__device__ inline float storedDeviation(typename S::C v, bool sto, unsigned r) {
+    if constexpr (sizeof(typename S::T) == 2) {
+        return static_cast<float>(roundTrip<S>(v, sto, r)) / S::K;
+    } else {
+        typename S::T t;
+        S::store(&t, v);
+        return static_cast<float>(S::load(&t)) / S::K;
+    }
 }
 
 template <class S, template <class> class M, bool FROM_L, bool WM>
@@ -475,8 +500,10 @@ stepRecordKernel(KArgs a, RecArgs rg) {
     if (yp < rg.cy0 || yp >= rg.cy1) return;  // workgroup-uniform: rows outside the restricted box
 
     const bool fluid = live && (k & kKindFlagMask) == kFluid;
+    const bool sto = sizeof(typename S::T) == 2 && a.stochastic;
+    const unsigned rh = fluid ? roundHash(nb(p, 0, 0, 0), a.roundSeed) : 0u;
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) sH[wv][q][lane] = fluid ? storedDeviation<S>(H[q]) : 0.0f;
+    for (int q = 0; q < kQ; ++q) sH[wv][q][lane] = fluid ? storedDeviation<S>(H[q], sto, roundBits(rh, q)) : 0.0f;
     sLive[wv][lane] = (live && (k & kKindFlagMask) != kSolid) ? 1 : 0;
     __syncthreads();
     if (wv != 0) return;
@@ -1056,6 +1083,8 @@ struct StepOp {
         a.uIn = sa.uInlet;
         a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
         if (sa.variant == kStepWide) a.wideIndex = 1;
+        a.stochastic = sa.stochastic;
+        a.roundSeed = sa.roundSeed;
         return dispatchStep<S>(a, lat, layout, sa.variant, st);
     }
 };
@@ -1070,6 +1099,8 @@ struct StepRecordOp {
         a.uIn = sa.uInlet;
         a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
         if (sa.variant == kStepWide) a.wideIndex = 1;
+        a.stochastic = sa.stochastic;
+        a.roundSeed = sa.roundSeed;
         const bool wm = lat.uwx && lat.uwy && lat.uwz;
         const bool fromL = layout == kLayoutL;
         const dim3 g(rowGrid(a)), b(kBlock);
@@ -1360,6 +1391,8 @@ int buildGroupEntry(const LatticeView& lat, int layout, const StepArgs& sa, int 
     e.a.pre = sa.smagPrefactor;
     e.a.uIn = sa.uInlet;
     e.a.writeMacro = (sa.writeMacro && lat.rho && lat.u && lat.v && lat.w) ? 1 : 0;
+    e.a.stochastic = sa.stochastic;
+    e.a.roundSeed = sa.roundSeed;
     e.fromL = layout == kLayoutL ? 1 : 0;
     e.wm = (lat.uwx && lat.uwy && lat.uwz) ? 1 : 0;
     e.taskBase = taskBase;

@@ -30,8 +30,9 @@
 namespace fc {
 // Bridge into the solver object (defined with it).
 bool solverReady(fc_solver* s);
-void solverStateRegions(fc_solver* s, std::vector<std::pair<void*, std::size_t>>& out, int layouts[3]);
-int solverStateRestored(fc_solver* s, const int layouts[3]);
+void solverStateRegions(fc_solver* s, std::vector<std::pair<void*, std::size_t>>& out, int layouts[3],
+                        unsigned long long* roundSeq);
+int solverStateRestored(fc_solver* s, const int layouts[3], unsigned long long roundSeq);
 void* solverStream(fc_solver* s);
 }  // namespace fc
 
@@ -262,6 +263,7 @@ struct fc_vram_snapshot {
     std::size_t bytes = 0;
     std::vector<std::size_t> sizes;  // region sizes at capture (the grid's byte shape)
     int layouts[3] = {0, -1, -1};
+    unsigned long long roundSeq = 0;  // the solver's rounding sequence position
     int storage = -1;
     fc_dims dims{0, 0, 0};
 
@@ -432,7 +434,8 @@ int fc_vram_snapshot_capture(fc_vram_snapshot* snap, fc_solver* s, const fc_snap
     }
     std::vector<std::pair<void*, std::size_t>> regions;
     int layouts[3];
-    fc::solverStateRegions(s, regions, layouts);
+    unsigned long long roundSeq = 0;
+    fc::solverStateRegions(s, regions, layouts, &roundSeq);
     std::vector<std::size_t> sizes;
     std::size_t need = 0;
     for (const auto& r : regions) {
@@ -471,6 +474,7 @@ int fc_vram_snapshot_capture(fc_vram_snapshot* snap, fc_solver* s, const fc_snap
     snap->key = Key::from(*key);
     snap->sizes = sizes;
     std::copy(layouts, layouts + 3, snap->layouts);
+    snap->roundSeq = roundSeq;
     snap->storage = fc_storage(s);
     snap->hasData = true;
     return FC_OK;
@@ -483,7 +487,8 @@ int fc_vram_snapshot_restore(fc_vram_snapshot* snap, fc_solver* s) {
     fc_get_dims(s, &d);
     std::vector<std::pair<void*, std::size_t>> regions;
     int layouts[3];
-    fc::solverStateRegions(s, regions, layouts);
+    unsigned long long roundSeq = 0;
+    fc::solverStateRegions(s, regions, layouts, &roundSeq);
     // The grid must be byte-identical (same dims, storage and level shapes).
     if (!snap->key.sameGrid(d) || d.nx != snap->dims.nx || d.ny != snap->dims.ny || d.nz != snap->dims.nz ||
         fc_storage(s) != snap->storage || regions.size() != snap->sizes.size())
@@ -498,7 +503,7 @@ int fc_vram_snapshot_restore(fc_vram_snapshot* snap, fc_solver* s) {
             return FC_ERR_DEVICE;
         off += (r.second + 255) & ~static_cast<std::size_t>(255);
     }
-    return fc::solverStateRestored(s, snap->layouts);
+    return fc::solverStateRestored(s, snap->layouts, snap->roundSeq);
 }
 
 int fc_vram_snapshot_has_data(const fc_vram_snapshot* s) { return s && s->hasData ? 1 : 0; }

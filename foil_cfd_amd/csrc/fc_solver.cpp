@@ -286,6 +286,7 @@ public:
     int reset() {
         if (!initialized_) return fail(FC_ERR_STATE, "solver not initialized");
         steps_ = 0;
+        roundSeq_ = 0;
         layout_ = kLayoutL;
         rampActive_ = true;
         rampStartStep_ = 0;
@@ -342,6 +343,16 @@ public:
     void setRampEnabled(bool on) { rampEnabled_ = on; }
     void setEmaWindow(float ft) { emaWindowFlowThroughs_ = std::max(ft, 1e-3f); }
     void setCs(float cs) { cs_ = std::max(cs, 0.0f); }
+    void setStochasticRounding(bool on) { stochastic_ = on; }
+    bool stochasticRounding() const { return stochastic_; }
+    // Seed of the rounding bits of one step launch: level 0 is the coarse
+    // step of coarse step `seq`, level 1 fine sub-step `sub` of it (nested
+    // levels hash their parent's seed further).
+    static unsigned roundSeed(unsigned long long seq, unsigned level, unsigned sub) {
+        const unsigned hi = roundHash(static_cast<unsigned>(seq >> 32), 0x68E31DA4u);
+        return roundHash(static_cast<unsigned>(seq), hi ^ (level * 0x85EBCA6Bu + sub * 0xC2B2AE35u + 0x27D4EB2Fu));
+    }
+
     int setStepVariant(int v) {
         if (v != kStepAuto && v != kStepSingle && v != kStepBuf && v != kStepWide)
             return fail(FC_ERR_INVALID, "unknown step variant");
@@ -365,6 +376,8 @@ public:
             a.uInlet = effectiveU();
             a.writeMacro = (i == n - 1) ? 1 : 0;
             a.variant = stepVariant_;
+            a.stochastic = stochastic_ ? 1 : 0;
+            a.roundSeed = roundSeed(roundSeq_, 0, 0);
             // The coarse step overwrites time t in place: keep the ring the
             // fine shell fill interpolates from (time t) first, before the
             // q-LIBB pre-pass rewrites any bounce-back slot.
@@ -382,6 +395,7 @@ public:
             }
             layout_ = 1 - layout_;
             ++steps_;
+            ++roundSeq_;
             if (++stepsSinceWatchdog_ >= kWatchdogPeriodSteps && !nanPending_) {
                 stepsSinceWatchdog_ = 0;
                 (void)hipMemsetAsync(dNan_, 0, sizeof(int), stream_);
@@ -454,6 +468,8 @@ public:
         a.smagPrefactor = smagPrefactor(cs_);
         a.uInlet = uAt(steps_ + i);
         a.writeMacro = (i == n - 1) ? 1 : 0;
+        a.stochastic = stochastic_ ? 1 : 0;
+        a.roundSeed = roundSeed(roundSeq_ + static_cast<unsigned long long>(i), 0, 0);
         const int lay = (i & 1) ? 1 - layout_ : layout_;
         return buildGroupEntry(view(), lay, a, taskBase, out);
     }
@@ -462,6 +478,7 @@ public:
     int groupEnd(int n) {
         if (n & 1) layout_ = 1 - layout_;
         steps_ += n;
+        roundSeq_ += static_cast<unsigned long long>(n);
         stepsSinceWatchdog_ += n;
         if (stepsSinceWatchdog_ >= kWatchdogPeriodSteps && !nanPending_) {
             stepsSinceWatchdog_ = 0;
@@ -1061,7 +1078,9 @@ public:
     // Exact state of the populations: the device regions an exact snapshot
     // copies (coarse f, outlet feed planes, then the fine and finer levels'
     // f when active) and the layout parity of each level.
-    void stateRegions(std::vector<std::pair<void*, std::size_t>>& out, int layouts[3]) const {
+    void stateRegions(std::vector<std::pair<void*, std::size_t>>& out, int layouts[3],
+                      unsigned long long* roundSeq) const {
+        *roundSeq = roundSeq_;
         const std::size_t sb = storageBytes(storage_);
         const std::size_t plane = static_cast<std::size_t>(dims_.ny) * dims_.nz;
         out.clear();
@@ -1079,9 +1098,10 @@ public:
         }
     }
     // After an exact snapshot's copies landed on this solver's stream.
-    int stateRestored(const int layouts[3]) {
+    int stateRestored(const int layouts[3], unsigned long long roundSeq) {
         if (hipStreamSynchronize(stream_) != hipSuccess) return fail(FC_ERR_DEVICE, "snapshot restore failed");
         layout_ = layouts[0];
+        roundSeq_ = roundSeq;  // the restored run draws the rounding bits the captured one drew
         if (fine_.active) fine_.layout = layouts[1];
         if (finer_.active) finer_.layout = layouts[2];
         notifySnapshotRestored(true);
@@ -1941,6 +1961,7 @@ public:
                              (fa.variant == kStepAuto || fa.variant == kStepWide) &&
                              stepRecordsSupported(fv, g) != 0;
         for (int k = 0; k < m; ++k) {
+            fa.roundSeed = roundSeed(roundSeq_, 1, static_cast<unsigned>(k));
             if (launchFineFill(cv, cLayout, fine_.dBand, fine_.dBandIndex, fine_.nband, fv, fine_.layout, g,
                                static_cast<float>(k) / static_cast<float>(m), neqScale, 0, fine_.dFillCache,
                                k == 0 ? 1 : 2, stream_) != 0)
@@ -2049,6 +2070,7 @@ public:
         const RefineGeom g = geomOf(fine_.dims, finer_);
         const LatticeView fv = viewOf(fine_), gv = viewOf(finer_);
         for (int q = 0; q < m2; ++q) {
+            ga.roundSeed = roundHash(fineArgs.roundSeed, 0x2545F491u + static_cast<unsigned>(q));
             if (launchFineFill(fv, fLayout, finer_.dBand, finer_.dBandIndex, finer_.nband, gv, finer_.layout, g,
                                static_cast<float>(q) / static_cast<float>(m2), neqScale, 0, finer_.dFillCache,
                                q == 0 ? 1 : 2, stream_) != 0)
@@ -2344,6 +2366,11 @@ private:
 
     int layout_ = kLayoutL;
     long long steps_ = 0;
+    // Coarse steps taken since init / reset for the stochastic rounding's
+    // seeds (exact snapshots carry it, so a restored run repeats the
+    // captured one bit for bit).
+    unsigned long long roundSeq_ = 0;
+    bool stochastic_ = true;
     bool rampActive_ = true, rampEnabled_ = true;
     long long rampStartStep_ = 0;
     float cs_ = kSmagorinskyCs;
@@ -2402,10 +2429,13 @@ struct fc_solver {
 
 namespace fc {
 bool solverReady(fc_solver* s) { return s && s->impl.initialized(); }
-void solverStateRegions(fc_solver* s, std::vector<std::pair<void*, std::size_t>>& out, int layouts[3]) {
-    s->impl.stateRegions(out, layouts);
+void solverStateRegions(fc_solver* s, std::vector<std::pair<void*, std::size_t>>& out, int layouts[3],
+                        unsigned long long* roundSeq) {
+    s->impl.stateRegions(out, layouts, roundSeq);
 }
-int solverStateRestored(fc_solver* s, const int layouts[3]) { return s->impl.stateRestored(layouts); }
+int solverStateRestored(fc_solver* s, const int layouts[3], unsigned long long roundSeq) {
+    return s->impl.stateRestored(layouts, roundSeq);
+}
 void* solverStream(fc_solver* s) { return s->impl.stream(); }
 }  // namespace fc
 
@@ -2652,6 +2682,14 @@ int fc_last_error(fc_solver* s, char* buf, int cap) {
 int fc_set_step_variant(fc_solver* s, int variant) {
     return s ? s->impl.setStepVariant(variant) : FC_ERR_INVALID;
 }
+
+int fc_set_stochastic_rounding(fc_solver* s, int enabled) {
+    if (!s) return FC_ERR_INVALID;
+    s->impl.setStochasticRounding(enabled != 0);
+    return FC_OK;
+}
+
+int fc_get_stochastic_rounding(const fc_solver* s) { return s && s->impl.stochasticRounding() ? 1 : 0; }
 
 int fc_step(fc_solver* s, int n) { return s ? s->impl.stepN(n) : FC_ERR_INVALID; }
 int fc_synchronize(fc_solver* s) { return s ? s->impl.synchronize() : FC_ERR_INVALID; }

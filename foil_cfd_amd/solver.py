@@ -447,6 +447,14 @@ class LBMSolver:
         v = {"auto": 0, "single": 1, "buf": 2, "wide": 3}[variant]
         check(self._lib.fc_set_step_variant(self._h, v), "fc_set_step_variant")
 
+    def set_stochastic_rounding(self, on: bool):
+        """Stochastic (True, the default) or round-to-nearest rounding of the
+        16-bit population stores; no effect on FP32 / FP64 storage."""
+        check(self._lib.fc_set_stochastic_rounding(self._h, 1 if on else 0), "fc_set_stochastic_rounding")
+
+    def stochastic_rounding(self) -> bool:
+        return bool(self._lib.fc_get_stochastic_rounding(self._h))
+
     def set_force_ema_window(self, ft: float):
         check(self._lib.fc_set_force_ema_window(self._h, ft), "fc_set_force_ema_window")
 
