@@ -42,16 +42,17 @@ struct StoreF16 {
     __device__ static inline void store(T* p, C v) { *p = static_cast<_Float16>(v); }
     __device__ static inline C fromBits(Bits b) { return static_cast<float>(__builtin_bit_cast(_Float16, b)); }
     __device__ static inline Bits toBits(C v) { return __builtin_bit_cast(Bits, static_cast<_Float16>(v)); }
-    // Stochastic rounding (random bits r, the top 13 used): the 13 FP32
-    // mantissa bits a normal FP16 drops get r added and are cleared, so the
-    // conversion is exact and rounds up with probability equal to the
-    // dropped fraction. Values in FP16's subnormal range (|v| < 2^-14, i.e.
-    // |f - w| < 2^-29 after the scale) keep round-to-nearest on the rest.
-    __device__ static inline Bits toBitsSR(C v, unsigned r) {
+    // Stochastic rounding when sto is set (random bits r, the top 13 used):
+    // the 13 FP32 mantissa bits a normal FP16 drops get r added and are
+    // cleared, so the conversion is exact and rounds up with probability
+    // equal to the dropped fraction. Values in FP16's subnormal range
+    // (|v| < 2^-14, i.e. |f - w| < 2^-29 after the scale) keep round-to-
+    // nearest on the rest. Else round to nearest; one conversion either way.
+    __device__ static inline Bits toBitsR(C v, bool sto, unsigned r) {
         const unsigned b = __builtin_bit_cast(unsigned, v);
         const bool finite = (b & 0x7F800000u) != 0x7F800000u;
         const float t = __builtin_bit_cast(float, (b + (r >> 19)) & ~0x1FFFu);
-        return __builtin_bit_cast(Bits, static_cast<_Float16>(finite ? t : v));
+        return __builtin_bit_cast(Bits, static_cast<_Float16>(sto && finite ? t : v));
     }
 };
 
@@ -64,13 +65,13 @@ struct StoreBF16 {
     __device__ static inline void store(T* p, C v) { *p = static_cast<__bf16>(v); }
     __device__ static inline C fromBits(Bits b) { return static_cast<float>(__builtin_bit_cast(__bf16, b)); }
     __device__ static inline Bits toBits(C v) { return __builtin_bit_cast(Bits, static_cast<__bf16>(v)); }
-    // Stochastic rounding (random bits r, the top 16 used): r is added to
-    // the 16 FP32 bits BF16 drops, then they are cut off. Infinities and NaN
-    // keep the round-to-nearest conversion.
-    __device__ static inline Bits toBitsSR(C v, unsigned r) {
+    // Stochastic rounding when sto is set (random bits r, the top 16 used):
+    // r is added to the 16 FP32 bits BF16 drops, then they are cut off.
+    // Else, and for infinities and NaN, the round-to-nearest conversion.
+    __device__ static inline Bits toBitsR(C v, bool sto, unsigned r) {
         const unsigned b = __builtin_bit_cast(unsigned, v);
         const bool finite = (b & 0x7F800000u) != 0x7F800000u;
-        return finite ? static_cast<Bits>((b + (r >> 16)) >> 16) : toBits(v);
+        return sto && finite ? static_cast<Bits>((b + (r >> 16)) >> 16) : toBits(v);
     }
 };
 
@@ -103,15 +104,17 @@ struct StoreF16C {
         const unsigned r = b + 0x7FFu + ((b >> 12) & 1u);
         return static_cast<Bits>(((r >> 12) & 0x7FFFu) | ((r >> 16) & 0x8000u));
     }
-    // Stochastic rounding (random bits r, the top 12 used): the same packing
-    // with r added below bit 12 instead of the round-to-nearest offset, so a
-    // value rounds up with probability equal to the dropped fraction. NaN
-    // (clamped to the largest magnitude by fmin / fmax) stays saturated.
-    __device__ static inline Bits toBitsSR(C v, unsigned r) {
+    // Stochastic rounding when sto is set (random bits r, the top 12 used):
+    // the packing of toBits with r added below bit 12 instead of the
+    // round-to-nearest-even offset, so a value rounds up with probability
+    // equal to the dropped fraction. NaN (clamped to the largest magnitude
+    // by fmin / fmax) stays saturated.
+    __device__ static inline Bits toBitsR(C v, bool sto, unsigned r) {
         constexpr float lim = 0x1.FFEp0f * 0x1p-112f;  // code 0x7FFF
         const float t = __builtin_fminf(__builtin_fmaxf(v * 0x1p-112f, -lim), lim);
-        const unsigned b = __builtin_bit_cast(unsigned, t) + (r >> 20);
-        return static_cast<Bits>(((b >> 12) & 0x7FFFu) | ((b >> 16) & 0x8000u));
+        const unsigned b = __builtin_bit_cast(unsigned, t);
+        const unsigned u = b + (sto ? (r >> 20) : 0x7FFu + ((b >> 12) & 1u));
+        return static_cast<Bits>(((u >> 12) & 0x7FFFu) | ((u >> 16) & 0x8000u));
     }
     __device__ static inline C load(const T* p) { return fromBits(*p); }
     __device__ static inline void store(T* p, C v) { *p = toBits(v); }

@@ -202,15 +202,9 @@ __device__ inline bool inDomain(const KArgs& a, const RowPos& p, int ex, int ey)
 template <class S>
 __device__ inline typename S::Bits roundedBits(typename S::C v, bool sto, unsigned r) {
     if constexpr (sizeof(typename S::T) == 2)
-        return sto ? S::toBitsSR(v, r) : S::toBits(v);
+        return S::toBitsR(v, sto, r);
     else
         return S::toBits(v);
-}
-
-// A value as it reads back after a rounded store.
-template <class S>
-__device__ inline typename S::C roundTrip(typename S::C v, bool sto, unsigned r) {
-    return S::fromBits(roundedBits<S>(v, sto, r));
 }
 
 template <class S>

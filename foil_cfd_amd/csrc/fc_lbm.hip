@@ -157,7 +157,9 @@ struct PullSlot {
 // local). WM compiles in the moving-wall (Ladd) slip term on solid links.
 // M is the population accessor (PopBuf for arrays under 2 GiB, else PopPtr).
 // ---------------------------------------------------------------------------
-template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY>
+// KEEP: a fluid cell's H[q] is left holding the value as stored (rounded
+// and decoded), for callers that use the stored state afterwards.
+template <class S, template <class> class M, bool FROM_L, bool WM, bool PX, bool PY, bool KEEP = false>
 __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop, std::uint8_t k,
                                 typename S::C (&H)[kQ]) {
     using T = typename S::T;
@@ -177,21 +179,26 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
     constexpr bool kNarrow = sizeof(T) == 2;
     const bool sto = kNarrow && a.stochastic;
     const unsigned rh = kNarrow ? roundHash(c, a.roundSeed) : 0u;
-    auto storeOut = [&](auto qc, C val, bool check) {
+    // Returns the 16-bit formats' stored bits (unused otherwise).
+    auto storeOut = [&](auto qc, C val, bool check) -> typename S::Bits {
         constexpr int q = decltype(qc)::value;
         if constexpr (kNarrow) {
             const typename S::Bits b = roundedBits<S>(val, sto, roundBits(rh, q));
             if (FROM_L) {
-                if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
+                if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return b;
                 pop.stBits(q, kCx[q], kCy[q], kCz[q], b);
             } else {
                 pop.stBits(kOpp[q], 0, 0, 0, b);
             }
-        } else if (FROM_L) {
-            if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return;
-            pop.st(q, kCx[q], kCy[q], kCz[q], val);
+            return b;
         } else {
-            pop.st(kOpp[q], 0, 0, 0, val);
+            if (FROM_L) {
+                if (check && !inDomain<PX>(a, p, kCx[q], kCy[q])) return 0;
+                pop.st(q, kCx[q], kCy[q], kCz[q], val);
+            } else {
+                pop.st(kOpp[q], 0, 0, 0, val);
+            }
+            return 0;
         }
     };
 
@@ -349,8 +356,13 @@ __device__ inline void cellBody(const KArgs& a, const RowPos& p, const M<S>& pop
 
     const Macro<C> mac = collide<C>(H, K, C(a.tau0), C(a.lam), C(a.pre));
 
-    staticFor<kQ>([&](auto qc) { storeOut(qc, H[decltype(qc)::value], false); });
-    if (k & kKindFeed) {  // the outlet's copy, rounded exactly as the slot store
+    staticFor<kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const typename S::Bits b = storeOut(qc, H[q], false);
+        if constexpr (KEEP && kNarrow) H[q] = S::fromBits(b);
+    });
+    if (k & kKindFeed) {  // the outlet's copy, rounded exactly as the slot store (re-rounding a
+                          // stored value, as KEEP leaves it, returns the same bits)
         T* dstFeed = static_cast<T*>(a.feed) + newLayout * kQ * plane;
         if constexpr (kNarrow) {
 #pragma unroll
@@ -458,18 +470,6 @@ struct RecArgs {
     int rw, rh;
 };
 
-// A value as it reads back from the fine array: stored (rounded as the step
-// stored it), loaded, unscaled.
-template <class S>
-__device__ inline float storedDeviation(typename S::C v, bool sto, unsigned r) {
-    if constexpr (sizeof(typename S::T) == 2) {
-        return static_cast<float>(roundTrip<S>(v, sto, r)) / S::K;
-    } else {
-        typename S::T t;
-        S::store(&t, v);
-        return static_cast<float>(S::load(&t)) / S::K;
-    }
-}
 
 template <class S, template <class> class M, bool FROM_L, bool WM>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, true>, 8)))
@@ -495,15 +495,15 @@ stepRecordKernel(KArgs a, RecArgs rg) {
         const M<S> pop(a, p);
         k = a.kind[nb(p, 0, 0, 0)];
         loadCell<S, M, FROM_L>(a, p, pop, k, H);
-        cellBody<S, M, FROM_L, WM, false, false>(a, p, pop, k, H);
+        cellBody<S, M, FROM_L, WM, false, false, true>(a, p, pop, k, H);
     }
     if (yp < rg.cy0 || yp >= rg.cy1) return;  // workgroup-uniform: rows outside the restricted box
 
+    // H holds what the step stored (the 16-bit formats' rounded values; FP32
+    // and FP64 store H exactly), which the child-average pass would read back
     const bool fluid = live && (k & kKindFlagMask) == kFluid;
-    const bool sto = sizeof(typename S::T) == 2 && a.stochastic;
-    const unsigned rh = fluid ? roundHash(nb(p, 0, 0, 0), a.roundSeed) : 0u;
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) sH[wv][q][lane] = fluid ? storedDeviation<S>(H[q], sto, roundBits(rh, q)) : 0.0f;
+    for (int q = 0; q < kQ; ++q) sH[wv][q][lane] = fluid ? static_cast<float>(H[q]) / S::K : 0.0f;
     sLive[wv][lane] = (live && (k & kKindFlagMask) != kSolid) ? 1 : 0;
     __syncthreads();
     if (wv != 0) return;
