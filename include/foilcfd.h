@@ -217,9 +217,10 @@ int fc_set_step_variant(fc_solver* s, int variant);
  * stay deterministic, identical across step variants and group stepping,
  * and exact snapshots restore the sequence. Disabled: round to nearest even.
  * Round to nearest leaves a systematic error in steady regions (the same
- * value rounds the same way every step): on the 23.6 M-cell wing at chord
- * Re ~1000, FP16C drag was 4.5 % low against FP64 with it and within 0.05 %
- * with stochastic rounding. 32- and 64-bit storages ignore the setting. */
+ * value rounds the same way every step): on the refined 23.6 M-cell wing at
+ * chord Re ~1000, FP16C drag came out 4.5 % low against FP64 with round to
+ * nearest and within 0.05 % with stochastic rounding. 32- and 64-bit
+ * storages ignore the setting. */
 int fc_set_stochastic_rounding(fc_solver* s, int enabled);
 int fc_get_stochastic_rounding(const fc_solver* s);
 
