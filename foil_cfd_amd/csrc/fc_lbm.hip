@@ -422,8 +422,9 @@ __device__ inline void stepCell(const KArgs& a, const RowPos& p) {
 
 // Resident waves per SIMD a step kernel asks the register allocator for.
 // Seven (at most 72 VGPRs) for the 16-bit storages' wall-model steps and
-// record-writing steps: they then fit with no hot-path spills (73-107 VGPRs
-// and 4-6 waves without the request) and run faster on the refined wing. The
+// record-writing steps: they then fit (the L->S wall-model step spills two
+// dwords; 73-107 VGPRs and 4-6 waves without the request) and run faster on
+// the refined wing. The
 // other variants keep the allocator's own choice (a request for one wave is
 // the default): the same cap spills the 32-bit and plain-wall variants and
 // slows them (both measured on MI355X).
