@@ -347,6 +347,8 @@ SIGNATURES = {
     "fc_kernel_timing_level": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "fc_time_steps": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "fc_probe_aa_traffic": (C.c_int, [FcDims, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "fc_probe_store_rounding": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_int,
+                                           C.c_longlong, C.POINTER(C.c_uint16)]),
     "fc_probe_copy": (C.c_int, [C.c_size_t, C.c_int, _dp]),
 }
 

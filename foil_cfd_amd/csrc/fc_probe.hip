@@ -97,6 +97,39 @@ __global__ void __launch_bounds__(kBlock) copyKernel(const float4* __restrict__ 
     if (i < n) dst[i] = src[i];
 }
 
+// The step kernels' 16-bit store codec applied to given values and random
+// bits (the value is the codec's input: the scaled deviation K (f - w)).
+template <class S>
+__global__ void __launch_bounds__(kBlock) roundKernel(const float* __restrict__ v, const unsigned* __restrict__ r,
+                                                      int sto, long long n, unsigned short* __restrict__ out) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) out[i] = roundedBits<S>(v[i], sto != 0, r[i]);
+}
+
+template <class S>
+int runRound(const float* v, const unsigned* r, int sto, long long n, unsigned short* out) {
+    float* dv = nullptr;
+    unsigned* dr = nullptr;
+    unsigned short* dout = nullptr;
+    const std::size_t m = static_cast<std::size_t>(n);
+    int rc = FC_OK;
+    if (hipMalloc(&dv, m * 4) != hipSuccess || hipMalloc(&dr, m * 4) != hipSuccess ||
+        hipMalloc(&dout, m * 2) != hipSuccess)
+        rc = FC_ERR_ALLOC;
+    if (rc == FC_OK && (hipMemcpy(dv, v, m * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(dr, r, m * 4, hipMemcpyHostToDevice) != hipSuccess))
+        rc = FC_ERR_DEVICE;
+    if (rc == FC_OK) {
+        roundKernel<S><<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock>>>(dv, dr, sto, n, dout);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(out, dout, m * 2, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = FC_ERR_DEVICE;
+    }
+    (void)hipFree(dv);
+    (void)hipFree(dr);
+    (void)hipFree(dout);
+    return rc;
+}
+
 // Milliseconds per launch of `launch`, averaged over reps after 3 warm-up
 // launches, bracketed by HIP events on the null stream.
 template <class L>
@@ -156,6 +189,15 @@ int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int pattern, int work, int
         return FC_ERR_INVALID;
     if (elem_bytes == 2) return fc::runAA<unsigned short>(dims, pattern, work, reps, ms_per_launch);
     if (elem_bytes == 4) return fc::runAA<float>(dims, pattern, work, reps, ms_per_launch);
+    return FC_ERR_INVALID;
+}
+
+int fc_probe_store_rounding(int storage, const float* values, const unsigned* random_bits, int stochastic,
+                            long long n, unsigned short* codes) {
+    if (!values || !random_bits || !codes || n < 1 || n > (1ll << 30)) return FC_ERR_INVALID;
+    if (storage == FC_STORAGE_F16C) return fc::runRound<fc::StoreF16C>(values, random_bits, stochastic, n, codes);
+    if (storage == FC_STORAGE_F16) return fc::runRound<fc::StoreF16>(values, random_bits, stochastic, n, codes);
+    if (storage == FC_STORAGE_BF16) return fc::runRound<fc::StoreBF16>(values, random_bits, stochastic, n, codes);
     return FC_ERR_INVALID;
 }
 

@@ -709,6 +709,14 @@ int fc_time_steps(fc_solver* s, int n, int steps_per_call, double* ms);
  * copy probe times a plain device copy of `bytes`. Both free what they
  * allocate; ms per launch. */
 int fc_probe_aa_traffic(fc_dims dims, int elem_bytes, int pattern, int work, int reps, double* ms_per_launch);
+/* The step kernels' store codec of a 16-bit storage (FC_STORAGE_F16C, _F16 or
+ * _BF16) run on the device over n given values (the codec's input, the scaled
+ * deviation K (f - w)) with the given 32-bit random words: stochastic = 1
+ * rounds as fc_set_stochastic_rounding's default, 0 to nearest even. Writes
+ * the n 16-bit codes (host arrays; allocates and frees its own device
+ * buffers). For checking the codec against its restatement. */
+int fc_probe_store_rounding(int storage, const float* values, const unsigned* random_bits, int stochastic,
+                            long long n, unsigned short* codes);
 int fc_probe_copy(size_t bytes, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus

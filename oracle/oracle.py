@@ -523,6 +523,60 @@ def f16c_decode(b):
     return float(lib().orc_f16c_decode(int(b)))
 
 
+# ---- 16-bit store rounding (round to nearest even / stochastic) -------------------
+# Restated from the formats' definitions and the stochastic-rounding rule (a
+# value rounds up with probability equal to the fraction the kept bits drop):
+# the random word's top bits are added below the kept significand before it
+# is cut. Pinned by tests/test_oracle_pins.py (round to nearest equals the C
+# oracle's FP16C encoder, numpy's float16 conversion and the BF16 bit rule;
+# over every value of the random bits the mean decoded value equals the
+# input exactly) and compared bit for bit with the device codec
+# (fc_probe_store_rounding) in tests/test_gpu_parity.py.
+F16C_LIM = np.float32(np.ldexp(np.float32(1) + np.float32(2047 / 2048), -112))  # code 0x7FFF
+
+
+def store_codes(storage, v, r, stochastic):
+    """16-bit codes of FP32 values v (the scaled deviations a step stores)
+    with 32-bit random words r: storage 'f16c', 'f16' or 'bf16'."""
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    r = np.ascontiguousarray(r, dtype=np.uint32)
+    if storage == "f16c":
+        t = np.fmin(np.fmax(v * np.float32(2.0 ** -112), -F16C_LIM), F16C_LIM)
+        b = t.view(np.uint32)
+        add = (r >> np.uint32(20)) if stochastic else (np.uint32(0x7FF) + ((b >> np.uint32(12)) & np.uint32(1)))
+        u = b + add
+        return (((u >> np.uint32(12)) & np.uint32(0x7FFF)) | ((u >> np.uint32(16)) & np.uint32(0x8000))).astype(
+            np.uint16)
+    b = v.view(np.uint32)
+    finite = (b & np.uint32(0x7F800000)) != np.uint32(0x7F800000)
+    if storage == "f16":
+        if not stochastic:
+            return v.astype(np.float16).view(np.uint16)
+        t = ((b + (r >> np.uint32(19))) & np.uint32(0xFFFFE000)).view(np.float32)
+        return np.where(finite, t, v).astype(np.float16).view(np.uint16)
+    if storage == "bf16":
+        rn = ((b + np.uint32(0x7FFF) + ((b >> np.uint32(16)) & np.uint32(1))) >> np.uint32(16)).astype(np.uint16)
+        if not stochastic:
+            return np.where(finite, rn, (b >> np.uint32(16)).astype(np.uint16))
+        return np.where(finite, ((b + (r >> np.uint32(16))) >> np.uint32(16)).astype(np.uint16),
+                        (b >> np.uint32(16)).astype(np.uint16))
+    raise ValueError(storage)
+
+
+def decode_codes(storage, c):
+    """FP64 values of 16-bit codes (the codec's scaled deviations)."""
+    c = np.asarray(c, dtype=np.uint16)
+    if storage == "f16c":
+        sx = c.view(np.int16).astype(np.int32).view(np.uint32)
+        u = (sx << np.uint32(12)) & np.uint32(0x87FFF000)
+        return u.view(np.float32).astype(np.float64) * 2.0 ** 112
+    if storage == "f16":
+        return c.view(np.float16).astype(np.float64)
+    if storage == "bf16":
+        return (c.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64)
+    raise ValueError(storage)
+
+
 def wall_update(f, dims, wl, utau, nu, cutoff, ema=0.25):
     """One wall-model update (lbm_wallmodel.cu:76-196) on canonical FP32 f.
     wl: dict of SoA arrays (cell, sample, nx, ny, nz, ys, mask); utau updated

@@ -503,3 +503,36 @@ def test_stochastic_rounding_setting_is_per_solver_and_ignored_at_32_bits(gpu):
     live = ~np.isnan(a)
     assert np.count_nonzero(a[live] != b[live]) > 0
 
+
+@pytest.mark.parametrize("storage", ["f16c", "f16", "bf16"])
+def test_device_store_codec_matches_its_restatement(gpu, storage):
+    """The step kernels' 16-bit store codec, run on the device through
+    fc_probe_store_rounding, against the oracle's restatement bit for bit:
+    random values across the format's range plus zeros, subnormals,
+    saturating / overflowing values and infinities (and NaN for FP16C, which
+    saturates it), with random 32-bit words, in both rounding modes."""
+    import ctypes as C
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    lo, hi = {"f16c": (-30, 2), "f16": (-30, 17), "bf16": (-140, 60)}[storage]
+    v = (np.where(rng.random(n) < 0.5, -1.0, 1.0) * np.exp2(rng.uniform(lo, hi, n))).astype(np.float32)
+    special = [0.0, -0.0, np.inf, -np.inf, 1e-40, -1e-42, 3.0, 70000.0, -1e30]
+    if storage == "f16c":
+        special.append(np.nan)
+    v[:len(special)] = np.float32(special)
+    r = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    from foil_cfd_amd import _abi
+    code = _abi.STORAGE_NAMES[storage]
+    lib = fc.load()
+    for sto in (0, 1):
+        out = np.zeros(n, np.uint16)
+        rc = lib.fc_probe_store_rounding(code, v.ctypes.data_as(C.POINTER(C.c_float)),
+                                         r.ctypes.data_as(C.POINTER(C.c_uint32)), sto, n,
+                                         out.ctypes.data_as(C.POINTER(C.c_uint16)))
+        assert rc == 0
+        with np.errstate(over="ignore"):
+            want = orc.store_codes(storage, v, r, bool(sto))
+        bad = np.nonzero(out != want)[0]
+        say(f"  {storage} {'stochastic' if sto else 'nearest'}: {bad.size} of {n} codes differ")
+        assert bad.size == 0, (v[bad[:5]], out[bad[:5]], want[bad[:5]])
+
