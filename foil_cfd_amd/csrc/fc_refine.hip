@@ -331,7 +331,11 @@ __global__ void __launch_bounds__(kBlock) fillKernel(const void* cf, const std::
 // cached finish (cacheMode 2) then writes the fine populations.
 // Groups g = (kz * 4 + lane kind) ...: per z pair, 4 rows of w + 1 x pairs
 // (x = 2kx - 1 + d), then 4 side columns of h - 1 y pairs (y = 2ky - 1 + d,
-// ky = 1 .. h - 1, members kept when 2 <= y < fny - 2).
+// ky = 1 .. h - 1, members kept when 2 <= y < fny - 2). The launch has
+// kInterpRows grid rows, each doing the gathers of a share of the 19
+// populations.
+constexpr unsigned kInterpRows = 4;
+
 template <class S>
 __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, const std::uint8_t* ckind, int cLayout,
                                                             const float* band, const int* bandIndex, int nband,
@@ -403,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
         const int xf = mx[d], yf = my[d], zf = mz[e];
         const long long tm = static_cast<long long>(zf) * ring + off[d];
         if ((fkind[cellOf(xf, yf, zf, r.fnx, r.fny)] & kKindFlagMask) != kInterface) {
-            cache[38 * ns + tm] = 0.0f;
+            if (blockIdx.y == 0) cache[38 * ns + tm] = 0.0f;
             continue;
         }
         const float gx = static_cast<float>(r.x0) + (static_cast<float>(xf) + 0.5f) * invM - 0.5f;
@@ -422,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
             wsum += wv[c];
         }
         if (wsum < 1e-6f) {
-            cache[38 * ns + tm] = 0.0f;
+            if (blockIdx.y == 0) cache[38 * ns + tm] = 0.0f;
             continue;
         }
         const float invW = 1.0f / wsum;
@@ -431,7 +435,11 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
         t[m] = tm;
     }
     if (t[0] < 0 && t[1] < 0 && t[2] < 0 && t[3] < 0) return;
-    for (int q = 0; q < kQ; ++q) {
+    // grid row y takes the populations [q0, q1): the groups' set-up repeats
+    // per row, the gathers and sums split, so more waves hide the latency
+    const int q0 = static_cast<int>(kQ * blockIdx.y / gridDim.y);
+    const int q1 = static_cast<int>(kQ * (blockIdx.y + 1) / gridDim.y);
+    for (int q = q0; q < q1; ++q) {
         float v0[8], v1[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -454,9 +462,11 @@ __global__ void __launch_bounds__(kBlock) shellInterp2Kernel(const void* cf, con
             cache[(2 * q + 1) * ns + t[m]] = s1;
         }
     }
+    if (blockIdx.y == 0) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-        if (t[m] >= 0) cache[38 * ns + t[m]] = 1.0f;
+        for (int m = 0; m < 4; ++m)
+            if (t[m] >= 0) cache[38 * ns + t[m]] = 1.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -910,7 +920,8 @@ struct RefineOps {
             if (cache && cacheMode == 1 && r.m == 2) {
                 // grouped interpolation into the cache, then the cached finish
                 const long long groups = 4LL * ((r.x1 - r.x0) + (r.y1 - r.y0)) * r.cnz;
-                shellInterp2Kernel<S><<<xcdGrid((groups + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                const dim3 grid(xcdGrid((groups + kBlock - 1) / kBlock), kInterpRows);
+                shellInterp2Kernel<S><<<grid, kBlock, 0, st>>>(
                     c.f, c.kind, cLayout, band, bandIndex, nband, f.kind, r, cache, groups);
                 cacheMode = 2;
             }
